@@ -1,0 +1,155 @@
+/*
+ * tnr.h - C ABI of libtnr_hip.so, the MI355X (gfx950) engine for the
+ * TRG / BTRG / HOTRG coarse-graining loop of TNRKit.jl v0.5.1.
+ *
+ * Every entry point returns TNR_OK (0) or a negative status; no C++
+ * exception crosses this boundary.  Host pointers are borrowed for the
+ * duration of the call only.  A handle is not thread-safe; distinct handles
+ * may be driven from different threads (each owns its own HIP stream).
+ *
+ * Tensor data format.  A scheme tensor T lives in V1 (x) V2 <- V3 (x) V4 with
+ * legs (left, bottom, top, right) (docs/src/index.md:107-119).  With
+ * Z_N symmetry (zn = N >= 2; zn = 1 is Trivial) every space is a direct sum
+ * of sectors a = 0..N-1 and
+ *     leg_dims[i*zn + a] = dimension of sector a in V_{i+1},  i = 0..3
+ * (V3, V4 as written in the domain).  `data` is the concatenation, for
+ * coupled charge c = 0..N-1, of the column-major block matrix of T at c.
+ * Block rows run over codomain sector pairs (a1, a2) with a1 + a2 = c mod N
+ * in colexicographic order (a1 fastest), each pair contributing
+ * d1(a1)*d2(a2) rows (index of leg 1 fastest); columns likewise over
+ * (a3, a4) with a3 + a4 = c.  For zn = 1 this is exactly the column-major
+ * dense array T[i1, i2, i3, i4] (TensorKit's `t.data` for a Trivial map).
+ */
+#ifndef TNR_H
+#define TNR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define TNR_OK              0
+#define TNR_ERR_ARG        -1  /* invalid argument (dims, chi, sizes, kind) */
+#define TNR_ERR_HIP        -2  /* HIP runtime error */
+#define TNR_ERR_NOMEM      -3  /* device allocation failed */
+#define TNR_ERR_NONFINITE  -4  /* a norm came out NaN/Inf/zero */
+#define TNR_ERR_NOCONV     -5  /* an eigen/singular solver did not converge */
+#define TNR_ERR_STATE      -6  /* call not valid in the handle's state */
+#define TNR_ERR_NODEVICE   -7  /* no usable gfx950 device */
+#define TNR_ERR_INTERNAL   -8  /* internal consistency check failed */
+#define TNR_ERR_BUDGET     -9  /* request exceeds the admission budget */
+#define TNR_ERR_BUSY      -10  /* result not ready yet (tnr_server_poll) */
+
+/* ---- scheme kinds and precision -------------------------------------- */
+#define TNR_TRG    0   /* src/schemes/trg.jl:26-41   */
+#define TNR_BTRG   1   /* src/schemes/btrg.jl:26-94  */
+#define TNR_HOTRG  2   /* src/schemes/hotrg.jl:27-160 */
+
+#define TNR_PREC_FP64     0  /* all contractions FP64 MFMA */
+#define TNR_PREC_SPLIT32  1  /* contractions on FP32 MFMA, hi/lo split
+                                products; truncation SVD/eigh, log-norms
+                                and finalize stay FP64 */
+
+typedef struct tnr_scheme tnr_scheme;
+
+/* Replaces the constructors TRG(T) (src/schemes/trg.jl:30-32),
+ * BTRG(T, k) (src/schemes/btrg.jl:39-48; k = -0.5 is the reference default)
+ * and HOTRG(T) (src/schemes/hotrg.jl:31-33).  Copies T to HBM.  `k` is read
+ * for BTRG only.  Legs 1/4 and 2/3 must carry the same spaces (the finalize
+ * trace connects them). */
+int tnr_create(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims,
+               const double* data, size_t ndata, double k, int precision);
+
+/* step!(scheme, truncrank(chi)): src/schemes/trg.jl:35-41,
+ * src/schemes/btrg.jl:60-94, src/schemes/hotrg.jl:154-160. */
+int tnr_step(tnr_scheme* s, int chi);
+
+/* finalize!(scheme): src/utility/finalize.jl:4-8 (TRG, HOTRG) and :10-14
+ * (BTRG).  Writes the norm n and divides T by n on the device. */
+int tnr_finalize(tnr_scheme* s, double* norm);
+
+/* run!(scheme, truncrank(chi), maxiter(maxiter); finalize_beginning):
+ * src/schemes/tnrscheme.jl:33-59 with src/utility/stopping.jl:3-5,18.
+ * `norms` must hold maxiter + 1 doubles; *n_out receives the count written
+ * (maxiter + finalize_beginning). */
+int tnr_run(tnr_scheme* s, int chi, int maxiter, int finalize_beginning,
+            double* norms, int* n_out);
+
+/* free_energy(data, beta; scalefactor, initial_size):
+ * src/utility/free_energy.jl:13-20. */
+int tnr_free_energy(const double* norms, int n, double beta,
+                    double scalefactor, double initial_size, double* f_out);
+
+/* Current sector dimensions (4*zn int32) and element count of T. */
+int tnr_get_dims(const tnr_scheme* s, int32_t* leg_dims, size_t* ndata);
+/* Copy T out (format above) / replace T (checkpoint, resume, debugging). */
+int tnr_get_tensor(tnr_scheme* s, double* data, size_t ndata);
+int tnr_set_tensor(tnr_scheme* s, const int32_t* leg_dims,
+                   const double* data, size_t ndata);
+/* BTRG bond weights: which = 1 -> S1 (vertical), 2 -> S2 (horizontal);
+ * diagonal entries ordered by sector, then index (src/schemes/btrg.jl:33-36). */
+int tnr_get_bond_weights(tnr_scheme* s, int which, double* w, size_t n);
+
+/* Per-factorization diagnostics of the last step: smallest kept and largest
+ * dropped singular value (or |eigenvalue|) of each truncation, and the
+ * solver iteration count, 4 doubles per factorization. */
+int tnr_get_trunc_log(tnr_scheme* s, double* out, int max_entries, int* n_entries);
+
+int tnr_destroy(tnr_scheme* s);
+const char* tnr_strerror(int status);
+/* Thread-local message of the last failing call. */
+const char* tnr_last_error(void);
+
+/* ---- batched runs ------------------------------------------------------
+ * Many independent run! calls (beta scans, mixed chi, mixed models) driven
+ * together: every coarse-graining step of every run in the batch is issued
+ * as grouped launches (one grouped GEMM / permute / solver launch per phase
+ * covering all runs and all symmetry blocks).  Results equal the one-run
+ * path within the documented tolerance. */
+typedef struct {
+    int kind;                 /* TNR_TRG / TNR_BTRG / TNR_HOTRG */
+    int zn;                   /* 1 = Trivial, N = Z_N            */
+    const int32_t* leg_dims;  /* 4*zn                            */
+    const double* data;       /* initial tensor                  */
+    size_t ndata;
+    double k;                 /* BTRG exponent                   */
+    int chi;                  /* truncrank(chi)                  */
+    int maxiter;              /* maxiter(n)                      */
+    int finalize_beginning;
+} tnr_run_desc;
+
+/* norms: concatenation over runs of (maxiter + finalize_beginning) doubles. */
+int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision,
+                  double* norms);
+
+/* ---- serving loop --------------------------------------------------------
+ * Continuous batching: requests are admitted as HBM and slots free up
+ * (estimate ~ chi^4 x sectors per scheme, checked against hbm_budget_bytes),
+ * each admitted scheme stays resident on the device across its iterations,
+ * and all resident schemes advance together one grouped step at a time. */
+typedef struct tnr_server tnr_server;
+
+int tnr_server_create(tnr_server** out, size_t hbm_budget_bytes, int max_slots,
+                      int precision);
+/* Queue one run; *id receives a ticket.  Fails with TNR_ERR_BUDGET if the
+ * request alone exceeds the budget. */
+int tnr_server_submit(tnr_server* s, const tnr_run_desc* req, int64_t* id);
+/* Advance every resident scheme by one coarse-graining step, admitting
+ * queued requests first.  *active receives the number still in flight. */
+int tnr_server_pump(tnr_server* s, int* active);
+/* Fetch a finished result: TNR_ERR_BUSY if not finished.  norms must hold
+ * maxiter + finalize_beginning doubles; t_admit/t_done are seconds since
+ * server creation. */
+int tnr_server_poll(tnr_server* s, int64_t id, double* norms, int* n_out,
+                    double* t_submit, double* t_done);
+int tnr_server_stats(tnr_server* s, size_t* hbm_in_use, int* resident,
+                     int* queued);
+int tnr_server_destroy(tnr_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNR_H */

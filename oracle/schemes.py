@@ -1,0 +1,224 @@
+"""TRG / BTRG / HOTRG coarse-graining, finalize, run loop, free energy (oracle).
+
+Restates, function by function:
+
+* ``TRG.step``   - src/schemes/trg.jl:35-41 (with ``svd12`` =
+  src/utility/projectors.jl:211-217)
+* ``BTRG``       - src/schemes/btrg.jl:26-94 (``pseudopow`` :50-58)
+* ``HOTRG``      - src/schemes/hotrg.jl:42-160
+* ``finalize``   - src/utility/finalize.jl:1-14
+* ``run``        - src/schemes/tnrscheme.jl:33-59 with ``maxiter``
+  (src/utility/stopping.jl:3-5,18)
+* ``free_energy``- src/utility/free_energy.jl:13-20
+
+Contractions are pairwise in a cost-optimal order (TensorOperations'
+order only changes roundoff).  Each factorization appends its truncation
+diagnostics (smallest kept / largest dropped value) to ``scheme.log``.
+
+Pins (tests/test_oracle_schemes.py): the published BTRG free energy
+(README.md:87), the reference's Onsager tolerances (test/schemes.jl:18-23,
+57-62, 94-99), exact small-torus partition functions for the un-truncated
+first steps of all three schemes on random tensors (oracle.torus), Z2 vs
+Trivial Ising equality, Z_N vs Trivial Potts/clock equality.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .ztensor import ZT, contract, permute, svd_trunc, eigh_trunc, \
+    project_hermitian, scale_leg
+
+EPS = np.finfo(np.float64).eps
+
+
+def _diag(s, q_cod, N) -> ZT:
+    """Dense (1,1) tensor diag(s) on a bond whose codomain leg has charges q."""
+    return ZT(np.diag(s), [q_cod, (-q_cod) % N], N)
+
+
+def svd12(T: ZT, chi: int, log=None):
+    """src/utility/projectors.jl:211-217: (U*sqrt(s), sqrt(s)*V)."""
+    U, s, Vt, info = svd_trunc(T, 2, chi)
+    if log is not None:
+        log.append(info)
+    sq = np.sqrt(s)
+    return scale_leg(U, 2, sq), scale_leg(Vt, 0, sq)
+
+
+def pseudopow(s, a, tol=EPS ** 0.75):
+    """src/schemes/btrg.jl:50-58: s**a where s >= tol, else s unchanged."""
+    s = np.asarray(s, dtype=np.float64)
+    out = s.copy()
+    m = ~(s < tol)
+    out[m] = s[m] ** a
+    return out
+
+
+class TRG:
+    """src/schemes/trg.jl:26-41"""
+    scalefactor = 2.0
+
+    def __init__(self, T: ZT):
+        self.T = T.copy()
+        self.log = []
+
+    def step(self, chi):
+        T = self.T
+        A, B = svd12(T, chi, self.log)
+        Tp = permute(T, (1, 3, 0, 2))            # transpose(T, ((2,4),(1,3)))
+        C, D = svd12(Tp, chi, self.log)
+        # T[-1 -2; -3 -4] := D[-2; 1 2] * B[-1; 4 1] * C[4 3; -3] * A[3 2; -4]
+        X = contract([-1, -2, -3, -4], (D, [-2, 1, -3], False), (B, [-1, -4, 1], False))  # X[p,q,2,4]
+        Y = contract([-1, -2, -3, -4], (C, [-1, 3, -3], False), (A, [3, -2, -4], False))  # Y[4,2,r,s]
+        self.T = contract([-1, -2, -3, -4], (X, [-1, -2, 2, 4], False), (Y, [4, 2, -3, -4], False))
+        return self
+
+    def finalize(self):
+        """src/utility/finalize.jl:4-8: n = |tr T[1 2; 2 1]|, T /= n."""
+        n = abs(float(np.einsum("abba->", self.T.data)))
+        self.T.data /= n
+        return n
+
+
+class HOTRG(TRG):
+    """src/schemes/hotrg.jl:27-160"""
+    scalefactor = 4.0
+
+    def step(self, chi):
+        Ux, _ = get_hotrg_xproj(self.T, self.T, chi, self.log)
+        self.T = step_hotrg_y(self.T, self.T, Ux)
+        Uy, _ = get_hotrg_yproj(self.T, self.T, chi, self.log)
+        self.T = step_hotrg_x(self.T, self.T, Uy)
+        return self
+
+
+class BTRG:
+    """src/schemes/btrg.jl:26-94 (S1 on vertical bonds, S2 on horizontal)."""
+    scalefactor = 2.0
+
+    def __init__(self, T: ZT, k=-0.5):
+        self.T = T.copy()
+        self.k = float(k)
+        N = T.N
+        self.S1 = _diag(np.ones(T.shape[1]), T.q[1], N)   # id(space(T, 2))
+        self.S2 = _diag(np.ones(T.shape[0]), T.q[0], N)   # id(space(T, 1))
+        self.log = []
+
+    def step(self, chi):
+        T, k = self.T, self.k
+        U, s, V, info = svd_trunc(T, 2, chi)
+        self.log.append(info)
+        sa, sb = pseudopow(s, (1 - k) / 2), pseudopow(s, k)
+        A = scale_leg(U, 2, sa)                  # A[-1 -2; -3] := U * S_a
+        B = scale_leg(V, 0, sa)                  # B[-1; -2 -3] := S_a * V
+        S1n = _diag(sb, V.q[0], T.N)             # S1'[-1; -2] := S_b
+        U, s, V, info = svd_trunc(permute(T, (2, 0, 3, 1)), 2, chi)
+        self.log.append(info)
+        sa, sb = pseudopow(s, (1 - k) / 2), pseudopow(s, k)
+        C = scale_leg(U, 2, sa)
+        D = scale_leg(V, 0, sa)
+        S2n = _diag(sb, V.q[0], T.N)
+        S1, S2 = self.S1, self.S2
+        # T[-1 -2; -3 -4] := D[-1; 4 7] * S1[1; 7] * B[-2; 1 3] * S2[3; 2]
+        #                    * C[8 2; -4] * S1[8; 5] * A[6 5; -3] * S2[4; 6]
+        # bond matrices are absorbed pairwise (same network, cost-optimal order)
+        D1 = contract([-1, -2, -3], (D, [-1, -2, 7], False), (S1, [-3, 7], False))   # D[-1;4 7]S1[1;7]
+        B1 = contract([-1, -2, -3], (B, [-1, -2, 3], False), (S2, [3, -3], False))   # B[-2;1 3]S2[3;2]
+        C1 = contract([-1, -2, -3], (C, [8, -2, -3], False), (S1, [8, -1], False))   # C[8 2;-4]S1[8;5]
+        A1 = contract([-1, -2, -3], (A, [6, -2, -3], False), (S2, [-1, 6], False))   # A[6 5;-3]S2[4;6]
+        # D1[p, 4, 1] B1[q, 1, 2] -> X[p, q, 4, 2];  C1[5, 2, s] A1[4, 5, r] -> Y[2, 4, r, s]
+        X = contract([-1, -2, -3, -4], (D1, [-1, -3, 1], False), (B1, [-2, 1, -4], False))
+        Y = contract([-1, -2, -3, -4], (C1, [5, -1, -4], False), (A1, [-2, 5, -3], False))
+        self.T = contract([-1, -2, -3, -4], (X, [-1, -2, 4, 2], False), (Y, [2, 4, -3, -4], False))
+        self.S1, self.S2 = S1n, S2n
+        return self
+
+    def finalize(self):
+        """src/utility/finalize.jl:10-14: n = |T[1 2; 4 3] S1[4; 2] S2[3; 1]|."""
+        n = abs(float(np.einsum("abdc,db,ca->", self.T.data, self.S1.data, self.S2.data)))
+        self.T.data /= n
+        return n
+
+
+# ---------------------------------------------------------------- HOTRG parts
+
+def _pick(A, B, chi, log):
+    MM = project_hermitian(A, 2)
+    _, U, eps, info = eigh_trunc(MM, 2, chi)
+    MM2 = project_hermitian(B, 2)
+    _, U2, eps2, info2 = eigh_trunc(MM2, 2, chi)
+    if log is not None:
+        log.append(info)
+        log.append(info2)
+    if eps > eps2:                                  # src/schemes/hotrg.jl:112,148
+        return U2, eps2
+    return U, eps
+
+
+def get_hotrg_xproj(A1: ZT, A2: ZT, chi, log=None):
+    """src/schemes/hotrg.jl:82-116"""
+    # MM[-1 -2; -3 -4] := A2[-1 5; 1 2] A1[-2 3; 5 4] conj(A2[-3 6; 1 2]) conj(A1[-4 3; 6 4])
+    E2 = contract([-1, -2, -3, -4], (A2, [-1, -2, 1, 2], False), (A2, [-3, -4, 1, 2], True))
+    E1 = contract([-1, -2, -3, -4], (A1, [-1, 3, -2, 4], False), (A1, [-3, 3, -4, 4], True))
+    MM = contract([-1, -2, -3, -4], (E2, [-1, 5, -3, 6], False), (E1, [-2, 5, -4, 6], False))
+    # MM[-1 -2; -3 -4] := conj(A2[2 5; 1 -1]) conj(A1[4 3; 5 -2]) A2[2 6; 1 -3] A1[4 3; 6 -4]
+    F2 = contract([-1, -2, -3, -4], (A2, [2, -1, 1, -2], True), (A2, [2, -3, 1, -4], False))
+    F1 = contract([-1, -2, -3, -4], (A1, [4, 3, -1, -2], True), (A1, [4, 3, -3, -4], False))
+    MMp = contract([-1, -2, -3, -4], (F2, [5, -1, 6, -3], False), (F1, [5, -2, 6, -4], False))
+    return _pick(MM, MMp, chi, log)
+
+
+def get_hotrg_yproj(A1: ZT, A2: ZT, chi, log=None):
+    """src/schemes/hotrg.jl:118-152"""
+    # MM[-1 -2; -3 -4] := A1[1 -1; 2 5] A2[5 -2; 4 3] conj(A1[1 -3; 2 6]) conj(A2[6 -4; 4 3])
+    G1 = contract([-1, -2, -3, -4], (A1, [1, -1, 2, -2], False), (A1, [1, -3, 2, -4], True))
+    G2 = contract([-1, -2, -3, -4], (A2, [-1, -2, 4, 3], False), (A2, [-3, -4, 4, 3], True))
+    MM = contract([-1, -2, -3, -4], (G1, [-1, 5, -3, 6], False), (G2, [5, -2, 6, -4], False))
+    # MM[-1 -2; -3 -4] := conj(A1[1 2; -1 5]) conj(A2[5 4; -2 3]) A1[1 2; -3 6] A2[6 4; -4 3]
+    H1 = contract([-1, -2, -3, -4], (A1, [1, 2, -1, -2], True), (A1, [1, 2, -3, -4], False))
+    H2 = contract([-1, -2, -3, -4], (A2, [-1, 4, -2, 3], True), (A2, [-3, 4, -4, 3], False))
+    MMp = contract([-1, -2, -3, -4], (H1, [-1, 5, -3, 6], False), (H2, [5, -2, 6, -4], False))
+    return _pick(MM, MMp, chi, log)
+
+
+def step_hotrg_y(A1: ZT, A2: ZT, Ux: ZT) -> ZT:
+    """src/schemes/hotrg.jl:42-60:
+    T[-1 -2; -3 -4] := conj(Ux[1 2; -1]) Ux[3 4; -4] A2[1 5; -3 3] A1[2 -2; 5 4]"""
+    P = contract([-1, -2, -3, -4, -5], (Ux, [1, -1, -2], True), (A2, [1, -3, -4, -5], False))
+    Q = contract([-1, -2, -3, -4, -5], (P, [2, -1, 5, -2, -3], False), (A1, [2, -4, 5, -5], False))
+    return contract([-1, -2, -3, -4], (Q, [-1, -3, 3, -2, 4], False), (Ux, [3, 4, -4], False))
+
+
+def step_hotrg_x(A1: ZT, A2: ZT, Uy: ZT) -> ZT:
+    """src/schemes/hotrg.jl:62-80:
+    T[-1 -2; -3 -4] := A1[-1 1; 3 5] A2[5 2; 4 -4] conj(Uy[1 2; -2]) Uy[3 4; -3]"""
+    R = contract([-1, -2, -3, -4, -5], (A1, [-1, 1, -2, -3], False), (Uy, [1, -4, -5], True))
+    S = contract([-1, -2, -3, -4, -5], (R, [-1, -2, 5, 2, -3], False), (A2, [5, 2, -4, -5], False))
+    return contract([-1, -2, -3, -4], (S, [-1, 3, -2, 4, -4], False), (Uy, [3, 4, -3], False))
+
+
+# ---------------------------------------------------------------- driver
+
+def run(scheme, chi, maxiter, finalize_beginning=True):
+    """src/schemes/tnrscheme.jl:33-57 with ``maxiter(n)`` (stopping.jl:18):
+    returns the list of norms (maxiter + 1 entries by default)."""
+    data = []
+    if finalize_beginning:
+        data.append(scheme.finalize())
+    steps = 0
+    crit = True
+    while crit:
+        scheme.step(chi)
+        data.append(scheme.finalize())
+        steps += 1
+        crit = steps < maxiter
+    return data
+
+
+def free_energy(data, beta, scalefactor=2.0, initial_size=1.0):
+    """src/utility/free_energy.jl:13-20"""
+    lnz = 0.0
+    x = 1.0 - np.log(initial_size) / np.log(scalefactor)
+    for i, z in enumerate(data, start=1):
+        lnz += np.log(z) * scalefactor ** (x - i)
+    return -lnz / beta

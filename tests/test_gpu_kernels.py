@@ -1,0 +1,136 @@
+"""Kernel-level parity on the GPU: grouped FP64 MFMA GEMM, FP32 split GEMM,
+grouped Jacobi eigensolver, truncated SVD and eigensolver, each against
+numpy/LAPACK FP64 on seeded inputs.  Tolerances are stated per test."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+D = C.POINTER(C.c_double)
+
+
+def p(a):
+    return a.ctypes.data_as(D)
+
+
+def gemm(tk, A, B, Cm, ta, tb, alpha, beta, split=0):
+    M, N = Cm.shape
+    K = A.shape[0] if ta else A.shape[1]
+    A = np.asfortranarray(A); B = np.asfortranarray(B); Co = np.asfortranarray(Cm.copy())
+    st = tk.lib().tnr_kernel_gemm(M, N, K, p(A), max(1, A.shape[0]), ta, p(B), max(1, B.shape[0]), tb, p(Co),
+                                  M, alpha, beta, split, 1, None)
+    assert st == 0, tk.lib().tnr_last_error()
+    return Co
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [
+    (1, 1, 1, 0, 0), (64, 64, 16, 0, 0), (100, 37, 53, 0, 0), (100, 37, 53, 1, 0),
+    (100, 37, 53, 0, 1), (100, 37, 53, 1, 1), (1024, 64, 1024, 0, 0), (64, 64, 4096, 1, 0),
+    (300, 200, 0, 0, 0), (513, 65, 129, 1, 1)])
+def test_gemm_f64(tk, M, N, K, ta, tb):
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    Cm = rng.standard_normal((M, N))
+    got = gemm(tk, A, B, Cm, ta, tb, 1.3, 0.7)
+    ref = 1.3 * ((A.T if ta else A) @ (B.T if tb else B)) + 0.7 * Cm
+    scale = 1.3 * np.sqrt(max(K, 1)) + 1
+    assert np.max(np.abs(got - ref)) <= 1e-14 * scale * 8
+
+
+@pytest.mark.parametrize("M,N,K", [(128, 128, 256), (200, 100, 1000)])
+def test_gemm_split32(tk, M, N, K):
+    rng = np.random.default_rng(K)
+    A = rng.standard_normal((M, K)); B = rng.standard_normal((K, N)); Cm = np.zeros((M, N))
+    got = gemm(tk, A, B, Cm, 0, 0, 1.0, 0.0, split=1)
+    ref = A @ B
+    # hi/lo split products accumulated on FP32 MFMA within 16-deep k-tiles:
+    # error ~ 2^-24 * sqrt(K) * |a||b| per entry; stated bound 1e-6 relative
+    # to the entry scale sqrt(K)
+    assert np.max(np.abs(got - ref)) / np.sqrt(K) < 1e-6
+
+
+def jacobi(tk, G):
+    cnt, n = G.shape[0], G.shape[1]
+    Gf = np.ascontiguousarray(np.stack([np.asfortranarray(g).ravel(order="F") for g in G]))
+    W = np.zeros((cnt, n * n)); lam = np.zeros((cnt, n)); sw = C.c_int()
+    st = tk.lib().tnr_kernel_jacobi(cnt, n, p(Gf), p(W), p(lam), C.byref(sw))
+    assert st == 0, tk.lib().tnr_last_error()
+    return lam, np.stack([w.reshape(n, n, order="F") for w in W]), sw.value
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 64, 95, 96])
+def test_jacobi_random_symmetric(tk, n):
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((3, n, n)); G = G + G.transpose(0, 2, 1)
+    lam, W, sw = jacobi(tk, G)
+    for g, l, w in zip(G, lam, W):
+        ref = np.linalg.eigvalsh(g)
+        ref = ref[np.argsort(-np.abs(ref), kind="stable")]
+        s = np.max(np.abs(ref))
+        assert np.max(np.abs(l - ref)) < 1e-13 * s * n
+        assert np.max(np.abs(g @ w - w * l)) < 1e-13 * s * n
+        assert np.max(np.abs(w.T @ w - np.eye(n))) < 1e-13 * n
+
+
+def test_jacobi_graded_gram_relative_accuracy(tk):
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((200, 64)) * np.logspace(0, -12, 64)
+    G = (A.T @ A)[None]
+    lam, W, sw = jacobi(tk, G)
+    ref = np.linalg.svd(A, compute_uv=False) ** 2
+    # relative accuracy of every eigenvalue, down to 1e-24 of the largest
+    assert np.max(np.abs(lam[0] / ref - 1)) < 1e-12
+
+
+def svd(tk, M, k):
+    m, n = M.shape
+    Mf = np.asfortranarray(M)
+    U = np.zeros((m, k), order="F"); s = np.zeros(k); V = np.zeros((n, k), order="F")
+    it = C.c_int()
+    st = tk.lib().tnr_kernel_svd(m, n, p(Mf), k, p(U), p(s), p(V), C.byref(it))
+    assert st == 0, tk.lib().tnr_last_error()
+    return U, s, V, it.value
+
+
+def decaying(rng, m, n, rate):
+    U, _ = np.linalg.qr(rng.standard_normal((m, min(m, n))))
+    V, _ = np.linalg.qr(rng.standard_normal((n, min(m, n))))
+    s = np.exp(-rate * np.arange(min(m, n)))
+    return (U * s) @ V.T
+
+
+@pytest.mark.parametrize("m,n,k,rate", [
+    (4, 4, 4, 0.5), (16, 16, 16, 0.3), (50, 200, 20, 0.1), (200, 50, 20, 0.1),
+    (256, 256, 16, 0.05), (1024, 1024, 32, 0.02), (300, 500, 48, 0.05)])
+def test_svd_truncated(tk, m, n, k, rate):
+    rng = np.random.default_rng(m + n + k)
+    M = decaying(rng, m, n, rate)
+    U, s, V, it = svd(tk, M, k)
+    Ur, sr, Vtr = np.linalg.svd(M, full_matrices=False)
+    assert np.max(np.abs(s - sr[:k])) < 1e-13 * sr[0]
+    # truncated reconstruction equals LAPACK's (the cut has a gap here)
+    Mk = (U * s) @ V.T
+    Mr = (Ur[:, :k] * sr[:k]) @ Vtr[:k]
+    assert np.max(np.abs(Mk - Mr)) < 1e-12 * sr[0]
+    assert np.max(np.abs(U.T @ U - np.eye(k))) < 1e-12
+    assert np.max(np.abs(V.T @ V - np.eye(k))) < 1e-12
+
+
+@pytest.mark.parametrize("n,k", [(64, 16), (512, 32)])
+def test_eigh_truncated(tk, n, k):
+    rng = np.random.default_rng(n)
+    X = decaying(rng, n, n, 0.04)
+    M = X @ X.T
+    Mf = np.asfortranarray(M)
+    U = np.zeros((n, k), order="F"); lam = np.zeros(k); it = C.c_int()
+    st = tk.lib().tnr_kernel_eigh(n, p(Mf), k, p(U), p(lam), C.byref(it))
+    assert st == 0, tk.lib().tnr_last_error()
+    w, v = np.linalg.eigh(M)
+    o = np.argsort(-np.abs(w)); w, v = w[o], v[:, o]
+    assert np.max(np.abs(lam - w[:k])) < 1e-13 * w[0]
+    Pk = U @ U.T
+    Pr = v[:, :k] @ v[:, :k].T
+    assert np.max(np.abs(Pk - Pr)) < 1e-9

@@ -1,0 +1,103 @@
+"""Scheme-level parity on the GPU through the C ABI: per-step norms and free
+energies against the FP64 oracle (relative 1e-10 on the plain FP64 path),
+the published BTRG free energy, exact small tori for random tensors, and
+batch / serving results equal to the one-run results."""
+import numpy as np
+import pytest
+
+from oracle import models as om
+from oracle import schemes as osch
+from oracle import torus
+from oracle.ztensor import ZT
+from conftest import rel, to_sym
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def gpu_run(tk, cls, T, chi, n, **kw):
+    sc = cls(T, **kw)
+    out = np.array(tk.run(sc, tk.truncrank(chi), tk.maxiter(n), verbosity=0))
+    sc.close()
+    return out
+
+
+def test_readme_btrg_z2_chi16(tk):
+    """README.md:78-87: f = -2.1096504926141826902647832 (26 norms)."""
+    d = gpu_run(tk, tk.BTRG, tk.classical_ising(symmetry="Z2"), 16, 25)
+    assert len(d) == 26
+    ref = np.array(osch.run(osch.BTRG(om.classical_ising_z2(om.ising_betac_mp)), 16, 25))
+    assert rel(d, ref) < TOL
+    f = tk.free_energy(d, tk.ising_betac)
+    assert abs(f / -2.1096504926141826902647832 - 1) < 1e-12
+
+
+@pytest.mark.parametrize("name,chi,n", [("TRG", 16, 20), ("BTRG", 24, 20), ("HOTRG", 8, 12)])
+@pytest.mark.parametrize("sym", ["Trivial", "Z2"])
+def test_ising_schemes_vs_oracle(tk, name, chi, n, sym):
+    T = tk.classical_ising(symmetry=sym)
+    zt = (om.classical_ising_z2 if sym == "Z2" else om.classical_ising_trivial)(om.ising_betac_mp)
+    d = gpu_run(tk, getattr(tk, name), T, chi, n)
+    ref = np.array(osch.run(getattr(osch, name)(zt), chi, n))
+    assert rel(d, ref) < TOL
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_exact_first_steps_random_tensor(tk, seed):
+    rng = np.random.default_rng(seed)
+    T = rng.standard_normal((2, 2, 2, 2)) + 1.0
+    z1, zt2, z22, z44 = (torus.z_1site(T), torus.z_tilted2(T), torus.z_torus(T, 2),
+                         torus.z_torus(T, 4))
+    for cls in (tk.TRG, tk.BTRG):
+        n = gpu_run(tk, cls, tk.trivial(T), 16, 2)
+        assert abs(n[0] / abs(z1) - 1) < 1e-13
+        assert abs(n[0] ** 2 * n[1] / abs(zt2) - 1) < 1e-12
+        assert abs(n[0] ** 4 * n[1] ** 2 * n[2] / abs(z22) - 1) < 1e-12
+    n = gpu_run(tk, tk.HOTRG, tk.trivial(T), 16, 2)
+    assert abs(n[0] ** 4 * n[1] / abs(z22) - 1) < 1e-12
+    assert abs(n[0] ** 16 * n[1] ** 4 * n[2] / abs(z44) - 1) < 1e-11
+
+
+@pytest.mark.parametrize("name,chi,n", [("BTRG", 16, 10), ("TRG", 16, 10), ("HOTRG", 9, 6)])
+def test_potts_z3_vs_oracle(tk, name, chi, n):
+    b = om.potts_betac(3)
+    zt = om.classical_potts_zn(3, b)
+    d = gpu_run(tk, getattr(tk, name), to_sym(tk, zt), chi, n)
+    ref = np.array(osch.run(getattr(osch, name)(zt), chi, n))
+    assert rel(d, ref) < TOL
+
+
+def test_clock_z6_vs_oracle(tk):
+    zt = om.classical_clock_zn(6, 1.1)
+    d = gpu_run(tk, tk.BTRG, to_sym(tk, zt), 12, 8)
+    ref = np.array(osch.run(osch.BTRG(zt), 12, 8))
+    assert rel(d, ref) < TOL
+
+
+def test_batch_equals_single_runs(tk):
+    reqs = [tk.Request("BTRG", tk.classical_ising(0.40, symmetry="Trivial"), 16, 12),
+            tk.Request("TRG", tk.classical_ising(0.45, symmetry="Z2"), 12, 10),
+            tk.Request("BTRG", tk.classical_potts(3, 1.0), 16, 8),
+            tk.Request("HOTRG", tk.classical_ising(0.50, symmetry="Trivial"), 8, 6)]
+    batch = tk.run_batch(reqs)
+    for r, b in zip(reqs, batch):
+        cls = {0: tk.TRG, 1: tk.BTRG, 2: tk.HOTRG}[r.kind]
+        single = gpu_run(tk, cls, r.T, r.chi, r.maxiter)
+        assert np.array_equal(single, b), (r.kind, rel(single, b))
+
+
+def test_server_continuous_batching(tk):
+    srv = tk.Server(max_slots=3)
+    reqs = [tk.Request("BTRG", tk.classical_ising(0.35 + 0.02 * i, symmetry="Z2"), 8, 5 + i)
+            for i in range(6)]
+    ids = [srv.submit(r) for r in reqs[:2]]
+    srv.pump()
+    ids += [srv.submit(r) for r in reqs[2:]]          # arrive while others run
+    while srv.pump() > 0:
+        pass
+    for r, i in zip(reqs, ids):
+        res = srv.poll(i)
+        assert res is not None
+        alone = gpu_run(tk, tk.BTRG, r.T, r.chi, r.maxiter)
+        assert np.array_equal(res[0], alone)
+    srv.close()

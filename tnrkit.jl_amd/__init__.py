@@ -1,0 +1,21 @@
+"""MI355X engine for TNRKit.jl's TRG / BTRG / HOTRG coarse-graining loop.
+
+The arithmetic runs in libtnr_hip.so (HIP, gfx950); this package is the host
+side mirroring TNRKit's interface:
+
+    T = classical_ising(symmetry="Z2")          # beta_c
+    scheme = BTRG(T)
+    data = run(scheme, truncrank(16), maxiter(25))
+    f = free_energy(data, ising_betac)
+
+The directory name is not a valid Python identifier; load it with
+``__graft_entry__.package()`` (it registers the module as ``tnrkit_jl_amd``).
+"""
+from ._lib import TNRError, lib  # noqa: F401
+from .models import (classical_clock, classical_ising, classical_potts, f_onsager,  # noqa: F401
+                     ising_betac, potts_betac)
+from .schemes import (BTRG, FP64, HOTRG, SPLIT32, TRG, Finalizer, MultipleCrit,  # noqa: F401
+                      convcrit, default_Finalizer, finalize, free_energy, maxiter, run, step,
+                      trivial_convcrit, truncrank)
+from .serving import Request, Server, run_batch  # noqa: F401
+from .tensors import SymTensor, trivial  # noqa: F401

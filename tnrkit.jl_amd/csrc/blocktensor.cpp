@@ -1,0 +1,568 @@
+// Block-sparse tensor layouts, permutes, contractions and grouped GEMM issue.
+#include "blocktensor.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+namespace tnr {
+
+static int64_t ipow(int b, int e) {
+    int64_t r = 1;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+}
+
+// ------------------------------------------------------------------ Layout
+Layout::Layout(int N_, int ncod_, const std::vector<Leg>& legs_) {
+    TNR_REQUIRE(N_ >= 1 && N_ <= MAXN, TNR_ERR_ARG, "Z_N order out of range");
+    TNR_REQUIRE((int)legs_.size() <= MAXLEG, TNR_ERR_ARG, "too many legs");
+    TNR_REQUIRE(ncod_ >= 0 && ncod_ <= (int)legs_.size(), TNR_ERR_ARG, "bad codomain size");
+    N = N_;
+    ncod = ncod_;
+    nleg = (int)legs_.size();
+    for (int i = 0; i < nleg; ++i) legs[i] = legs_[i];
+    build();
+}
+
+void Layout::build() {
+    R.fill(0);
+    C.fill(0);
+    off.fill(0);
+    const int nd = nleg - ncod;
+    roff.assign(ipow(N, ncod), 0);
+    coff.assign(ipow(N, nd), 0);
+    for (int64_t t = 0; t < (int64_t)roff.size(); ++t) {
+        int64_t x = t, c = 0, sz = 1;
+        for (int k = 0; k < ncod; ++k) {
+            int l = (int)(x % N);
+            x /= N;
+            c += l;
+            sz *= legs[k].d[l];
+        }
+        c %= N;
+        roff[t] = R[c];
+        R[c] += sz;
+    }
+    for (int64_t t = 0; t < (int64_t)coff.size(); ++t) {
+        int64_t x = t, c = 0, sz = 1;
+        for (int k = 0; k < nd; ++k) {
+            int l = (int)(x % N);
+            x /= N;
+            c += l;
+            sz *= legs[ncod + k].d[(N - l) % N];
+        }
+        c %= N;
+        coff[t] = C[c];
+        C[c] += sz;
+    }
+    int64_t o = 0;
+    for (int c = 0; c < N; ++c) {
+        off[c] = o;
+        o += R[c] * C[c];
+    }
+    size = o;
+}
+
+bool Layout::same(const Layout& o) const {
+    if (N != o.N || nleg != o.nleg || ncod != o.ncod) return false;
+    for (int i = 0; i < nleg; ++i)
+        if (legs[i] != o.legs[i]) return false;
+    return true;
+}
+
+std::string Layout::str() const {
+    std::ostringstream s;
+    s << "Z" << N << " (";
+    for (int i = 0; i < nleg; ++i) {
+        if (i == ncod) s << "| ";
+        s << "[";
+        for (int q = 0; q < N; ++q) s << legs[i].d[q] << (q + 1 < N ? "," : "");
+        s << "] ";
+    }
+    s << ") size " << size;
+    return s.str();
+}
+
+BT::BT(Ctx& ctx, const Layout& L_) : L(L_), buf(&ctx.pool, std::max<int64_t>(L_.size, 1) * sizeof(double)) {}
+
+// ---------------------------------------------------------------- BondVec
+LegWeights BondVec::same_side() const {
+    LegWeights w;
+    w.w = buf.d();
+    for (int q = 0; q < N; ++q) w.off[q] = off[q];
+    return w;
+}
+
+LegWeights BondVec::dual_side() const {
+    LegWeights w;
+    w.w = buf.d();
+    for (int q = 0; q < N; ++q) w.off[q] = off[(N - q) % N];
+    return w;
+}
+
+Leg BondVec::leg_same() const {
+    Leg l;
+    for (int c = 0; c < N; ++c) l.d[c] = dim[c];
+    return l;
+}
+
+Leg BondVec::leg_dual() const {
+    Leg l;
+    for (int c = 0; c < N; ++c) l.d[(N - c) % N] = dim[c];
+    return l;
+}
+
+// --------------------------------------------------- sub-block enumeration
+// Calls f(q[], sizes[]) for every outgoing-charge assignment with sum 0 and
+// all leg sectors non-empty.
+template <class F>
+static void for_each_sector(const Layout& L, F&& f) {
+    const int n = L.nleg, N = L.N;
+    int q[MAXLEG] = {0};
+    const int64_t tot = ipow(N, n > 0 ? n - 1 : 0);
+    for (int64_t t = 0; t < tot; ++t) {
+        int64_t x = t, s = 0;
+        for (int k = 0; k + 1 < n; ++k) {
+            q[k] = (int)(x % N);
+            x /= N;
+            s += q[k];
+        }
+        if (n > 0) q[n - 1] = (int)((N - s % N) % N);
+        bool ok = true;
+        for (int k = 0; k < n; ++k)
+            if (L.legs[k].d[q[k]] == 0) { ok = false; break; }
+        if (ok) f(q);
+    }
+}
+
+// element offset of a sub-block and the stride of every leg inside it
+static int64_t subblock(const Layout& L, const int* q, int64_t* stride) {
+    const int N = L.N;
+    int64_t tc = 0, td = 0, mul = 1, c = 0;
+    int64_t st = 1;
+    for (int k = 0; k < L.ncod; ++k) {
+        int l = q[k];
+        tc += l * mul;
+        mul *= N;
+        c += l;
+        stride[k] = st;
+        st *= L.legs[k].d[q[k]];
+    }
+    c %= N;
+    mul = 1;
+    st = L.R[c];
+    for (int k = L.ncod; k < L.nleg; ++k) {
+        int l = (N - q[k]) % N;
+        td += l * mul;
+        mul *= N;
+        stride[k] = st;
+        st *= L.legs[k].d[q[k]];
+    }
+    return L.off[c] + L.roff[tc] + L.R[c] * L.coff[td];
+}
+
+// ------------------------------------------------------------------ permute
+static Layout permuted_layout(const Layout& S, const std::vector<int>& perm, int ncod, bool conj) {
+    const int n = S.nleg;
+    TNR_REQUIRE((int)perm.size() == n, TNR_ERR_INTERNAL, "permutation size mismatch");
+    std::vector<Leg> dl(n);
+    std::vector<int> seen(n, 0);
+    for (int j = 0; j < n; ++j) {
+        TNR_REQUIRE(perm[j] >= 0 && perm[j] < n && !seen[perm[j]], TNR_ERR_INTERNAL,
+                    "invalid permutation");
+        seen[perm[j]] = 1;
+        dl[j] = conj ? S.legs[perm[j]].dual(S.N) : S.legs[perm[j]];
+    }
+    return Layout(S.N, ncod, dl);
+}
+
+// append the sub-block copy descriptors of one permute job
+static void permute_descs(const BT& src, BT& dst, const std::vector<int>& perm, bool conj,
+                          const std::vector<WeightSpec>& ws, std::vector<CopyDesc>& descs,
+                          int64_t& start) {
+    const Layout& S = src.L;
+    const int n = S.nleg, N = S.N;
+    TNR_REQUIRE(ws.size() <= 2, TNR_ERR_INTERNAL, "at most two weighted legs");
+    std::vector<int> inv(n);
+    for (int j = 0; j < n; ++j) inv[perm[j]] = j;
+    for_each_sector(S, [&](const int* q) {
+        int64_t ss[MAXLEG], ds[MAXLEG];
+        int qd[MAXLEG];
+        int64_t sbase = subblock(S, q, ss);
+        for (int j = 0; j < n; ++j) qd[j] = conj ? (N - q[perm[j]]) % N : q[perm[j]];
+        int64_t dbase = subblock(dst.L, qd, ds);
+        CopyDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.src = src.d() + sbase;
+        d.dst = dst.d() + dbase;
+        d.nd = n;
+        int64_t cnt = 1;
+        for (int j = 0; j < n; ++j) {
+            d.dim[j] = S.legs[perm[j]].d[q[perm[j]]];
+            d.sstr[j] = ss[perm[j]];
+            d.dstr[j] = ds[j];
+            cnt *= d.dim[j];
+        }
+        // every access of the job must stay inside both tensors
+        int64_t smax = sbase, dmax = dbase;
+        for (int j = 0; j < n; ++j) {
+            smax += (int64_t)(d.dim[j] - 1) * d.sstr[j];
+            dmax += (int64_t)(d.dim[j] - 1) * d.dstr[j];
+        }
+        TNR_REQUIRE(sbase >= 0 && dbase >= 0 && smax < S.size && dmax < dst.L.size,
+                    TNR_ERR_INTERNAL, "permute descriptor out of bounds");
+        d.wleg[0] = d.wleg[1] = -1;
+        for (size_t w = 0; w < ws.size(); ++w) {
+            d.wleg[w] = inv[ws[w].leg];
+            d.w[w] = ws[w].w.w + ws[w].w.off[q[ws[w].leg]];
+        }
+        d.start = start;
+        start += cnt;
+        descs.push_back(d);
+    });
+}
+
+static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t total) {
+    if (total <= 0 || descs.empty()) return;
+    CopyDesc* dd = ctx.stage->put(ctx.stream, descs);
+    launch_copy(ctx.stream, dd, (int)descs.size(), total);
+}
+
+std::vector<BT> permute_many(Ctx& ctx, const std::vector<PermJob>& jobs) {
+    std::vector<BT> out;
+    out.reserve(jobs.size());
+    std::vector<CopyDesc> descs;
+    int64_t start = 0;
+    for (const PermJob& j : jobs) {
+        out.emplace_back(ctx, permuted_layout(j.src->L, j.perm, j.ncod, j.conj));
+        permute_descs(*j.src, out.back(), j.perm, j.conj, j.ws, descs, start);
+    }
+    issue_copies(ctx, descs, start);
+    return out;
+}
+
+BT permute(Ctx& ctx, const BT& src, const std::vector<int>& perm, int ncod, bool conj,
+           const std::vector<WeightSpec>& ws) {
+    std::vector<PermJob> j(1);
+    j[0].src = &src;
+    j[0].perm = perm;
+    j[0].ncod = ncod;
+    j[0].conj = conj;
+    j[0].ws = ws;
+    return std::move(permute_many(ctx, j)[0]);
+}
+
+// -------------------------------------------------------------- grouped GEMM
+void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32) {
+    std::vector<GemmDesc> descs;
+    std::vector<int64_t> red_starts;
+    int64_t tiles = 0, red_total = 0;
+    size_t work_elems = 0;
+    descs.reserve(jobs.size());
+    for (const GemmJob& j : jobs) {
+        if (j.M <= 0 || j.N <= 0) continue;
+        TNR_REQUIRE(j.K >= 0 && j.lda >= 1 && j.ldb >= 1 && j.ldc >= j.M, TNR_ERR_INTERNAL,
+                    "bad GEMM job");
+        GemmDesc g;
+        std::memset(&g, 0, sizeof(g));
+        g.A = j.A; g.B = j.B; g.C = j.C;
+        g.M = j.M; g.N = j.N; g.K = j.K;
+        g.lda = j.lda; g.ldb = j.ldb; g.ldc = j.ldc;
+        g.ta = j.ta; g.tb = j.tb;
+        g.alpha = j.alpha; g.beta = j.beta;
+        g.tiles_m = (j.M + GEMM_BM - 1) / GEMM_BM;
+        g.tiles_n = (j.N + GEMM_BN - 1) / GEMM_BN;
+        // split K by the problem's own shape only (identical alone/batched)
+        int ks = 1;
+        const int64_t mn = (int64_t)g.tiles_m * g.tiles_n;
+        const int ktiles = (j.K + GEMM_BK - 1) / GEMM_BK;
+        if (mn < 128 && ktiles >= 32) {
+            ks = (int)std::min<int64_t>(ktiles / 16, std::max<int64_t>(1, 256 / mn));
+            ks = std::max(1, std::min(ks, 64));
+        }
+        g.ksplit = ks;
+        g.tile_start = tiles;
+        tiles += mn * ks;
+        if (ks > 1) work_elems += (size_t)j.M * j.N * ks;
+        descs.push_back(g);
+    }
+    if (descs.empty()) return;
+    Buf work;
+    if (work_elems) {
+        work = Buf(&ctx.pool, work_elems * sizeof(double));
+        size_t o = 0;
+        for (auto& g : descs)
+            if (g.ksplit > 1) {
+                g.work = work.d() + o;
+                o += (size_t)g.M * g.N * g.ksplit;
+            }
+    }
+    GemmDesc* dd = ctx.stage->put(ctx.stream, descs);
+    launch_gemm(ctx.stream, dd, (int)descs.size(), tiles, split32);
+    if (work_elems) {
+        std::vector<GemmDesc> rd;
+        for (auto& g : descs)
+            if (g.ksplit > 1) {
+                rd.push_back(g);
+                red_starts.push_back(red_total);
+                red_total += (int64_t)g.M * g.N;
+            }
+        GemmDesc* drd = ctx.stage->put(ctx.stream, rd);
+        int64_t* dst = ctx.stage->put(ctx.stream, red_starts);
+        launch_gemm_reduce(ctx.stream, drd, (int)rd.size(), dst, red_total);
+    }
+    // `work` returns to the pool here; its next user is queued after the
+    // reduce on the same stream.
+}
+
+// --------------------------------------------------------------- contract
+static bool is_identity(const std::vector<int>& p) {
+    for (size_t i = 0; i < p.size(); ++i)
+        if (p[i] != (int)i) return false;
+    return true;
+}
+
+namespace {
+struct Plan {
+    std::vector<int> ca, cb, oa, ob;   // contracted / open legs
+    int transA = 0, transB = 0;
+    int permA = -1, permB = -1;        // index into the operand-permute batch
+    std::vector<int> pc;               // output permutation
+    Layout Cl;
+};
+}  // namespace
+
+std::vector<BT> contract_many(Ctx& ctx, const std::vector<ContractJob>& jobs) {
+    const size_t J = jobs.size();
+    std::vector<Plan> plans(J);
+    std::vector<PermJob> pj;
+    for (size_t x = 0; x < J; ++x) {
+        const Operand& a = jobs[x].a;
+        const Operand& b = jobs[x].b;
+        Plan& P = plans[x];
+        const Layout& LA = a.t->L;
+        const Layout& LB = b.t->L;
+        const int N = LA.N;
+        TNR_REQUIRE(LB.N == N, TNR_ERR_INTERNAL, "mixed symmetry groups");
+        TNR_REQUIRE((int)a.labels.size() == LA.nleg && (int)b.labels.size() == LB.nleg,
+                    TNR_ERR_INTERNAL, "label count mismatch");
+        for (int i = 0; i < LA.nleg; ++i) {
+            int l = a.labels[i];
+            if (l > 0) {
+                auto it = std::find(b.labels.begin(), b.labels.end(), l);
+                TNR_REQUIRE(it != b.labels.end(), TNR_ERR_INTERNAL, "unmatched contracted label");
+                P.ca.push_back(i);
+                P.cb.push_back((int)(it - b.labels.begin()));
+            }
+        }
+        for (int l : jobs[x].out) {
+            auto ia = std::find(a.labels.begin(), a.labels.end(), l);
+            if (ia != a.labels.end()) { P.oa.push_back((int)(ia - a.labels.begin())); continue; }
+            auto ib = std::find(b.labels.begin(), b.labels.end(), l);
+            TNR_REQUIRE(ib != b.labels.end(), TNR_ERR_INTERNAL, "unknown output label");
+            P.ob.push_back((int)(ib - b.labels.begin()));
+        }
+        TNR_REQUIRE(P.oa.size() + P.ca.size() == (size_t)LA.nleg &&
+                        P.ob.size() + P.cb.size() == (size_t)LB.nleg,
+                    TNR_ERR_INTERNAL, "every leg must be open or contracted once");
+        const bool conj_noop = N <= 2;   // -q == q
+        const bool cja = a.conj && !conj_noop, cjb = b.conj && !conj_noop;
+        std::vector<int> pa(P.oa), pb(P.cb);
+        pa.insert(pa.end(), P.ca.begin(), P.ca.end());
+        pb.insert(pb.end(), P.ob.begin(), P.ob.end());
+        // A as (open | contracted): as stored, as a transposed view (only when
+        // tuple orders coincide, N <= 2), or permuted
+        std::vector<int> ta(P.ca), tb(P.ob);
+        ta.insert(ta.end(), P.oa.begin(), P.oa.end());
+        tb.insert(tb.end(), P.cb.begin(), P.cb.end());
+        if (!cja && a.ws.empty() && is_identity(pa) && LA.ncod == (int)P.oa.size()) {
+        } else if (conj_noop && a.ws.empty() && is_identity(ta) && LA.ncod == (int)P.ca.size()) {
+            P.transA = 1;
+        } else {
+            P.permA = (int)pj.size();
+            pj.push_back(PermJob{a.t, pa, (int)P.oa.size(), cja, a.ws});
+        }
+        if (!cjb && b.ws.empty() && is_identity(pb) && LB.ncod == (int)P.cb.size()) {
+        } else if (conj_noop && b.ws.empty() && is_identity(tb) && LB.ncod == (int)P.ob.size()) {
+            P.transB = 1;
+        } else {
+            P.permB = (int)pj.size();
+            pj.push_back(PermJob{b.t, pb, (int)P.cb.size(), cjb, b.ws});
+        }
+        auto leg_of = [&](const Operand& o, int i) {
+            return (o.conj && !conj_noop) ? o.t->L.legs[i].dual(N) : o.t->L.legs[i];
+        };
+        for (size_t i = 0; i < P.ca.size(); ++i)
+            TNR_REQUIRE(leg_of(a, P.ca[i]) == leg_of(b, P.cb[i]).dual(N), TNR_ERR_INTERNAL,
+                        "contracted legs do not carry dual spaces");
+        std::vector<Leg> cl;
+        for (int i : P.oa) cl.push_back(leg_of(a, i));
+        for (int i : P.ob) cl.push_back(leg_of(b, i));
+        P.Cl = Layout(N, (int)P.oa.size(), cl);
+        for (int l : jobs[x].out) {
+            int pos = -1;
+            for (size_t i = 0; i < P.oa.size(); ++i)
+                if (a.labels[P.oa[i]] == l) pos = (int)i;
+            for (size_t i = 0; i < P.ob.size(); ++i)
+                if (b.labels[P.ob[i]] == l) pos = (int)(P.oa.size() + i);
+            P.pc.push_back(pos);
+        }
+    }
+    std::vector<BT> ops = permute_many(ctx, pj);
+    std::vector<BT> Cs;
+    Cs.reserve(J);
+    std::vector<GemmJob> gj;
+    for (size_t x = 0; x < J; ++x) {
+        Plan& P = plans[x];
+        const BT* A = P.permA >= 0 ? &ops[P.permA] : jobs[x].a.t;
+        const BT* B = P.permB >= 0 ? &ops[P.permB] : jobs[x].b.t;
+        Cs.emplace_back(ctx, P.Cl);
+        BT& C = Cs.back();
+        for (int c = 0; c < C.L.N; ++c) {
+            const int64_t M = C.L.R[c], Nn = C.L.C[c];
+            if (M == 0 || Nn == 0) continue;
+            const int64_t K = P.transA ? A->L.R[c] : A->L.C[c];
+            const int64_t KB = P.transB ? B->L.C[c] : B->L.R[c];
+            TNR_REQUIRE(K == KB, TNR_ERR_INTERNAL, "contracted dimension mismatch");
+            GemmJob j;
+            j.A = A->d() + A->L.off[c];
+            j.B = B->d() + B->L.off[c];
+            j.C = C.d() + C.L.off[c];
+            j.M = (int)M; j.N = (int)Nn; j.K = (int)K;
+            j.lda = (int)std::max<int64_t>(A->L.R[c], 1);
+            j.ldb = (int)std::max<int64_t>(B->L.R[c], 1);
+            j.ldc = (int)M;
+            j.ta = P.transA; j.tb = P.transB;
+            gj.push_back(j);
+        }
+    }
+    gemm_grouped(ctx, gj, ctx.precision == TNR_PREC_SPLIT32);
+    // output arrangement
+    std::vector<PermJob> fj;
+    std::vector<int> fidx(J, -1);
+    for (size_t x = 0; x < J; ++x) {
+        const Plan& P = plans[x];
+        if (is_identity(P.pc) && jobs[x].out_ncod == (int)P.oa.size()) continue;
+        fidx[x] = (int)fj.size();
+        fj.push_back(PermJob{&Cs[x], P.pc, jobs[x].out_ncod, false, {}});
+    }
+    std::vector<BT> fin = permute_many(ctx, fj);
+    std::vector<BT> out;
+    out.reserve(J);
+    for (size_t x = 0; x < J; ++x)
+        out.push_back(fidx[x] >= 0 ? std::move(fin[fidx[x]]) : std::move(Cs[x]));
+    return out;
+}
+
+// -------------------------------------------------------- trace/normalise
+void trace_normalize_many(Ctx& ctx, const std::vector<TraceJob>& jobs) {
+    std::vector<TraceDesc> descs;
+    std::vector<TraceProb> probs;
+    Buf partial(&ctx.pool, std::max<size_t>(1, jobs.size()) * TRACE_PARTS * sizeof(double));
+    int64_t max_size = 0;
+    for (size_t x = 0; x < jobs.size(); ++x) {
+        BT& t = *jobs[x].t;
+        const Layout& L = t.L;
+        const int N = L.N;
+        TNR_REQUIRE(L.nleg == 4 && L.ncod == 2, TNR_ERR_INTERNAL, "finalize needs a (2,2) tensor");
+        TNR_REQUIRE(L.legs[0] == L.legs[3].dual(N) && L.legs[1] == L.legs[2].dual(N),
+                    TNR_ERR_ARG, "legs 1/4 and 2/3 must carry the same spaces");
+        TraceProb P;
+        P.first = (int)descs.size();
+        int64_t start = 0;
+        for (int q1 = 0; q1 < N; ++q1)
+            for (int q2 = 0; q2 < N; ++q2) {
+                int q[4] = {q1, q2, (N - q2) % N, (N - q1) % N};
+                if (!L.legs[0].d[q1] || !L.legs[1].d[q2]) continue;
+                int64_t st[4];
+                int64_t base = subblock(L, q, st);
+                TNR_REQUIRE(base + (int64_t)(L.legs[0].d[q1] - 1) * (st[0] + st[3]) +
+                                    (int64_t)(L.legs[1].d[q2] - 1) * (st[1] + st[2]) < L.size,
+                            TNR_ERR_INTERNAL, "trace descriptor out of bounds");
+                TraceDesc d;
+                d.T = t.d() + base;
+                d.s1 = st[0] + st[3];
+                d.s2 = st[1] + st[2];
+                d.d1 = L.legs[0].d[q1];
+                d.d2 = L.legs[1].d[q2];
+                d.w1 = jobs[x].w_x1 ? jobs[x].w_x1->w + jobs[x].w_x1->off[q1] : nullptr;
+                d.w2 = jobs[x].w_x2 ? jobs[x].w_x2->w + jobs[x].w_x2->off[q2] : nullptr;
+                d.start = start;
+                start += (int64_t)d.d1 * d.d2;
+                descs.push_back(d);
+            }
+        TNR_REQUIRE(start > 0, TNR_ERR_ARG, "empty tensor");
+        P.count = (int)descs.size() - P.first;
+        P.total = start;
+        P.T = t.d();
+        P.size = L.size;
+        P.partial = partial.d() + x * TRACE_PARTS;
+        P.norm_slot = jobs[x].norm_slot;
+        probs.push_back(P);
+        max_size = std::max(max_size, L.size);
+    }
+    if (probs.empty()) return;
+    TraceDesc* dd = ctx.stage->put(ctx.stream, descs);
+    TraceProb* dp = ctx.stage->put(ctx.stream, probs);
+    launch_trace_normalize(ctx.stream, dd, dp, (int)probs.size(), max_size);
+}
+
+// ------------------------------------------------------------------ gather
+void gather_many(Ctx& ctx, std::vector<GatherDesc> descs) {
+    int64_t start = 0;
+    std::vector<GatherDesc> d;
+    for (auto& g : descs) {
+        if (g.rows <= 0 || g.cols <= 0) continue;
+        g.start = start;
+        start += (int64_t)g.rows * g.cols;
+        d.push_back(g);
+    }
+    if (d.empty()) return;
+    GatherDesc* dd = ctx.stage->put(ctx.stream, d);
+    launch_gather(ctx.stream, dd, (int)d.size(), start);
+}
+
+// ---------------------------------------------------------- ABI transfer
+Layout abi_layout(int N, const int32_t* leg_dims) {
+    TNR_REQUIRE(N >= 1 && N <= MAXN, TNR_ERR_ARG, "zn must be in 1..8");
+    std::vector<Leg> legs(4);
+    for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < N; ++a) {
+            int d = leg_dims[i * N + a];
+            TNR_REQUIRE(d >= 0, TNR_ERR_ARG, "negative sector dimension");
+            // codomain legs: q = a; domain legs: q = -a
+            int q = (i < 2) ? a : (N - a) % N;
+            legs[i].d[q] = d;
+        }
+    return Layout(N, 2, legs);
+}
+
+void abi_dims(const Layout& L, int32_t* leg_dims) {
+    const int N = L.N;
+    for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < N; ++a) {
+            int q = (i < 2) ? a : (N - a) % N;
+            leg_dims[i * N + a] = L.legs[i].d[q];
+        }
+}
+
+BT upload(Ctx& ctx, int N, const int32_t* leg_dims, const double* data, size_t n) {
+    TNR_REQUIRE(leg_dims && data, TNR_ERR_ARG, "null pointer");
+    Layout L = abi_layout(N, leg_dims);
+    TNR_REQUIRE((int64_t)n == L.size, TNR_ERR_ARG,
+                "ndata = " + std::to_string(n) + " does not match the sector dimensions (" +
+                    std::to_string(L.size) + ")");
+    TNR_REQUIRE(L.size > 0, TNR_ERR_ARG, "empty tensor");
+    BT t(ctx, L);
+    TNR_HIP(hipMemcpyAsync(t.d(), data, n * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+    ctx.sync();
+    return t;
+}
+
+void download(Ctx& ctx, const BT& t, double* data) {
+    ctx.fetch(data, t.d(), t.L.size * sizeof(double));
+}
+
+}  // namespace tnr

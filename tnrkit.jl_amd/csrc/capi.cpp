@@ -1,0 +1,468 @@
+// extern "C" entry points of libtnr_hip.so (include/tnr.h).  No exception
+// crosses this boundary: every call maps failures to a status code and keeps
+// the message for tnr_last_error().
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+
+#include "schemes.h"
+
+using namespace tnr;
+
+namespace {
+thread_local std::string g_last_error;
+
+}  // namespace
+
+void tnr::set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+int fail(int code, const char* msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return TNR_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return TNR_ERR_INTERNAL;
+    } catch (...) {
+        g_last_error = "unknown failure";
+        return TNR_ERR_INTERNAL;
+    }
+}
+
+int steps_of(int maxiter) { return maxiter < 1 ? 1 : maxiter; }   // run! steps at least once
+
+void check_norms(const double* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i]) || v[i] == 0.0)
+            throw Error(TNR_ERR_NONFINITE, "norm " + std::to_string(i) + " is " + std::to_string(v[i]));
+}
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+}  // namespace
+
+struct tnr_scheme {
+    std::unique_ptr<Ctx> ctx;
+    Scheme s;
+    Buf norms;
+    int cap = 0;
+    ~tnr_scheme() {
+        if (ctx) ctx->sync();
+        s = Scheme();
+        norms.release();
+        ctx.reset();
+    }
+    double* slots(int n) {
+        if (n > cap) {
+            norms = Buf(&ctx->pool, (size_t)n * sizeof(double));
+            cap = n;
+        }
+        return norms.d();
+    }
+};
+
+extern "C" {
+
+int tnr_create(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims, const double* data,
+               size_t ndata, double k, int precision) {
+    return guarded([&] {
+        if (!out) return fail(TNR_ERR_ARG, "out is null");
+        *out = nullptr;
+        if (kind != TNR_TRG && kind != TNR_BTRG && kind != TNR_HOTRG)
+            return fail(TNR_ERR_ARG, "kind must be TNR_TRG, TNR_BTRG or TNR_HOTRG");
+        if (zn < 1 || zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
+        if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
+            return fail(TNR_ERR_ARG, "unknown precision");
+        if (!std::isfinite(k)) return fail(TNR_ERR_ARG, "k must be finite");
+        std::unique_ptr<tnr_scheme> h(new tnr_scheme());
+        h->ctx.reset(new Ctx());
+        h->ctx->precision = precision;
+        BT T = upload(*h->ctx, zn, leg_dims, data, ndata);
+        scheme_init(*h->ctx, h->s, kind, std::move(T), k);
+        *out = h.release();
+        return TNR_OK;
+    });
+}
+
+int tnr_step(tnr_scheme* s, int chi) {
+    return guarded([&] {
+        if (!s) return fail(TNR_ERR_ARG, "null handle");
+        if (chi < 1) return fail(TNR_ERR_ARG, "chi must be >= 1");
+        step_many(*s->ctx, {&s->s}, {chi});
+        s->ctx->sync();
+        return TNR_OK;
+    });
+}
+
+int tnr_finalize(tnr_scheme* s, double* norm) {
+    return guarded([&] {
+        if (!s || !norm) return fail(TNR_ERR_ARG, "null pointer");
+        double* slot = s->slots(1);
+        finalize_many(*s->ctx, {&s->s}, {slot});
+        s->ctx->fetch(norm, slot, sizeof(double));
+        check_norms(norm, 1);
+        return TNR_OK;
+    });
+}
+
+int tnr_run(tnr_scheme* s, int chi, int maxiter, int finalize_beginning, double* norms,
+            int* n_out) {
+    return guarded([&] {
+        if (!s || !norms) return fail(TNR_ERR_ARG, "null pointer");
+        if (chi < 1) return fail(TNR_ERR_ARG, "chi must be >= 1");
+        const int steps = steps_of(maxiter);
+        const int n = steps + (finalize_beginning ? 1 : 0);
+        double* slot = s->slots(n);
+        Ctx& ctx = *s->ctx;
+        int w = 0;
+        if (finalize_beginning) finalize_many(ctx, {&s->s}, {slot + w++});
+        for (int i = 0; i < steps; ++i) {
+            step_many(ctx, {&s->s}, {chi});
+            finalize_many(ctx, {&s->s}, {slot + w++});
+        }
+        ctx.fetch(norms, slot, n * sizeof(double));
+        if (n_out) *n_out = n;
+        check_norms(norms, n);
+        return TNR_OK;
+    });
+}
+
+int tnr_free_energy(const double* norms, int n, double beta, double scalefactor,
+                    double initial_size, double* f_out) {
+    return guarded([&] {
+        if (!norms || !f_out) return fail(TNR_ERR_ARG, "null pointer");
+        if (n < 1 || !(scalefactor > 0) || scalefactor == 1.0 || !(initial_size > 0) || beta == 0)
+            return fail(TNR_ERR_ARG, "invalid free-energy arguments");
+        *f_out = free_energy(norms, n, beta, scalefactor, initial_size);
+        return TNR_OK;
+    });
+}
+
+int tnr_get_dims(const tnr_scheme* s, int32_t* leg_dims, size_t* ndata) {
+    return guarded([&] {
+        if (!s) return fail(TNR_ERR_ARG, "null handle");
+        if (leg_dims) abi_dims(s->s.T.L, leg_dims);
+        if (ndata) *ndata = (size_t)s->s.T.L.size;
+        return TNR_OK;
+    });
+}
+
+int tnr_get_tensor(tnr_scheme* s, double* data, size_t ndata) {
+    return guarded([&] {
+        if (!s || !data) return fail(TNR_ERR_ARG, "null pointer");
+        if ((int64_t)ndata != s->s.T.L.size) return fail(TNR_ERR_ARG, "ndata does not match");
+        download(*s->ctx, s->s.T, data);
+        return TNR_OK;
+    });
+}
+
+int tnr_set_tensor(tnr_scheme* s, const int32_t* leg_dims, const double* data, size_t ndata) {
+    return guarded([&] {
+        if (!s) return fail(TNR_ERR_ARG, "null handle");
+        BT T = upload(*s->ctx, s->s.N, leg_dims, data, ndata);
+        scheme_set_tensor(*s->ctx, s->s, std::move(T));
+        return TNR_OK;
+    });
+}
+
+int tnr_get_bond_weights(tnr_scheme* s, int which, double* w, size_t n) {
+    return guarded([&] {
+        if (!s || !w) return fail(TNR_ERR_ARG, "null pointer");
+        if (s->s.kind != TNR_BTRG) return fail(TNR_ERR_STATE, "only BTRG has bond weights");
+        if (which != 1 && which != 2) return fail(TNR_ERR_ARG, "which must be 1 or 2");
+        const BondVec& b = which == 1 ? s->s.S1 : s->s.S2;
+        if (n != b.host.size()) return fail(TNR_ERR_ARG, "size does not match");
+        std::memcpy(w, b.host.data(), n * sizeof(double));
+        return TNR_OK;
+    });
+}
+
+int tnr_get_trunc_log(tnr_scheme* s, double* out, int max_entries, int* n_entries) {
+    return guarded([&] {
+        if (!s || !n_entries) return fail(TNR_ERR_ARG, "null pointer");
+        const int n = (int)s->s.tlog.size();
+        *n_entries = n;
+        if (out)
+            for (int i = 0; i < std::min(n, max_entries); ++i)
+                std::memcpy(out + 4 * i, s->s.tlog[i].data(), 4 * sizeof(double));
+        return TNR_OK;
+    });
+}
+
+int tnr_destroy(tnr_scheme* s) {
+    return guarded([&] {
+        delete s;
+        return TNR_OK;
+    });
+}
+
+const char* tnr_strerror(int status) {
+    switch (status) {
+        case TNR_OK: return "success";
+        case TNR_ERR_ARG: return "invalid argument";
+        case TNR_ERR_HIP: return "HIP runtime error";
+        case TNR_ERR_NOMEM: return "out of device memory";
+        case TNR_ERR_NONFINITE: return "non-finite or zero norm";
+        case TNR_ERR_NOCONV: return "solver did not converge";
+        case TNR_ERR_STATE: return "invalid state";
+        case TNR_ERR_NODEVICE: return "no usable gfx950 device";
+        case TNR_ERR_INTERNAL: return "internal error";
+        case TNR_ERR_BUDGET: return "request exceeds the HBM budget";
+        case TNR_ERR_BUSY: return "result not ready";
+        default: return "unknown status";
+    }
+}
+
+const char* tnr_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+// ======================================================== serving / batches
+namespace {
+
+struct Req {
+    int64_t id = 0;
+    int kind = 0, zn = 1, chi = 1, maxiter = 1, fb = 1;
+    double k = -0.5;
+    std::vector<int32_t> dims;
+    std::vector<double> data;
+    size_t est = 0;
+    double t_submit = 0, t_done = 0;
+    std::vector<double> norms;
+    int status = TNR_ERR_BUSY;
+};
+
+struct Slot {
+    Req* r = nullptr;
+    Scheme s;
+    Buf norms;
+    int nsteps = 0, done = 0, written = 0;
+};
+
+}  // namespace
+
+struct tnr_server {
+    Ctx ctx;
+    size_t budget = 0, in_use = 0;
+    int max_slots = 1;
+    int64_t next_id = 1;
+    std::map<int64_t, std::unique_ptr<Req>> reqs;
+    std::deque<Req*> queue;
+    std::vector<std::unique_ptr<Slot>> active;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    void admit() {
+        std::vector<Slot*> fresh;
+        while (!queue.empty() && (int)active.size() < max_slots) {
+            Req* r = queue.front();
+            if (!active.empty() && in_use + r->est > budget) break;   // wait for memory
+            queue.pop_front();
+            std::unique_ptr<Slot> sl(new Slot());
+            sl->r = r;
+            sl->nsteps = steps_of(r->maxiter);
+            const int n = sl->nsteps + (r->fb ? 1 : 0);
+            try {
+                BT T = upload(ctx, r->zn, r->dims.data(), r->data.data(), r->data.size());
+                scheme_init(ctx, sl->s, r->kind, std::move(T), r->k);
+                sl->norms = Buf(&ctx.pool, (size_t)n * sizeof(double));
+            } catch (const Error& e) {
+                r->status = e.code;
+                r->t_done = now_s(t0);
+                continue;
+            }
+            in_use += r->est;
+            fresh.push_back(sl.get());
+            active.push_back(std::move(sl));
+        }
+        // finalize at the beginning, grouped over the newly admitted
+        std::vector<Scheme*> g;
+        std::vector<double*> slots;
+        for (Slot* sl : fresh)
+            if (sl->r->fb) {
+                g.push_back(&sl->s);
+                slots.push_back(sl->norms.d() + sl->written++);
+            }
+        if (!g.empty()) finalize_many(ctx, g, slots);
+    }
+
+    void retire() {
+        std::vector<std::unique_ptr<Slot>> keep;
+        for (auto& sl : active) {
+            if (sl->done < sl->nsteps) {
+                keep.push_back(std::move(sl));
+                continue;
+            }
+            Req* r = sl->r;
+            r->norms.resize(sl->written);
+            ctx.fetch(r->norms.data(), sl->norms.d(), sl->written * sizeof(double));
+            r->status = TNR_OK;
+            for (double v : r->norms)
+                if (!std::isfinite(v) || v == 0.0) r->status = TNR_ERR_NONFINITE;
+            r->t_done = now_s(t0);
+            in_use -= r->est;
+        }
+        active.swap(keep);
+    }
+
+    int pump() {
+        admit();
+        if (!active.empty()) {
+            std::vector<Scheme*> g;
+            std::vector<int> chis;
+            std::vector<double*> slots;
+            for (auto& sl : active) {
+                g.push_back(&sl->s);
+                chis.push_back(sl->r->chi);
+            }
+            step_many(ctx, g, chis);
+            for (auto& sl : active) {
+                slots.push_back(sl->norms.d() + sl->written++);
+                sl->done++;
+            }
+            finalize_many(ctx, g, slots);
+        }
+        retire();
+        return (int)(active.size() + queue.size());
+    }
+};
+
+extern "C" {
+
+int tnr_server_create(tnr_server** out, size_t hbm_budget_bytes, int max_slots, int precision) {
+    return guarded([&] {
+        if (!out) return fail(TNR_ERR_ARG, "out is null");
+        *out = nullptr;
+        if (max_slots < 1) return fail(TNR_ERR_ARG, "max_slots must be >= 1");
+        if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
+            return fail(TNR_ERR_ARG, "unknown precision");
+        std::unique_ptr<tnr_server> s(new tnr_server());
+        s->ctx.precision = precision;
+        size_t fr = 0, tot = 0;
+        TNR_HIP(hipMemGetInfo(&fr, &tot));
+        s->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(0.8 * (double)fr);
+        s->max_slots = max_slots;
+        *out = s.release();
+        return TNR_OK;
+    });
+}
+
+int tnr_server_submit(tnr_server* s, const tnr_run_desc* req, int64_t* id) {
+    return guarded([&] {
+        if (!s || !req || !id) return fail(TNR_ERR_ARG, "null pointer");
+        if (req->kind != TNR_TRG && req->kind != TNR_BTRG && req->kind != TNR_HOTRG)
+            return fail(TNR_ERR_ARG, "unknown scheme kind");
+        if (req->zn < 1 || req->zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
+        if (req->chi < 1) return fail(TNR_ERR_ARG, "chi must be >= 1");
+        if (!req->leg_dims || !req->data) return fail(TNR_ERR_ARG, "null tensor");
+        Layout L = abi_layout(req->zn, req->leg_dims);
+        if ((int64_t)req->ndata != L.size || L.size == 0)
+            return fail(TNR_ERR_ARG, "ndata does not match the sector dimensions");
+        std::unique_ptr<Req> r(new Req());
+        r->id = s->next_id++;
+        r->kind = req->kind; r->zn = req->zn; r->chi = req->chi;
+        r->maxiter = req->maxiter; r->fb = req->finalize_beginning ? 1 : 0; r->k = req->k;
+        r->dims.assign(req->leg_dims, req->leg_dims + 4 * req->zn);
+        r->data.assign(req->data, req->data + req->ndata);
+        r->est = scheme_hbm_estimate(r->kind, r->zn, r->chi);
+        if (r->est > s->budget) return fail(TNR_ERR_BUDGET, "request exceeds the HBM budget");
+        r->t_submit = now_s(s->t0);
+        *id = r->id;
+        s->queue.push_back(r.get());
+        s->reqs[r->id] = std::move(r);
+        return TNR_OK;
+    });
+}
+
+int tnr_server_pump(tnr_server* s, int* active) {
+    return guarded([&] {
+        if (!s) return fail(TNR_ERR_ARG, "null handle");
+        int a = s->pump();
+        if (active) *active = a;
+        return TNR_OK;
+    });
+}
+
+int tnr_server_poll(tnr_server* s, int64_t id, double* norms, int* n_out, double* t_submit,
+                    double* t_done) {
+    return guarded([&] {
+        if (!s) return fail(TNR_ERR_ARG, "null handle");
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) return fail(TNR_ERR_ARG, "unknown request id");
+        Req& r = *it->second;
+        if (r.status == TNR_ERR_BUSY) return TNR_ERR_BUSY;
+        if (t_submit) *t_submit = r.t_submit;
+        if (t_done) *t_done = r.t_done;
+        if (n_out) *n_out = (int)r.norms.size();
+        if (norms && !r.norms.empty()) std::memcpy(norms, r.norms.data(), r.norms.size() * sizeof(double));
+        return r.status;
+    });
+}
+
+int tnr_server_stats(tnr_server* s, size_t* hbm_in_use, int* resident, int* queued) {
+    return guarded([&] {
+        if (!s) return fail(TNR_ERR_ARG, "null handle");
+        if (hbm_in_use) *hbm_in_use = s->ctx.pool.in_use();
+        if (resident) *resident = (int)s->active.size();
+        if (queued) *queued = (int)s->queue.size();
+        return TNR_OK;
+    });
+}
+
+int tnr_server_destroy(tnr_server* s) {
+    return guarded([&] {
+        if (s) {
+            s->ctx.sync();
+            s->active.clear();
+        }
+        delete s;
+        return TNR_OK;
+    });
+}
+
+int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision, double* norms) {
+    return guarded([&] {
+        if (nruns < 0 || (nruns > 0 && (!runs || !norms))) return fail(TNR_ERR_ARG, "null pointer");
+        tnr_server* s = nullptr;
+        int st = tnr_server_create(&s, 0, nruns > 0 ? nruns : 1, precision);
+        if (st != TNR_OK) return st;
+        std::unique_ptr<tnr_server> hold(s);
+        std::vector<int64_t> ids(nruns);
+        for (int i = 0; i < nruns; ++i) {
+            st = tnr_server_submit(s, &runs[i], &ids[i]);
+            if (st != TNR_OK) return st;
+        }
+        int active = 1;
+        while (active > 0) s->pump(), active = (int)(s->active.size() + s->queue.size());
+        size_t o = 0;
+        for (int i = 0; i < nruns; ++i) {
+            Req& r = *s->reqs[ids[i]];
+            if (r.status != TNR_OK) {
+                g_last_error = "run " + std::to_string(i) + " failed: " + tnr_strerror(r.status);
+                return r.status;
+            }
+            std::memcpy(norms + o, r.norms.data(), r.norms.size() * sizeof(double));
+            o += r.norms.size();
+        }
+        s->active.clear();
+        return TNR_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+// Kernel-level entry points (include/tnr_kernels.h).
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "solver.h"
+#include "../../include/tnr_kernels.h"
+
+using namespace tnr;
+
+namespace {
+template <class F>
+int kguard(F&& f) {
+    try {
+        return f();
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return TNR_ERR_INTERNAL;
+    }
+}
+
+Buf up(Ctx& ctx, const double* h, size_t n) {
+    Buf b(&ctx.pool, std::max<size_t>(n, 1) * 8);
+    if (n) TNR_HIP(hipMemcpyAsync(b.d(), h, n * 8, hipMemcpyHostToDevice, ctx.stream));
+    return b;
+}
+}  // namespace
+
+extern "C" {
+
+int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta, const double* B,
+                    int ldb, int tb, double* C, int ldc, double alpha, double beta, int split32,
+                    int repeat, double* ms_out) {
+    return kguard([&] {
+        TNR_REQUIRE(M > 0 && N > 0 && K >= 0 && A && B && C && repeat >= 1, TNR_ERR_ARG,
+                    "bad GEMM arguments");
+        TNR_REQUIRE(lda >= (ta ? std::max(K, 1) : M) && ldb >= (tb ? N : std::max(K, 1)) && ldc >= M,
+                    TNR_ERR_ARG, "leading dimension too small");
+        Ctx ctx;
+        const size_t na = (size_t)lda * (ta ? M : K), nb = (size_t)ldb * (tb ? K : N),
+                     nc = (size_t)ldc * N;
+        Buf dA = up(ctx, A, na), dB = up(ctx, B, nb), dC = up(ctx, C, nc);
+        Buf dC0 = up(ctx, C, nc);
+        std::vector<GemmJob> j{GemmJob{dA.d(), dB.d(), dC.d(), M, N, K, lda, ldb, ldc, ta, tb}};
+        j[0].alpha = alpha;
+        j[0].beta = beta;
+        hipEvent_t e0, e1;
+        TNR_HIP(hipEventCreate(&e0));
+        TNR_HIP(hipEventCreate(&e1));
+        float ms = 0;
+        for (int r = 0; r < repeat; ++r) {
+            if (r > 0 && beta != 0.0)
+                TNR_HIP(hipMemcpyAsync(dC.d(), dC0.d(), nc * 8, hipMemcpyDeviceToDevice, ctx.stream));
+            if (r == repeat - 1 || r == 0) TNR_HIP(hipEventRecord(e0, ctx.stream));
+            gemm_grouped(ctx, j, split32 != 0);
+            if (r == repeat - 1 || r == 0) {
+                TNR_HIP(hipEventRecord(e1, ctx.stream));
+                TNR_HIP(hipEventSynchronize(e1));
+                float t = 0;
+                TNR_HIP(hipEventElapsedTime(&t, e0, e1));
+                if (r == repeat - 1) ms = t;
+            }
+        }
+        ctx.fetch(C, dC.d(), nc * 8);
+        if (ms_out) *ms_out = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return TNR_OK;
+    });
+}
+
+int tnr_kernel_svd(int m, int n, const double* M, int k, double* U, double* s, double* V,
+                   int* iters) {
+    return kguard([&] {
+        TNR_REQUIRE(m > 0 && n > 0 && k > 0 && k <= std::min(m, n) && M && U && s && V,
+                    TNR_ERR_ARG, "bad SVD arguments");
+        Ctx ctx;
+        Buf dM = up(ctx, M, (size_t)m * n);
+        std::vector<SvdIn> in{SvdIn{dM.d(), m, n, m, k, 12345}};
+        std::vector<SvdOut> out;
+        svd_grouped(ctx, in, out);
+        std::vector<double> hu((size_t)out[0].ldu * k), hv((size_t)out[0].ldv * k);
+        ctx.fetch(hu.data(), out[0].U.d(), hu.size() * 8);
+        ctx.fetch(hv.data(), out[0].V.d(), hv.size() * 8);
+        for (int j = 0; j < k; ++j) {
+            std::memcpy(U + (size_t)j * m, hu.data() + (size_t)j * out[0].ldu, m * 8);
+            std::memcpy(V + (size_t)j * n, hv.data() + (size_t)j * out[0].ldv, n * 8);
+        }
+        std::memcpy(s, out[0].s_host.data(), k * 8);
+        if (iters) *iters = out[0].iters;
+        return TNR_OK;
+    });
+}
+
+int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam, int* iters) {
+    return kguard([&] {
+        TNR_REQUIRE(n > 0 && k > 0 && k <= n && M && U && lam, TNR_ERR_ARG, "bad eigh arguments");
+        Ctx ctx;
+        Buf dM = up(ctx, M, (size_t)n * n);
+        std::vector<EighIn> in{EighIn{dM.d(), n, n, k, 12345}};
+        std::vector<EighOut> out;
+        eigh_grouped(ctx, in, out);
+        std::vector<double> hu((size_t)n * out[0].ncols);
+        ctx.fetch(hu.data(), out[0].U.d(), hu.size() * 8);
+        std::memcpy(U, hu.data(), (size_t)n * k * 8);
+        std::memcpy(lam, out[0].lam_host.data(), k * 8);
+        if (iters) *iters = out[0].iters;
+        return TNR_OK;
+    });
+}
+
+int tnr_kernel_jacobi(int count, int n, const double* G, double* W, double* lam, int* sweeps) {
+    return kguard([&] {
+        TNR_REQUIRE(count > 0 && n > 0 && n <= LMAX && G && W && lam, TNR_ERR_ARG,
+                    "bad Jacobi arguments");
+        Ctx ctx;
+        const size_t nn = (size_t)n * n;
+        Buf dG = up(ctx, G, nn * count);
+        Buf dW(&ctx.pool, nn * count * 8), dL(&ctx.pool, (size_t)n * count * 8);
+        Buf dS(&ctx.pool, (size_t)count * sizeof(int));
+        std::vector<JacobiDesc> jd;
+        for (int i = 0; i < count; ++i)
+            jd.push_back(JacobiDesc{dG.d() + nn * i, dW.d() + nn * i, dL.d() + (size_t)n * i,
+                                    dS.as<int>() + i, n, n, n});
+        JacobiDesc* d = ctx.stage->put(ctx.stream, jd);
+        launch_jacobi(ctx.stream, d, count, n, solver_cfg().jacobi_tol, solver_cfg().jacobi_sweeps);
+        ctx.fetch(W, dW.d(), nn * count * 8);
+        ctx.fetch(lam, dL.d(), (size_t)n * count * 8);
+        std::vector<int> sw(count);
+        ctx.fetch(sw.data(), dS.d(), count * sizeof(int));
+        if (sweeps) *sweeps = *std::max_element(sw.begin(), sw.end());
+        return TNR_OK;
+    });
+}
+
+}  // extern "C"

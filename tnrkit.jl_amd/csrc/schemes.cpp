@@ -1,0 +1,582 @@
+// TRG / BTRG / HOTRG steps (see schemes.h).  Networks follow
+// src/schemes/trg.jl:35-41, src/schemes/btrg.jl:60-94 and
+// src/schemes/hotrg.jl:42-160; finalize follows src/utility/finalize.jl:1-14.
+#include "schemes.h"
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <numeric>
+
+namespace tnr {
+
+namespace {
+
+uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// seed of a solver problem: a function of its shape only, so a run gets the
+// same start vectors alone and inside a batch
+uint64_t problem_seed(int m, int n, int c, int tag) {
+    return mix64(((uint64_t)m << 40) ^ ((uint64_t)n << 16) ^ ((uint64_t)c << 8) ^ (uint64_t)tag);
+}
+
+BondVec make_bond(Ctx& ctx, int N, const int* dim, const std::vector<double>& vals) {
+    BondVec b;
+    b.N = N;
+    int o = 0;
+    for (int c = 0; c < N; ++c) {
+        b.dim[c] = dim[c];
+        b.off[c] = o;
+        o += dim[c];
+    }
+    TNR_REQUIRE((size_t)o == vals.size(), TNR_ERR_INTERNAL, "bond vector size mismatch");
+    b.host = vals;
+    b.buf = Buf(&ctx.pool, std::max<size_t>(1, vals.size()) * sizeof(double));
+    if (!vals.empty()) {
+        void* tmp = ctx.stage->put(ctx.stream, vals.data(), vals.size() * sizeof(double));
+        TNR_HIP(hipMemcpyAsync(b.buf.d(), tmp, vals.size() * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx.stream));
+    }
+    return b;
+}
+
+// s^a where s >= tol, else s unchanged (src/schemes/btrg.jl:50-58)
+double pseudopow(double s, double a) {
+    const double tol = std::pow(2.220446049250313e-16, 0.75);
+    return s < tol ? s : std::pow(s, a);
+}
+
+struct Factor {
+    BT U, V;                 // U-side (codomain legs | new), V-side (new | domain legs)
+    std::vector<double> s;   // kept singular values, by sector c then descending
+    int dim[MAXN] = {0};     // kept count per sector c
+    double kept_min = 0, dropped_max = 0;
+    int iters = 0;
+    double conv = 0;
+};
+
+// weights (wu on U's new leg, wv on V's new leg) of tensor t from its kept
+// singular values
+using Weigh = std::function<void(size_t t, const std::vector<double>&, std::vector<double>&,
+                                  std::vector<double>&)>;
+
+// svd_trunc(T; trunc = truncrank(chi)) per coupled-charge block with the
+// chi largest values kept across blocks (stable, descending), then
+// U * diag(wu) and diag(wv) * V^T as block tensors.
+std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
+                                   const std::vector<int>& chis, int tag, const Weigh& weigh) {
+    std::vector<SvdIn> in;
+    std::vector<std::pair<int, int>> owner;
+    for (size_t t = 0; t < Ts.size(); ++t) {
+        const Layout& L = Ts[t]->L;
+        TNR_REQUIRE(chis[t] >= 1, TNR_ERR_ARG, "chi must be >= 1");
+        for (int c = 0; c < L.N; ++c) {
+            const int R = (int)L.R[c], C = (int)L.C[c];
+            if (R == 0 || C == 0) continue;
+            SvdIn p;
+            p.M = Ts[t]->d() + L.off[c];
+            p.m = R; p.n = C; p.ld = R;
+            p.kmax = std::min(chis[t], std::min(R, C));
+            p.seed = problem_seed(R, C, c, tag);
+            in.push_back(p);
+            owner.emplace_back((int)t, c);
+        }
+    }
+    std::vector<SvdOut> out;
+    svd_grouped(ctx, in, out);
+    std::vector<Factor> F(Ts.size());
+    std::vector<GatherDesc> gd;
+    size_t p0 = 0;
+    for (size_t t = 0; t < Ts.size(); ++t) {
+        const Layout& L = Ts[t]->L;
+        const int N = L.N;
+        size_t p1 = p0;
+        while (p1 < owner.size() && owner[p1].first == (int)t) ++p1;
+        // global cut
+        struct Cand { double v; int blk, idx; };
+        std::vector<Cand> flat;
+        for (size_t p = p0; p < p1; ++p)
+            for (int i = 0; i < in[p].kmax; ++i) flat.push_back(Cand{out[p].s_host[i], (int)p, i});
+        std::stable_sort(flat.begin(), flat.end(),
+                         [](const Cand& a, const Cand& b) { return a.v > b.v; });
+        const size_t keep = std::min<size_t>(chis[t], flat.size());
+        TNR_REQUIRE(keep > 0, TNR_ERR_ARG, "tensor has no non-empty block");
+        Factor& f = F[t];
+        std::vector<int> kc(in.size(), 0);
+        for (size_t i = 0; i < keep; ++i) kc[flat[i].blk]++;
+        f.kept_min = flat[keep - 1].v;
+        f.dropped_max = keep < flat.size() ? flat[keep].v : 0.0;
+        for (size_t p = p0; p < p1; ++p) {
+            f.dim[owner[p].second] = kc[p];
+            f.iters = std::max(f.iters, out[p].iters);
+            f.conv = std::max(f.conv, out[p].conv);
+        }
+        int off[MAXN], o = 0;
+        for (int c = 0; c < N; ++c) { off[c] = o; o += f.dim[c]; }
+        f.s.assign(o, 0.0);
+        for (size_t p = p0; p < p1; ++p) {
+            const int c = owner[p].second;
+            for (int i = 0; i < kc[p]; ++i) f.s[off[c] + i] = out[p].s_host[i];
+        }
+        std::vector<double> wu, wv;
+        weigh(t, f.s, wu, wv);
+        const double* dwu = (const double*)ctx.stage->put(ctx.stream, wu.data(), wu.size() * 8);
+        const double* dwv = (const double*)ctx.stage->put(ctx.stream, wv.data(), wv.size() * 8);
+        // layouts of the factors
+        Leg nl_u, nl_v;
+        for (int c = 0; c < N; ++c) {
+            nl_u.d[(N - c) % N] = f.dim[c];   // domain leg with label c
+            nl_v.d[c] = f.dim[c];             // codomain leg with label c
+        }
+        std::vector<Leg> lu, lv;
+        for (int i = 0; i < L.ncod; ++i) lu.push_back(L.legs[i]);
+        lu.push_back(nl_u);
+        lv.push_back(nl_v);
+        for (int i = L.ncod; i < L.nleg; ++i) lv.push_back(L.legs[i]);
+        f.U = BT(ctx, Layout(N, L.ncod, lu));
+        f.V = BT(ctx, Layout(N, 1, lv));
+        for (size_t p = p0; p < p1; ++p) {
+            const int c = owner[p].second;
+            if (!kc[p]) continue;
+            TNR_REQUIRE(f.U.L.R[c] == in[p].m && f.U.L.C[c] == kc[p] && f.V.L.R[c] == kc[p] &&
+                            f.V.L.C[c] == in[p].n,
+                        TNR_ERR_INTERNAL, "factor layout mismatch");
+            GatherDesc gu{};
+            gu.in = out[p].U.d(); gu.out = f.U.d() + f.U.L.off[c];
+            gu.cw = dwu + off[c];
+            gu.rows = in[p].m; gu.cols = kc[p]; gu.ldi = out[p].ldu; gu.ldo = in[p].m;
+            gd.push_back(gu);
+            GatherDesc gv{};
+            gv.in = out[p].V.d(); gv.out = f.V.d() + f.V.L.off[c];
+            gv.rw = dwv + off[c];
+            gv.rows = kc[p]; gv.cols = in[p].n; gv.ldi = out[p].ldv; gv.ldo = kc[p]; gv.trans = 1;
+            gd.push_back(gv);
+        }
+        p0 = p1;
+    }
+    gather_many(ctx, gd);
+    return F;
+}
+
+struct EFactor {
+    BT U;                    // (codomain legs | new)
+    double eps = 0;          // 2-norm of the discarded eigenvalues
+    double kept_min = 0, dropped_max = 0;
+    int iters = 0;
+    double conv = 0;
+};
+
+// eigh_trunc!(project_hermitian!(MM); trunc = truncrank(chi)): the chi
+// largest |lambda| across blocks; MM is overwritten.
+std::vector<EFactor> eigh_trunc_many(Ctx& ctx, const std::vector<BT*>& Ms,
+                                     const std::vector<int>& chis, int tag) {
+    std::vector<EighIn> in;
+    std::vector<std::pair<int, int>> owner;
+    for (size_t t = 0; t < Ms.size(); ++t) {
+        const Layout& L = Ms[t]->L;
+        for (int c = 0; c < L.N; ++c) {
+            const int R = (int)L.R[c];
+            if (R == 0) continue;
+            TNR_REQUIRE(L.C[c] == R, TNR_ERR_INTERNAL, "environment block is not square");
+            EighIn p;
+            p.M = Ms[t]->d() + L.off[c];
+            p.n = R; p.ld = R;
+            p.kmax = std::min(chis[t], R);
+            p.seed = problem_seed(R, R, c, tag);
+            in.push_back(p);
+            owner.emplace_back((int)t, c);
+        }
+    }
+    std::vector<EighOut> out;
+    eigh_grouped(ctx, in, out);
+    std::vector<EFactor> F(Ms.size());
+    std::vector<int> kc(in.size(), 0);
+    std::vector<double> eps2(Ms.size(), 0.0);
+    // cut and the directly known part of the truncation error
+    size_t p0 = 0;
+    std::vector<std::pair<size_t, size_t>> ranges;
+    for (size_t t = 0; t < Ms.size(); ++t) {
+        size_t p1 = p0;
+        while (p1 < owner.size() && owner[p1].first == (int)t) ++p1;
+        ranges.emplace_back(p0, p1);
+        struct Cand { double a; int blk, idx; };
+        std::vector<Cand> flat;
+        for (size_t p = p0; p < p1; ++p)
+            for (int i = 0; i < in[p].kmax; ++i)
+                flat.push_back(Cand{std::fabs(out[p].lam_host[i]), (int)p, i});
+        std::stable_sort(flat.begin(), flat.end(),
+                         [](const Cand& a, const Cand& b) { return a.a > b.a; });
+        const size_t keep = std::min<size_t>(chis[t], flat.size());
+        TNR_REQUIRE(keep > 0, TNR_ERR_ARG, "environment has no non-empty block");
+        for (size_t i = 0; i < keep; ++i) kc[flat[i].blk]++;
+        F[t].kept_min = flat[keep - 1].a;
+        F[t].dropped_max = keep < flat.size() ? flat[keep].a : 0.0;
+        for (size_t p = p0; p < p1; ++p) {
+            F[t].iters = std::max(F[t].iters, out[p].iters);
+            F[t].conv = std::max(F[t].conv, out[p].conv);
+            if (out[p].direct)
+                for (int i = kc[p]; i < in[p].n; ++i) eps2[t] += out[p].lam_host[i] * out[p].lam_host[i];
+        }
+        p0 = p1;
+    }
+    // iterative blocks: |MM - U_k L_k U_k^T|_F^2 on the device
+    {
+        std::vector<GatherDesc> gd;
+        std::vector<GemmJob> gj;
+        std::vector<ColDesc> fd;
+        std::vector<Buf> tmp;
+        std::vector<size_t> which;
+        Buf fro(&ctx.pool, std::max<size_t>(1, in.size()) * sizeof(double));
+        for (size_t p = 0; p < in.size(); ++p) {
+            if (out[p].direct) continue;
+            const int n = in[p].n, k = kc[p];
+            if (k > 0) {
+                tmp.emplace_back(&ctx.pool, (size_t)n * k * 8);
+                GatherDesc g{};
+                g.in = out[p].U.d(); g.out = tmp.back().d(); g.cw = out[p].lam.d();
+                g.rows = n; g.cols = k; g.ldi = n; g.ldo = n;
+                gd.push_back(g);
+                GemmJob j{tmp.back().d(), out[p].U.d(), in[p].M, n, n, k, n, n, in[p].ld, 0, 1};
+                j.alpha = -1.0;
+                j.beta = 1.0;
+                gj.push_back(j);
+            }
+            fd.push_back(ColDesc{in[p].M, nullptr, fro.d() + which.size(), n, n, in[p].ld, 0});
+            which.push_back(p);
+        }
+        if (!which.empty()) {
+            gather_many(ctx, gd);
+            gemm_grouped(ctx, gj);
+            ColDesc* dfd = ctx.stage->put(ctx.stream, fd);
+            launch_frob(ctx.stream, dfd, (int)fd.size());
+            std::vector<double> h(which.size());
+            ctx.fetch(h.data(), fro.d(), h.size() * 8);
+            for (size_t i = 0; i < which.size(); ++i) eps2[owner[which[i]].first] += h[i];
+        }
+    }
+    std::vector<GatherDesc> gd;
+    for (size_t t = 0; t < Ms.size(); ++t) {
+        const Layout& L = Ms[t]->L;
+        const int N = L.N;
+        F[t].eps = std::sqrt(eps2[t]);
+        Leg nl;
+        for (size_t p = ranges[t].first; p < ranges[t].second; ++p)
+            nl.d[(N - owner[p].second) % N] = kc[p];
+        F[t].U = BT(ctx, Layout(N, 2, {L.legs[0], L.legs[1], nl}));
+        for (size_t p = ranges[t].first; p < ranges[t].second; ++p) {
+            const int c = owner[p].second;
+            if (!kc[p]) continue;
+            TNR_REQUIRE(F[t].U.L.R[c] == in[p].n && F[t].U.L.C[c] == kc[p], TNR_ERR_INTERNAL,
+                        "projector layout mismatch");
+            GatherDesc g{};
+            g.in = out[p].U.d(); g.out = F[t].U.d() + F[t].U.L.off[c];
+            g.rows = in[p].n; g.cols = kc[p]; g.ldi = in[p].n; g.ldo = in[p].n;
+            gd.push_back(g);
+        }
+    }
+    gather_many(ctx, gd);
+    return F;
+}
+
+void log_factor(Scheme& s, double kept, double dropped, int iters, double conv) {
+    s.tlog.push_back({kept, dropped, (double)iters, conv});
+}
+
+// ------------------------------------------------------------------- TRG
+void step_trg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
+    const size_t n = G.size();
+    auto sqrtw = [](size_t, const std::vector<double>& s, std::vector<double>& wu,
+                    std::vector<double>& wv) {
+        wu.resize(s.size());
+        for (size_t i = 0; i < s.size(); ++i) wu[i] = std::sqrt(s[i]);
+        wv = wu;
+    };
+    std::vector<const BT*> Ts;
+    for (auto* s : G) Ts.push_back(&s->T);
+    // A, B = SVD12(T) (src/utility/projectors.jl:211-217)
+    auto F1 = svd_trunc_many(ctx, Ts, chis, 11, sqrtw);
+    // Tp = transpose(T, ((2, 4), (1, 3)))
+    std::vector<PermJob> pj;
+    for (auto* s : G) pj.push_back(PermJob{&s->T, {1, 3, 0, 2}, 2, false, {}});
+    auto Tp = permute_many(ctx, pj);
+    std::vector<const BT*> Tps;
+    for (auto& t : Tp) Tps.push_back(&t);
+    auto F2 = svd_trunc_many(ctx, Tps, chis, 12, sqrtw);
+    // T[-1 -2; -3 -4] := D[-2; 1 2] * B[-1; 4 1] * C[4 3; -3] * A[3 2; -4]
+    std::vector<ContractJob> cj;
+    for (size_t i = 0; i < n; ++i) {
+        const BT& A = F1[i].U; const BT& B = F1[i].V;
+        const BT& C = F2[i].U; const BT& D = F2[i].V;
+        cj.push_back(ContractJob{{&B, {-1, -4, 1}}, {&D, {-2, 1, -3}}, {-1, -2, -4, -3}, 2});
+        cj.push_back(ContractJob{{&C, {-1, 1, -3}}, {&A, {1, -2, -4}}, {-1, -2, -3, -4}, 2});
+    }
+    auto XY = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)
+        cj.push_back(ContractJob{{&XY[2 * i], {-1, -2, 1, 2}}, {&XY[2 * i + 1], {1, 2, -3, -4}},
+                                 {-1, -2, -3, -4}, 2});
+    auto Tn = contract_many(ctx, cj);
+    for (size_t i = 0; i < n; ++i) {
+        G[i]->tlog.clear();
+        log_factor(*G[i], F1[i].kept_min, F1[i].dropped_max, F1[i].iters, F1[i].conv);
+        log_factor(*G[i], F2[i].kept_min, F2[i].dropped_max, F2[i].iters, F2[i].conv);
+        G[i]->T = std::move(Tn[i]);
+    }
+}
+
+// ------------------------------------------------------------------ BTRG
+void step_btrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
+    const size_t n = G.size();
+    std::vector<const BT*> Ts;
+    for (auto* s : G) Ts.push_back(&s->T);
+    std::vector<std::vector<double>> sb1(n), sb2(n);
+    auto w_for = [&](std::vector<std::vector<double>>& sb) {
+        return [&G, &sb](size_t t, const std::vector<double>& s, std::vector<double>& wu,
+                         std::vector<double>& wv) {
+            const double k = G[t]->k;
+            wu.resize(s.size());
+            sb[t].resize(s.size());
+            for (size_t i = 0; i < s.size(); ++i) {
+                wu[i] = pseudopow(s[i], (1.0 - k) / 2.0);   // S_a
+                sb[t][i] = pseudopow(s[i], k);               // S_b
+            }
+            wv = wu;
+        };
+    };
+    // U, S, V = svd_trunc(T): A = U S_a, B = S_a V, S1' = S_b
+    auto F1 = svd_trunc_many(ctx, Ts, chis, 21, w_for(sb1));
+    // U, S, V = svd_trunc(permute(T, ((3, 1), (4, 2)))): C, D, S2'
+    std::vector<PermJob> pj;
+    for (auto* s : G) pj.push_back(PermJob{&s->T, {2, 0, 3, 1}, 2, false, {}});
+    auto P = permute_many(ctx, pj);
+    std::vector<const BT*> Ps;
+    for (auto& t : P) Ps.push_back(&t);
+    auto F2 = svd_trunc_many(ctx, Ps, chis, 22, w_for(sb2));
+    // T[-1 -2; -3 -4] := D[-1; 4 7] S1[1; 7] B[-2; 1 3] S2[3; 2]
+    //                    C[8 2; -4] S1[8; 5] A[6 5; -3] S2[4; 6]
+    // with diagonal S1 (vertical) and S2 (horizontal) absorbed as leg weights:
+    //   X[p,q,f,c] = sum_g D[p,f,g] s2[f] s1[g] B[q,g,c]
+    //   Y[f,c,r,s] = sum_h C[h,c,s] s1[h] s2[c] A[f,h,r]
+    //   T'[p,q,r,s] = sum_{f,c} X[p,q,f,c] Y[f,c,r,s]
+    std::vector<LegWeights> w1s(n), w1d(n), w2s(n), w2d(n);
+    for (size_t i = 0; i < n; ++i) {
+        w1s[i] = G[i]->S1.same_side(); w1d[i] = G[i]->S1.dual_side();
+        w2s[i] = G[i]->S2.same_side(); w2d[i] = G[i]->S2.dual_side();
+    }
+    std::vector<ContractJob> cj;
+    for (size_t i = 0; i < n; ++i) {
+        const BT& A = F1[i].U; const BT& B = F1[i].V;
+        const BT& C = F2[i].U; const BT& D = F2[i].V;
+        Operand d{&D, {-1, -2, 1}, false, {WeightSpec{1, w2d[i]}, WeightSpec{2, w1s[i]}}};
+        Operand b{&B, {-3, 1, -4}};
+        cj.push_back(ContractJob{d, b, {-1, -3, -2, -4}, 2});
+        Operand c{&C, {1, -2, -3}, false, {WeightSpec{0, w1d[i]}, WeightSpec{1, w2s[i]}}};
+        Operand a{&A, {-1, 1, -4}};
+        cj.push_back(ContractJob{c, a, {-1, -2, -4, -3}, 2});
+    }
+    auto XY = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)
+        cj.push_back(ContractJob{{&XY[2 * i], {-1, -2, 1, 2}}, {&XY[2 * i + 1], {1, 2, -3, -4}},
+                                 {-1, -2, -3, -4}, 2});
+    auto Tn = contract_many(ctx, cj);
+    for (size_t i = 0; i < n; ++i) {
+        Scheme& s = *G[i];
+        s.tlog.clear();
+        log_factor(s, F1[i].kept_min, F1[i].dropped_max, F1[i].iters, F1[i].conv);
+        log_factor(s, F2[i].kept_min, F2[i].dropped_max, F2[i].iters, F2[i].conv);
+        s.T = std::move(Tn[i]);
+        s.S1 = make_bond(ctx, s.N, F1[i].dim, sb1[i]);
+        s.S2 = make_bond(ctx, s.N, F2[i].dim, sb2[i]);
+    }
+}
+
+// ----------------------------------------------------------------- HOTRG
+// environments of src/schemes/hotrg.jl:82-116 (x) and :118-152 (y); pick
+// U unless eps > eps' (:112-115, :148-151)
+std::vector<BT> hotrg_proj(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis,
+                           bool xdir, int tag) {
+    const size_t n = G.size();
+    std::vector<ContractJob> cj;
+    for (size_t i = 0; i < n; ++i) {
+        const BT* T = &G[i]->T;
+        if (xdir) {
+            // MM  = A2[-1 5; 1 2] A1[-2 3; 5 4] conj(A2[-3 6; 1 2]) conj(A1[-4 3; 6 4])
+            cj.push_back(ContractJob{{T, {-1, -2, 1, 2}}, {T, {-3, -4, 1, 2}, true}, {-1, -2, -3, -4}, 2});
+            cj.push_back(ContractJob{{T, {-1, 3, -2, 4}}, {T, {-3, 3, -4, 4}, true}, {-1, -2, -3, -4}, 2});
+            // MM' = conj(A2[2 5; 1 -1]) conj(A1[4 3; 5 -2]) A2[2 6; 1 -3] A1[4 3; 6 -4]
+            cj.push_back(ContractJob{{T, {2, -1, 1, -2}, true}, {T, {2, -3, 1, -4}}, {-1, -2, -3, -4}, 2});
+            cj.push_back(ContractJob{{T, {4, 3, -1, -2}, true}, {T, {4, 3, -3, -4}}, {-1, -2, -3, -4}, 2});
+        } else {
+            // MM  = A1[1 -1; 2 5] A2[5 -2; 4 3] conj(A1[1 -3; 2 6]) conj(A2[6 -4; 4 3])
+            cj.push_back(ContractJob{{T, {1, -1, 2, -2}}, {T, {1, -3, 2, -4}, true}, {-1, -2, -3, -4}, 2});
+            cj.push_back(ContractJob{{T, {-1, -2, 4, 3}}, {T, {-3, -4, 4, 3}, true}, {-1, -2, -3, -4}, 2});
+            // MM' = conj(A1[1 2; -1 5]) conj(A2[5 4; -2 3]) A1[1 2; -3 6] A2[6 4; -4 3]
+            cj.push_back(ContractJob{{T, {1, 2, -1, -2}, true}, {T, {1, 2, -3, -4}}, {-1, -2, -3, -4}, 2});
+            cj.push_back(ContractJob{{T, {-1, 4, -2, 3}, true}, {T, {-3, 4, -4, 3}}, {-1, -2, -3, -4}, 2});
+        }
+    }
+    auto E = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < n; ++i) {
+        if (xdir) {
+            cj.push_back(ContractJob{{&E[4 * i], {-1, 5, -3, 6}}, {&E[4 * i + 1], {-2, 5, -4, 6}}, {-1, -2, -3, -4}, 2});
+            cj.push_back(ContractJob{{&E[4 * i + 2], {5, -1, 6, -3}}, {&E[4 * i + 3], {5, -2, 6, -4}}, {-1, -2, -3, -4}, 2});
+        } else {
+            cj.push_back(ContractJob{{&E[4 * i], {-1, 5, -3, 6}}, {&E[4 * i + 1], {5, -2, 6, -4}}, {-1, -2, -3, -4}, 2});
+            cj.push_back(ContractJob{{&E[4 * i + 2], {-1, 5, -3, 6}}, {&E[4 * i + 3], {5, -2, 6, -4}}, {-1, -2, -3, -4}, 2});
+        }
+    }
+    auto MM = contract_many(ctx, cj);
+    E.clear();
+    std::vector<BT*> Ms;
+    std::vector<int> ch;
+    for (size_t i = 0; i < n; ++i) {
+        Ms.push_back(&MM[2 * i]);
+        Ms.push_back(&MM[2 * i + 1]);
+        ch.push_back(chis[i]);
+        ch.push_back(chis[i]);
+    }
+    auto F = eigh_trunc_many(ctx, Ms, ch, tag);
+    std::vector<BT> U;
+    for (size_t i = 0; i < n; ++i) {
+        EFactor& a = F[2 * i];
+        EFactor& b = F[2 * i + 1];
+        log_factor(*G[i], a.kept_min, a.dropped_max, a.iters, a.conv);
+        log_factor(*G[i], b.kept_min, b.dropped_max, b.iters, b.conv);
+        U.push_back(a.eps > b.eps ? std::move(b.U) : std::move(a.U));
+    }
+    return U;
+}
+
+void step_hotrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
+    const size_t n = G.size();
+    for (auto* s : G) s->tlog.clear();
+    auto Ux = hotrg_proj(ctx, G, chis, true, 31);
+    // _step_hotrg_y: T[-1 -2; -3 -4] := conj(Ux[1 2; -1]) Ux[3 4; -4] A2[1 5; -3 3] A1[2 -2; 5 4]
+    std::vector<ContractJob> cj;
+    for (size_t i = 0; i < n; ++i)   // P[m1, m3, x3 | x2, x5]
+        cj.push_back(ContractJob{{&Ux[i], {1, -4, -1}, true}, {&G[i]->T, {1, -5, -2, -3}},
+                                 {-1, -2, -3, -4, -5}, 3});
+    auto P = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // Q[m1, m2, m3 | x3, x4]
+        cj.push_back(ContractJob{{&P[i], {-1, -3, -4, 2, 5}}, {&G[i]->T, {2, -2, 5, -5}},
+                                 {-1, -2, -3, -4, -5}, 3});
+    auto Q = contract_many(ctx, cj);
+    P.clear();
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // T[m1, m2 | m3, m4]
+        cj.push_back(ContractJob{{&Q[i], {-1, -2, -3, 3, 4}}, {&Ux[i], {3, 4, -4}},
+                                 {-1, -2, -3, -4}, 2});
+    auto Ty = contract_many(ctx, cj);
+    Q.clear();
+    for (size_t i = 0; i < n; ++i) G[i]->T = std::move(Ty[i]);
+    auto Uy = hotrg_proj(ctx, G, chis, false, 32);
+    // _step_hotrg_x: T[-1 -2; -3 -4] := A1[-1 1; 3 5] A2[5 2; 4 -4] conj(Uy[1 2; -2]) Uy[3 4; -3]
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // R[m1, x3, m2 | x2, x5]
+        cj.push_back(ContractJob{{&G[i]->T, {-1, 1, -2, -5}}, {&Uy[i], {1, -4, -3}, true},
+                                 {-1, -2, -3, -4, -5}, 3});
+    auto R = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // S[m1, m2, m4 | x3, x4]
+        cj.push_back(ContractJob{{&R[i], {-1, -4, -2, 2, 5}}, {&G[i]->T, {5, 2, -5, -3}},
+                                 {-1, -2, -3, -4, -5}, 3});
+    auto S = contract_many(ctx, cj);
+    R.clear();
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // T[m1, m2 | m3, m4]
+        cj.push_back(ContractJob{{&S[i], {-1, -2, -4, 3, 4}}, {&Uy[i], {3, 4, -3}},
+                                 {-1, -2, -3, -4}, 2});
+    auto Tx = contract_many(ctx, cj);
+    for (size_t i = 0; i < n; ++i) G[i]->T = std::move(Tx[i]);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ public
+void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T) {
+    TNR_REQUIRE(T.L.nleg == 4 && T.L.ncod == 2, TNR_ERR_ARG, "scheme tensors have 2+2 legs");
+    TNR_REQUIRE(T.L.legs[0] == T.L.legs[3].dual(T.L.N) && T.L.legs[1] == T.L.legs[2].dual(T.L.N),
+                TNR_ERR_ARG, "legs 1/4 and 2/3 must carry the same spaces");
+    s.N = T.L.N;
+    s.T = std::move(T);
+    if (s.kind == TNR_BTRG) {
+        // S1 = id(space(T, 2)), S2 = id(space(T, 1)) (src/schemes/btrg.jl:41-43)
+        std::vector<double> o1(s.T.L.legs[1].total(s.N), 1.0), o2(s.T.L.legs[0].total(s.N), 1.0);
+        s.S1 = make_bond(ctx, s.N, s.T.L.legs[1].d, o1);
+        s.S2 = make_bond(ctx, s.N, s.T.L.legs[0].d, o2);
+    }
+}
+
+void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k) {
+    TNR_REQUIRE(kind == TNR_TRG || kind == TNR_BTRG || kind == TNR_HOTRG, TNR_ERR_ARG,
+                "unknown scheme kind");
+    s.kind = kind;
+    s.k = k;
+    s.steps = 0;
+    scheme_set_tensor(ctx, s, std::move(T));
+}
+
+void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<int>& chis) {
+    for (int kind : {TNR_TRG, TNR_BTRG, TNR_HOTRG}) {
+        std::vector<Scheme*> g;
+        std::vector<int> c;
+        for (size_t i = 0; i < group.size(); ++i)
+            if (group[i]->kind == kind) {
+                g.push_back(group[i]);
+                c.push_back(chis[i]);
+            }
+        if (g.empty()) continue;
+        if (kind == TNR_TRG) step_trg(ctx, g, c);
+        else if (kind == TNR_BTRG) step_btrg(ctx, g, c);
+        else step_hotrg(ctx, g, c);
+        for (auto* s : g) s->steps++;
+    }
+}
+
+void finalize_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<double*>& slots) {
+    std::vector<TraceJob> tj(group.size());
+    std::vector<LegWeights> w1(group.size()), w2(group.size());
+    for (size_t i = 0; i < group.size(); ++i) {
+        Scheme& s = *group[i];
+        tj[i].t = &s.T;
+        tj[i].norm_slot = slots[i];
+        if (s.kind == TNR_BTRG) {
+            // n = |T[1 2; 4 3] S1[4; 2] S2[3; 1]| (src/utility/finalize.jl:10-14)
+            w1[i] = s.S2.same_side();
+            w2[i] = s.S1.same_side();
+            tj[i].w_x1 = &w1[i];
+            tj[i].w_x2 = &w2[i];
+        }
+    }
+    trace_normalize_many(ctx, tj);
+}
+
+double free_energy(const double* norms, int n, double beta, double scalefactor,
+                   double initial_size) {
+    // src/utility/free_energy.jl:13-20
+    double lnz = 0.0;
+    const double x = 1.0 - std::log(initial_size) / std::log(scalefactor);
+    for (int i = 1; i <= n; ++i) lnz += std::log(norms[i - 1]) * std::pow(scalefactor, x - i);
+    return -lnz / beta;
+}
+
+size_t scheme_hbm_estimate(int kind, int N, int chi) {
+    // steady-state live set: a handful of chi^4 tensors (T, permuted T,
+    // contraction intermediates) spread over N sectors, the solver blocks,
+    // and for HOTRG the chi^5 intermediates
+    const double c4 = std::pow((double)chi, 4) / std::max(1, N) * 8.0;
+    const double c5 = std::pow((double)chi, 5) / std::max(1, N) * 8.0;
+    double b = 8.0 * c4 + 4.0 * (double)chi * chi * LMAX * 8.0 * 6.0;
+    if (kind == TNR_HOTRG) b += 3.0 * c5;
+    return (size_t)b + (size_t(8) << 20);
+}
+
+}  // namespace tnr

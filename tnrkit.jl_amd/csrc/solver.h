@@ -1,0 +1,65 @@
+// Grouped truncated SVD and symmetric eigensolver on the device (FP64).
+//
+// Small problems (min(m, n) <= LMAX) are solved directly: Gram matrix on
+// MFMA, parallel-order Jacobi in LDS, a second Gram/Jacobi pass on the
+// rotated columns (which are then nearly orthogonal, so the second pass is
+// relatively accurate), column norms.  Larger problems run block subspace
+// iteration (Y = M V, Z = M^T U) with the same two-pass Gram/Jacobi
+// Rayleigh-Ritz step after every product, until the top-k subspace has
+// converged to working precision; the Ritz triplets are then extracted from
+// Y = M V.  All problems of a call advance together in grouped launches.
+#pragma once
+
+#include <vector>
+
+#include "blocktensor.h"
+
+namespace tnr {
+
+struct SvdIn {
+    const double* M;     // m x n, column-major, leading dim ld
+    int m, n, ld;
+    int kmax;            // triplets wanted, <= min(m, n)
+    uint64_t seed;
+};
+
+struct SvdOut {
+    Buf U, V, s;         // U: m x kmax (ld m), V: n x kmax (ld n), s: kmax
+    int ldu = 0, ldv = 0;
+    std::vector<double> s_host;
+    int iters = 0;       // subspace iterations (0 = direct)
+    double conv = 0.0;   // accumulated contraction bound of the top-k subspace
+};
+
+void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& out);
+
+struct EighIn {
+    double* M;           // n x n symmetric (symmetrised in place), leading dim ld
+    int n, ld;
+    int kmax;
+    uint64_t seed;
+};
+
+struct EighOut {
+    Buf U, lam;          // U: n x ncols (ld n), lam: ncols, by |lambda| descending
+    int ncols = 0;       // n (direct) or the block size (iterative)
+    std::vector<double> lam_host;
+    bool direct = false; // all n eigenvalues available
+    int iters = 0;
+    double conv = 0.0;
+};
+
+void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>& out);
+
+// Solver knobs (the autotuner may override them).
+struct SolverCfg {
+    double jacobi_tol = 1e-15;   // relative off-diagonal threshold
+    int jacobi_sweeps = 40;
+    double null_rtol = 1e-14;    // columns below this x sigma_max are null
+    double conv_target = 1e-20;  // accumulated contraction target
+    int min_iters = 2;
+    int max_iters = 80;
+};
+SolverCfg& solver_cfg();
+
+}  // namespace tnr

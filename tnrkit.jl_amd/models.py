@@ -1,0 +1,122 @@
+"""Initial tensors of the 2D classical models, in the engine's format.
+
+src/models/ising.jl:1-67, src/models/potts.jl:6-81, src/models/clock.jl:1-49.
+Leg order (left, bottom, top, right) = T[1 2; 3 4]; index i of a Z_N leg
+carries sector i.
+"""
+from __future__ import annotations
+
+import math
+from decimal import Decimal, getcontext
+
+import numpy as np
+
+from .tensors import SymTensor, trivial
+
+getcontext().prec = 50
+#: src/models/ising.jl:1 (beta_c = ln(1 + sqrt 2) / 2)
+ising_betac = math.log(1.0 + math.sqrt(2.0)) / 2.0
+#: src/models/ising.jl:2
+f_onsager = -2.1096511446082074596677792835110847808254932754354
+
+
+def _bond(beta, exact_betac):
+    """(sqrt(cosh b), sqrt(sinh b)); at the reference's BigFloat beta_c the
+    entries are correctly rounded (src/models/ising.jl:10-15)."""
+    if exact_betac:
+        b = (Decimal(1) + Decimal(2).sqrt()).ln() / 2
+        e, ei = b.exp(), (-b).exp()
+        x, y = (e + ei) / 2, (e - ei) / 2
+        return float(x.sqrt()), float(y.sqrt()), float(x), float(y)
+    x, y = math.cosh(beta), math.sinh(beta)
+    return math.sqrt(x), math.sqrt(y), x, y
+
+
+def classical_ising(beta=None, symmetry="Z2"):
+    """2D Ising partition-function tensor at h = 0.
+
+    ``symmetry`` "Trivial" follows src/models/ising.jl:44-55, "Z2" follows
+    :56-67.  ``beta=None`` means the reference's beta_c."""
+    exact = beta is None
+    b = ising_betac if exact else float(beta)
+    sx, sy, x, y = _bond(b, exact)
+    if symmetry == "Trivial":
+        w = np.array([sx, sy])
+        T = np.zeros((2, 2, 2, 2))
+        for i in range(2):
+            for j in range(2):
+                for k in range(2):
+                    for l in range(2):
+                        if (i + j + k + l) % 2 == 0:
+                            T[i, j, k, l] = 2.0 * w[i] * w[j] * w[k] * w[l]
+        return trivial(T)
+    if symmetry == "Z2":
+        if exact:
+            bd = (Decimal(1) + Decimal(2).sqrt()).ln() / 2
+            e, ei = bd.exp(), (-bd).exp()
+            xd, yd = (e + ei) / 2, (e - ei) / 2
+            xx, xy, yy = float(2 * xd * xd), float(2 * xd * yd), float(2 * yd * yd)
+        else:
+            xx, xy, yy = 2 * x * x, 2 * x * y, 2 * y * y
+        # blocks: c=0 rows (0,0),(1,1) -> [2x^2 2xy; 2xy 2y^2]; c=1 -> 2xy
+        data = np.array([xx, xy, xy, yy, xy, xy, xy, xy])
+        return SymTensor(2, np.ones((4, 2), dtype=np.int32), data)
+    raise ValueError("symmetry must be 'Trivial' or 'Z2'")
+
+
+def potts_betac(q):
+    """src/models/potts.jl:39"""
+    return math.log(1.0 + math.sqrt(q))
+
+
+def _potts_dense(q, beta):
+    A = np.zeros((q, q, q, q))
+    for i in range(q):
+        for j in range(q):
+            for k in range(q):
+                for l in range(q):
+                    E = -((i == j) + (j == l) + (k == l) + (k == i))
+                    A[i, j, k, l] = math.exp(-beta * E)
+    return A
+
+
+def _clock_dense(q, beta):
+    def clock(a, b):
+        return -math.cos(2 * math.pi / q * (a - b))
+    A = np.zeros((q, q, q, q))
+    for i in range(q):
+        for j in range(q):
+            for k in range(q):
+                for l in range(q):
+                    E = clock(i, j) + clock(j, l) + clock(l, k) + clock(k, i)
+                    A[i, j, k, l] = math.exp(-beta * E)
+    return A
+
+
+def _fourier(A, q):
+    """conj(F) on codomain legs, F on domain legs, F[x, a] = w^{xa}/sqrt(q);
+    real part of the Z_q-allowed entries."""
+    idx = np.arange(q)
+    F = np.exp(2j * np.pi * np.outer(idx, idx) / q) / math.sqrt(q)
+    U = np.einsum("wxyz,wa,xb,yc,zd->abcd", A.astype(complex), F.conj(), F.conj(), F, F)
+    lab = [idx, idx, idx, idx]
+    allowed = ((idx[:, None, None, None] + idx[None, :, None, None]
+                - idx[None, None, :, None] - idx[None, None, None, :]) % q) == 0
+    return SymTensor.from_dense(np.where(allowed, U.real, 0.0), lab, q)
+
+
+def classical_potts(q, beta=None, symmetry="ZN"):
+    """q-state Potts tensor (src/models/potts.jl:22-30,63-81)."""
+    b = potts_betac(q) if beta is None else float(beta)
+    A = _potts_dense(q, b)
+    if symmetry == "Trivial":
+        return trivial(A)
+    return _fourier(A, q)
+
+
+def classical_clock(q, beta, symmetry="ZN"):
+    """q-state clock tensor (src/models/clock.jl:1-12,26-49)."""
+    A = _clock_dense(q, float(beta))
+    if symmetry == "Trivial":
+        return trivial(A)
+    return _fourier(A, q)
