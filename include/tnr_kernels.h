@@ -17,13 +17,15 @@
 extern "C" {
 #endif
 
-/* C = alpha op(A) op(B) + beta C, column-major FP64; split32 != 0 uses the
- * FP32-MFMA hi/lo split kernel.  repeat >= 1 launches the product that many
- * times on device-resident copies; *ms_out (optional) receives the mean
- * device time per launch from HIP events. */
+/* C = alpha op(A) op(B) + beta C, column-major FP64.  mode: 0 = FP64 MFMA
+ * with the tile shape the engine would choose, 1 = FP32-MFMA hi/lo split
+ * kernel, 2 / 3 / 4 = FP64 MFMA with 64x64 / 128x64 / 128x128 tiles.
+ * repeat >= 1 launches the product that many times on device-resident
+ * copies; *ms_out (optional) receives the device time of the last launch
+ * from HIP events. */
 int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta,
                     const double* B, int ldb, int tb, double* C, int ldc,
-                    double alpha, double beta, int split32, int repeat,
+                    double alpha, double beta, int mode, int repeat,
                     double* ms_out);
 
 /* Top-k singular triplets of the m x n matrix M (column-major, ld = m):
@@ -39,9 +41,23 @@ int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam,
 
 /* Eigen-decomposition of `count` symmetric n x n matrices (n <= 96) by the
  * grouped parallel-order Jacobi kernel: W (n x n each), lam (n each), by
- * |lambda| descending; *sweeps receives the largest sweep count. */
+ * |lambda| descending; sweeps (count ints, optional) receives each sweep
+ * count and *ms_out (optional) the device time of the launch. */
 int tnr_kernel_jacobi(int count, int n, const double* G, double* W,
-                      double* lam, int* sweeps);
+                      double* lam, int* sweeps, double* ms_out);
+
+/* Device-time accounting per kernel class (HIP events around every launch
+ * on the launching stream).  tnr_profile_enable(1) clears and starts it,
+ * (0) stops it.  Classes: 0 FP64 GEMM, 1 split-FP32 GEMM, 2 split-K reduce,
+ * 3 Jacobi, 4 permute/copy, 5 column norms, 6 gather, 7 trace/normalise,
+ * 8 random fill, 9 other.  flops/bytes are the algorithmic work the
+ * launches were asked to do (GEMM: 2MNK summed over the grouped problems;
+ * copies: bytes read + written). */
+int tnr_profile_enable(int on);
+/* hipDeviceSynchronize() on the current device (timing brackets). */
+int tnr_device_sync(void);
+int tnr_profile_read(int kind, int64_t* launches, double* ms, double* flops,
+                     double* bytes);
 
 #ifdef __cplusplus
 }

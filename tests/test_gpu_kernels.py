@@ -40,6 +40,24 @@ def test_gemm_f64(tk, M, N, K, ta, tb):
     assert np.max(np.abs(got - ref)) <= 1e-14 * scale * 8
 
 
+@pytest.mark.parametrize("M,N,K,ta,tb", [
+    (1, 1, 1, 0, 0), (100, 37, 53, 1, 1), (300, 200, 70, 0, 1), (129, 257, 33, 1, 0),
+    (1024, 64, 1024, 0, 0), (64, 64, 4096, 1, 0), (1024, 1024, 32, 0, 0)])
+def test_gemm_tile_shapes_agree_bitwise(tk, M, N, K, ta, tb):
+    """Each output element is summed in the same k order under every tile
+    shape (64x64, 128x64, 128x128), so the three kernels agree bit for bit;
+    each also matches numpy to the FP64 GEMM bound."""
+    rng = np.random.default_rng(M + 11 * N + 101 * K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    Cm = rng.standard_normal((M, N))
+    outs = [gemm(tk, A, B, Cm, ta, tb, 0.9, -0.4, split=mode) for mode in (2, 3, 4, 0)]
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    ref = 0.9 * ((A.T if ta else A) @ (B.T if tb else B)) - 0.4 * Cm
+    assert np.max(np.abs(outs[0] - ref)) <= 1e-14 * (np.sqrt(K) + 1) * 8
+
+
 @pytest.mark.parametrize("M,N,K", [(128, 128, 256), (200, 100, 1000)])
 def test_gemm_split32(tk, M, N, K):
     rng = np.random.default_rng(K)
@@ -55,10 +73,11 @@ def test_gemm_split32(tk, M, N, K):
 def jacobi(tk, G):
     cnt, n = G.shape[0], G.shape[1]
     Gf = np.ascontiguousarray(np.stack([np.asfortranarray(g).ravel(order="F") for g in G]))
-    W = np.zeros((cnt, n * n)); lam = np.zeros((cnt, n)); sw = C.c_int()
-    st = tk.lib().tnr_kernel_jacobi(cnt, n, p(Gf), p(W), p(lam), C.byref(sw))
+    W = np.zeros((cnt, n * n)); lam = np.zeros((cnt, n)); sw = np.zeros(cnt, dtype=np.int32)
+    st = tk.lib().tnr_kernel_jacobi(cnt, n, p(Gf), p(W), p(lam),
+                                    sw.ctypes.data_as(C.POINTER(C.c_int)), None)
     assert st == 0, tk.lib().tnr_last_error()
-    return lam, np.stack([w.reshape(n, n, order="F") for w in W]), sw.value
+    return lam, np.stack([w.reshape(n, n, order="F") for w in W]), int(sw.max())
 
 
 @pytest.mark.parametrize("n", [1, 2, 5, 64, 95, 96])

@@ -101,3 +101,16 @@ def test_server_continuous_batching(tk):
         alone = gpu_run(tk, tk.BTRG, r.T, r.chi, r.maxiter)
         assert np.array_equal(res[0], alone)
     srv.close()
+
+
+def test_batch_mixed_symmetries_equals_single_runs(tk):
+    """Z2 Ising, Z3 Potts and Z6 clock BTRG runs stepped in the same grouped
+    launches give the same norms as each run alone."""
+    reqs = [tk.Request("BTRG", tk.classical_ising(0.44, symmetry="Z2"), 16, 8),
+            tk.Request("BTRG", tk.classical_potts(3, 1.0), 16, 8),
+            tk.Request("BTRG", tk.classical_clock(6, 1.1), 12, 8),
+            tk.Request("BTRG", tk.classical_ising(0.40, symmetry="Trivial"), 16, 8)]
+    batch = tk.run_batch(reqs)
+    for r, b in zip(reqs, batch):
+        single = gpu_run(tk, tk.BTRG, r.T, r.chi, r.maxiter)
+        assert rel(single, b) < 1e-12

@@ -90,14 +90,14 @@ BT::BT(Ctx& ctx, const Layout& L_) : L(L_), buf(&ctx.pool, std::max<int64_t>(L_.
 // ---------------------------------------------------------------- BondVec
 LegWeights BondVec::same_side() const {
     LegWeights w;
-    w.w = buf.d();
+    w.w = ptr();
     for (int q = 0; q < N; ++q) w.off[q] = off[q];
     return w;
 }
 
 LegWeights BondVec::dual_side() const {
     LegWeights w;
-    w.w = buf.d();
+    w.w = ptr();
     for (int q = 0; q < N; ++q) w.off[q] = off[(N - q) % N];
     return w;
 }
@@ -224,10 +224,68 @@ static void permute_descs(const BT& src, BT& dst, const std::vector<int>& perm, 
     });
 }
 
+// Sub-blocks whose source- and destination-contiguous dims differ (and are
+// both long enough) go to the tiled transpose; the rest to the element copy.
 static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t total) {
     if (total <= 0 || descs.empty()) return;
-    CopyDesc* dd = ctx.stage->put(ctx.stream, descs);
-    launch_copy(ctx.stream, dd, (int)descs.size(), total);
+    std::vector<CopyDesc> elem;
+    std::vector<TransDesc> tr;
+    int64_t estart = 0, tiles = 0, telems = 0;
+    for (const CopyDesc& d : descs) {
+        int s0 = -1, d0 = -1;
+        for (int j = 0; j < d.nd; ++j) {
+            if (d.dim[j] <= 1) continue;
+            if (d.sstr[j] == 1) s0 = j;
+            if (d.dstr[j] == 1) d0 = j;
+        }
+        const int64_t cnt = (int64_t)d.dim[0] * (d.nd > 1 ? d.dim[1] : 1) *
+                            (d.nd > 2 ? d.dim[2] : 1) * (d.nd > 3 ? d.dim[3] : 1) *
+                            (d.nd > 4 ? d.dim[4] : 1) * (d.nd > 5 ? d.dim[5] : 1);
+        const bool weighted = d.wleg[0] >= 0 || d.wleg[1] >= 0;
+        if (!weighted && s0 >= 0 && d0 >= 0 && s0 != d0 && d.dim[s0] >= 16 && d.dim[d0] >= 16 &&
+            d.nd - 2 <= 4) {
+            TransDesc t;
+            std::memset(&t, 0, sizeof(t));
+            t.src = d.src;
+            t.dst = d.dst;
+            t.n0 = d.dim[s0];
+            t.ds0 = d.dstr[s0];
+            t.n1 = d.dim[d0];
+            t.ss1 = d.sstr[d0];
+            int k = 0;
+            int64_t nb = 1;
+            for (int j = 0; j < d.nd; ++j) {
+                if (j == s0 || j == d0) continue;
+                t.bdim[k] = d.dim[j];
+                t.bs[k] = d.sstr[j];
+                t.bd[k] = d.dstr[j];
+                nb *= d.dim[j];
+                ++k;
+            }
+            for (; k < 4; ++k) t.bdim[k] = 1;
+            t.nb = (int32_t)nb;
+            t.t0 = (t.n0 + 63) / 64;
+            t.t1 = (t.n1 + 63) / 64;
+            t.tile_start = tiles;
+            tiles += (int64_t)t.t0 * t.t1 * nb;
+            telems += cnt;
+            tr.push_back(t);
+        } else {
+            CopyDesc e = d;
+            e.start = estart;
+            estart += cnt;
+            elem.push_back(e);
+        }
+    }
+    (void)total;
+    if (!tr.empty()) {
+        TransDesc* dt = ctx.stage->put(ctx.stream, tr);
+        launch_transpose(ctx.stream, dt, (int)tr.size(), tiles, telems);
+    }
+    if (!elem.empty()) {
+        CopyDesc* dd = ctx.stage->put(ctx.stream, elem);
+        launch_copy(ctx.stream, dd, (int)elem.size(), estart);
+    }
 }
 
 std::vector<BT> permute_many(Ctx& ctx, const std::vector<PermJob>& jobs) {
@@ -255,12 +313,59 @@ BT permute(Ctx& ctx, const BT& src, const std::vector<int>& perm, int ncod, bool
 }
 
 // -------------------------------------------------------------- grouped GEMM
-void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32) {
-    std::vector<GemmDesc> descs;
-    std::vector<int64_t> red_starts;
-    int64_t tiles = 0, red_total = 0;
+// The staging ring recycles a chunk only after synchronising the stream, so
+// the uploaded descriptors stay valid for every launch queued before the
+// next wrap-around.
+void DescArena::commit(Ctx& ctx) {
+    if (host_.empty()) return;
+    base_ = static_cast<char*>(ctx.stage->put(ctx.stream, host_.data(), host_.size()));
+}
+
+// Split-K factor from the problem's own shape (64 x 64 tile count), so a
+// problem is summed in the same order alone or inside any batch and under
+// any tile shape.
+static int split_k_of(const GemmJob& j) {
+    const int64_t mn = (int64_t)((j.M + 63) / 64) * ((j.N + 63) / 64);
+    const int ktiles = (j.K + GEMM_BK - 1) / GEMM_BK;
+    int ks = 1;
+    if (mn < 128 && ktiles >= 32) {
+        ks = (int)std::min<int64_t>(ktiles / 16, std::max<int64_t>(1, 256 / mn));
+        ks = std::max(1, std::min(ks, 64));
+    }
+    return ks;
+}
+
+// Tile shape of one grouped launch: the largest tile that still gives the
+// chip >= 4 workgroups per CU and pads the problems by at most 15 % more
+// than 64 x 64 tiles do.  Larger tiles halve (128 x 64) or quarter
+// (128 x 128) the operand bytes per flop the workgroups pull through L2.
+static int pick_shape(const std::vector<GemmJob>& jobs, bool split32) {
+    if (split32) return GEMM_64x64;
+    const int bm[3] = {64, 128, 128}, bn[3] = {64, 64, 128};
+    double pad[3] = {0, 0, 0};
+    int64_t tiles[3] = {0, 0, 0};
+    for (const GemmJob& j : jobs) {
+        if (j.M <= 0 || j.N <= 0) continue;
+        const int ks = split_k_of(j);
+        for (int s = 0; s < 3; ++s) {
+            const int64_t tm = (j.M + bm[s] - 1) / bm[s], tn = (j.N + bn[s] - 1) / bn[s];
+            tiles[s] += tm * tn * ks;
+            pad[s] += (double)(tm * bm[s]) * (double)(tn * bn[s]) * std::max(j.K, 1);
+        }
+    }
+    for (int s = 2; s >= 1; --s)
+        if (tiles[s] >= 1024 && pad[s] <= 1.15 * pad[0]) return s;
+    return GEMM_64x64;
+}
+
+GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape) {
+    GemmPlan P;
+    P.split32 = split32;
+    P.shape = (force_shape >= 0 && !split32) ? force_shape : pick_shape(jobs, split32);
+    const int BMs = P.shape == GEMM_64x64 ? 64 : 128;
+    const int BNs = P.shape == GEMM_128x128 ? 128 : 64;
     size_t work_elems = 0;
-    descs.reserve(jobs.size());
+    P.descs.reserve(jobs.size());
     for (const GemmJob& j : jobs) {
         if (j.M <= 0 || j.N <= 0) continue;
         TNR_REQUIRE(j.K >= 0 && j.lda >= 1 && j.ldb >= 1 && j.ldc >= j.M, TNR_ERR_INTERNAL,
@@ -272,49 +377,63 @@ void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32) {
         g.lda = j.lda; g.ldb = j.ldb; g.ldc = j.ldc;
         g.ta = j.ta; g.tb = j.tb;
         g.alpha = j.alpha; g.beta = j.beta;
-        g.tiles_m = (j.M + GEMM_BM - 1) / GEMM_BM;
-        g.tiles_n = (j.N + GEMM_BN - 1) / GEMM_BN;
-        // split K by the problem's own shape only (identical alone/batched)
-        int ks = 1;
+        g.skip = j.skip;
+        g.tiles_m = (j.M + BMs - 1) / BMs;
+        g.tiles_n = (j.N + BNs - 1) / BNs;
+        const int ks = split_k_of(j);
         const int64_t mn = (int64_t)g.tiles_m * g.tiles_n;
-        const int ktiles = (j.K + GEMM_BK - 1) / GEMM_BK;
-        if (mn < 128 && ktiles >= 32) {
-            ks = (int)std::min<int64_t>(ktiles / 16, std::max<int64_t>(1, 256 / mn));
-            ks = std::max(1, std::min(ks, 64));
-        }
         g.ksplit = ks;
-        g.tile_start = tiles;
-        tiles += mn * ks;
+        g.tile_start = P.tiles;
+        P.tiles += mn * ks;
         if (ks > 1) work_elems += (size_t)j.M * j.N * ks;
-        descs.push_back(g);
+        P.flops += 2.0 * j.M * j.N * j.K;
+        P.descs.push_back(g);
     }
-    if (descs.empty()) return;
-    Buf work;
     if (work_elems) {
-        work = Buf(&ctx.pool, work_elems * sizeof(double));
+        P.work = Buf(&ctx.pool, work_elems * sizeof(double));
         size_t o = 0;
-        for (auto& g : descs)
+        for (auto& g : P.descs)
             if (g.ksplit > 1) {
-                g.work = work.d() + o;
+                g.work = P.work.d() + o;
                 o += (size_t)g.M * g.N * g.ksplit;
+                P.red.push_back(g);
+                P.red_starts.push_back(P.red_total);
+                P.red_total += (int64_t)g.M * g.N;
             }
     }
-    GemmDesc* dd = ctx.stage->put(ctx.stream, descs);
-    launch_gemm(ctx.stream, dd, (int)descs.size(), tiles, split32);
-    if (work_elems) {
-        std::vector<GemmDesc> rd;
-        for (auto& g : descs)
-            if (g.ksplit > 1) {
-                rd.push_back(g);
-                red_starts.push_back(red_total);
-                red_total += (int64_t)g.M * g.N;
-            }
-        GemmDesc* drd = ctx.stage->put(ctx.stream, rd);
-        int64_t* dst = ctx.stage->put(ctx.stream, red_starts);
-        launch_gemm_reduce(ctx.stream, drd, (int)rd.size(), dst, red_total);
-    }
-    // `work` returns to the pool here; its next user is queued after the
-    // reduce on the same stream.
+    return P;
+}
+
+void gemm_stage(GemmPlan& p, DescArena& arena) {
+    p.off_descs = arena.add(p.descs);
+    p.off_red = arena.add(p.red);
+    p.off_starts = arena.add(p.red_starts);
+}
+
+void gemm_bind(GemmPlan& p, const DescArena& arena) {
+    p.d_descs = arena.at<GemmDesc>(p.off_descs);
+    p.d_red = arena.at<GemmDesc>(p.off_red);
+    p.d_starts = arena.at<int64_t>(p.off_starts);
+}
+
+void gemm_issue(Ctx& ctx, const GemmPlan& p) {
+    if (p.descs.empty()) return;
+    launch_gemm(ctx.stream, p.d_descs, (int)p.descs.size(), p.tiles, p.split32, p.flops,
+                p.shape);
+    if (!p.red.empty())
+        launch_gemm_reduce(ctx.stream, p.d_red, (int)p.red.size(), p.d_starts, p.red_total);
+}
+
+void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape) {
+    GemmPlan p = gemm_plan(ctx, jobs, split32, force_shape);
+    if (p.descs.empty()) return;
+    DescArena arena;
+    gemm_stage(p, arena);
+    arena.commit(ctx);
+    gemm_bind(p, arena);
+    gemm_issue(ctx, p);
+    // the arena and the split-K workspace return to the pool here; their
+    // next users are queued after these launches on the same stream
 }
 
 // --------------------------------------------------------------- contract

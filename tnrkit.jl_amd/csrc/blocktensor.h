@@ -14,6 +14,8 @@
 #pragma once
 
 #include <array>
+#include <memory>
+#include <cstring>
 #include <utility>
 #include <vector>
 
@@ -83,8 +85,10 @@ struct LegWeights {
 // A bond vector (singular values, BTRG bond matrix diagonal) stored per
 // sector label c of the side where the bond leg is outgoing with q = c.
 struct BondVec {
-    Buf buf;                         // device values
+    std::shared_ptr<Buf> store;      // device values (possibly shared by many bonds)
+    size_t base = 0;                 // first element of this bond in the store
     std::vector<double> host;        // same values on the host
+    const double* ptr() const { return store->d() + base; }
     int N = 1;
     int dim[MAXN] = {0};
     int off[MAXN] = {0};
@@ -140,8 +144,50 @@ struct GemmJob {
     double* C;
     int M, N, K, lda, ldb, ldc, ta, tb;
     double alpha = 1.0, beta = 0.0;
+    const int* skip = nullptr;       // device flag: problem finished
 };
-void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32 = false);
+void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32 = false,
+                  int force_shape = -1);
+
+// Descriptor arrays gathered on the host and uploaded to the device in one
+// copy (through the staging ring); every launch queued before the ring's
+// next wrap-around may read them, any number of times.
+class DescArena {
+  public:
+    template <class T> size_t add(const std::vector<T>& v) {
+        size_t off = (host_.size() + 255) & ~size_t(255);
+        host_.resize(off + v.size() * sizeof(T));
+        if (!v.empty()) std::memcpy(host_.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+    void commit(Ctx& ctx);
+    template <class T> T* at(size_t off) const { return reinterpret_cast<T*>(base_ + off); }
+
+  private:
+    std::vector<char> host_;
+    char* base_ = nullptr;
+};
+
+// A grouped GEMM prepared once and issued many times.
+struct GemmPlan {
+    std::vector<GemmDesc> descs, red;
+    std::vector<int64_t> red_starts;
+    int64_t tiles = 0, red_total = 0;
+    double flops = 0;
+    bool split32 = false;
+    int shape = 0;                     // GemmShape of the launch
+    Buf work;
+    size_t off_descs = 0, off_red = 0, off_starts = 0;
+    const GemmDesc* d_descs = nullptr;
+    const GemmDesc* d_red = nullptr;
+    const int64_t* d_starts = nullptr;
+};
+// force_shape >= 0 fixes the FP64 tile shape (GemmShape) instead of choosing it
+GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32 = false,
+                   int force_shape = -1);
+void gemm_stage(GemmPlan& p, DescArena& arena);          // before arena.commit
+void gemm_bind(GemmPlan& p, const DescArena& arena);     // after arena.commit
+void gemm_issue(Ctx& ctx, const GemmPlan& p);
 
 // n = |sum T[x1, x2, x2, x1] w1[x1] w2[x2]| (legs 1<->4, 2<->3), written to
 // the device slot; then T /= n, all on the device; grouped over tensors.

@@ -32,11 +32,11 @@ Buf up(Ctx& ctx, const double* h, size_t n) {
 extern "C" {
 
 int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta, const double* B,
-                    int ldb, int tb, double* C, int ldc, double alpha, double beta, int split32,
+                    int ldb, int tb, double* C, int ldc, double alpha, double beta, int mode,
                     int repeat, double* ms_out) {
     return kguard([&] {
-        TNR_REQUIRE(M > 0 && N > 0 && K >= 0 && A && B && C && repeat >= 1, TNR_ERR_ARG,
-                    "bad GEMM arguments");
+        TNR_REQUIRE(M > 0 && N > 0 && K >= 0 && A && B && C && repeat >= 1 && mode >= 0 &&
+                        mode <= 4, TNR_ERR_ARG, "bad GEMM arguments");
         TNR_REQUIRE(lda >= (ta ? std::max(K, 1) : M) && ldb >= (tb ? N : std::max(K, 1)) && ldc >= M,
                     TNR_ERR_ARG, "leading dimension too small");
         Ctx ctx;
@@ -55,7 +55,7 @@ int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta, const
             if (r > 0 && beta != 0.0)
                 TNR_HIP(hipMemcpyAsync(dC.d(), dC0.d(), nc * 8, hipMemcpyDeviceToDevice, ctx.stream));
             if (r == repeat - 1 || r == 0) TNR_HIP(hipEventRecord(e0, ctx.stream));
-            gemm_grouped(ctx, j, split32 != 0);
+            gemm_grouped(ctx, j, mode == 1, mode >= 2 ? mode - 2 : -1);
             if (r == repeat - 1 || r == 0) {
                 TNR_HIP(hipEventRecord(e1, ctx.stream));
                 TNR_HIP(hipEventSynchronize(e1));
@@ -112,7 +112,8 @@ int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam, int* 
     });
 }
 
-int tnr_kernel_jacobi(int count, int n, const double* G, double* W, double* lam, int* sweeps) {
+int tnr_kernel_jacobi(int count, int n, const double* G, double* W, double* lam, int* sweeps,
+                      double* ms_out) {
     return kguard([&] {
         TNR_REQUIRE(count > 0 && n > 0 && n <= LMAX && G && W && lam, TNR_ERR_ARG,
                     "bad Jacobi arguments");
@@ -126,12 +127,48 @@ int tnr_kernel_jacobi(int count, int n, const double* G, double* W, double* lam,
             jd.push_back(JacobiDesc{dG.d() + nn * i, dW.d() + nn * i, dL.d() + (size_t)n * i,
                                     dS.as<int>() + i, n, n, n});
         JacobiDesc* d = ctx.stage->put(ctx.stream, jd);
+        hipEvent_t e0, e1;
+        TNR_HIP(hipEventCreate(&e0));
+        TNR_HIP(hipEventCreate(&e1));
+        ctx.sync();
+        TNR_HIP(hipEventRecord(e0, ctx.stream));
         launch_jacobi(ctx.stream, d, count, n, solver_cfg().jacobi_tol, solver_cfg().jacobi_sweeps);
+        TNR_HIP(hipEventRecord(e1, ctx.stream));
+        TNR_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        TNR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms_out) *ms_out = ms;
         ctx.fetch(W, dW.d(), nn * count * 8);
         ctx.fetch(lam, dL.d(), (size_t)n * count * 8);
-        std::vector<int> sw(count);
-        ctx.fetch(sw.data(), dS.d(), count * sizeof(int));
-        if (sweeps) *sweeps = *std::max_element(sw.begin(), sw.end());
+        if (sweeps) ctx.fetch(sweeps, dS.d(), count * sizeof(int));
+        return TNR_OK;
+    });
+}
+
+int tnr_profile_enable(int on) {
+    return kguard([&] {
+        prof_enable(on != 0);
+        return TNR_OK;
+    });
+}
+
+int tnr_device_sync(void) {
+    return kguard([&] {
+        TNR_HIP(hipDeviceSynchronize());
+        return TNR_OK;
+    });
+}
+
+int tnr_profile_read(int kind, int64_t* launches, double* ms, double* flops, double* bytes) {
+    return kguard([&] {
+        TNR_REQUIRE(kind >= 0 && kind < PK_COUNT, TNR_ERR_ARG, "unknown kernel class");
+        ProfTotals t = prof_read(kind);
+        if (launches) *launches = t.launches;
+        if (ms) *ms = t.ms;
+        if (flops) *flops = t.flops;
+        if (bytes) *bytes = t.bytes;
         return TNR_OK;
     });
 }

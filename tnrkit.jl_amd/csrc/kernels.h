@@ -19,6 +19,21 @@ struct CopyDesc {
     int64_t start;                // prefix sum of element counts
 };
 
+// Tiled transpose job: a sub-block whose source-contiguous dim (extent n0,
+// source stride 1, destination stride ds0) differs from its
+// destination-contiguous dim (extent n1, destination stride 1, source
+// stride ss1); up to 4 further batch dims.  Handled in 64 x 64 LDS tiles.
+struct TransDesc {
+    const double* src;
+    double* dst;
+    int64_t ds0, ss1;
+    int64_t bs[4], bd[4];         // batch strides (src, dst)
+    int32_t bdim[4];
+    int32_t n0, n1, nb;           // nb = product of batch extents
+    int32_t t0, t1;               // tiles along n0, n1
+    int64_t tile_start;
+};
+
 // Matrix product job: C = alpha * op(A) * op(B) + beta * C, column-major.
 struct GemmDesc {
     const double* A;
@@ -32,6 +47,7 @@ struct GemmDesc {
     int32_t ksplit;               // >1: partial products into `work`
     int64_t tile_start;
     double* work;                 // ksplit partials, each M*N (ld = M)
+    const int* skip;              // optional: problem finished, do nothing
 };
 
 // Weighted partial trace sum_{x1,x2} T[x1*s1 + x2*s2] w1[x1] w2[x2] over one
@@ -61,6 +77,7 @@ struct JacobiDesc {
     double* lam;
     int* sweeps;
     int32_t n, ldg, ldw;
+    const int* skip;              // optional
 };
 
 // Column-norm / normalise job (also the Frobenius-norm job).
@@ -69,6 +86,21 @@ struct ColDesc {
     double* U;
     double* sigma;
     int32_t nrow, ncol, ldy, ldu;
+    const int* skip;              // optional
+};
+
+// Device-side stopping rule of the block iteration (one thread per problem):
+// with sigma the Ritz values of the current basis (descending), kk the
+// number of the first kmax above null_rtol * sigma_0 and
+// r = min(1, sigma_{l-1} / sigma_{kk-1})^(squared ? 2 : 1), iteration `it`
+// multiplies prod by r once it >= min_iters and sets done when
+// prod < target or it >= max_iters.
+struct ConvDesc {
+    const double* sigma;
+    int32_t l, kmax, squared;
+    int* it;
+    double* prod;
+    int* done;
 };
 
 // Deterministic uniform(-1,1) fill of a rows x cols column-major matrix.
@@ -89,8 +121,13 @@ struct GatherDesc {
     int64_t start;
 };
 
-void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t total);
-void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32);
+void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t total);  // bytes = 16*total
+void launch_transpose(hipStream_t st, const TransDesc* d_descs, int nd, int64_t tiles,
+                      int64_t elems);
+// tile shapes of the FP64 GEMM kernel (the split-FP32 kernel is 64 x 64)
+enum GemmShape { GEMM_64x64 = 0, GEMM_128x64 = 1, GEMM_128x128 = 2 };
+void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32,
+                 double flops, int shape);
 void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const int64_t* d_starts,
                         int64_t total);
 void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const TraceProb* d_probs,
@@ -100,11 +137,32 @@ void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, 
                    int max_sweeps);
 void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, double rtol);
 void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax);
+void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rtol, double target,
+                 int min_iters, int max_iters);
 void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd);
 void launch_randfill(hipStream_t st, const FillDesc* d_descs, int nd, int64_t total);
 void launch_gather(hipStream_t st, const GatherDesc* d_descs, int nd, int64_t total);
 
-// GEMM tile geometry shared with the host planner.
-constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 16;
+// ------------------------------------------------------------- profiling
+// Optional per-kernel-class device timing (HIP events on the launching
+// stream) with the algorithmic flops / bytes each launch was asked to do.
+enum ProfKind {
+    PK_GEMM_F64 = 0, PK_GEMM_SPLIT32, PK_GEMM_REDUCE, PK_JACOBI, PK_COPY, PK_COLNORM,
+    PK_GATHER, PK_TRACE, PK_FILL, PK_MISC, PK_COUNT
+};
+void prof_enable(bool on);
+bool prof_enabled();
+struct ProfScope {
+    ProfScope(hipStream_t st, int kind, double flops, double bytes);
+    ~ProfScope();
+    hipStream_t st;
+    int kind, slot;
+    double flops, bytes;
+};
+struct ProfTotals { int64_t launches = 0; double ms = 0, flops = 0, bytes = 0; };
+ProfTotals prof_read(int kind);   // synchronises the device
+
+// GEMM k-tile depth shared with the host planner.
+constexpr int GEMM_BK = 16;
 
 }  // namespace tnr

@@ -9,6 +9,9 @@
 // run and alone vs batched.
 #include "kernels.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace tnr {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -53,14 +56,59 @@ __global__ __launch_bounds__(256) void k_copy(const CopyDesc* __restrict__ descs
     }
 }
 
+// ------------------------------------------------------- tiled transpose
+// One 64 x 64 tile per workgroup iteration: 256 threads read 4 rows of 64
+// consecutive source elements per pass (coalesced), and write 4 rows of 64
+// consecutive destination elements per pass (coalesced) through LDS.
+__global__ __launch_bounds__(256) void k_transpose(const TransDesc* __restrict__ descs, int nd,
+                                                   int64_t tiles) {
+    __shared__ double tile[64][65];
+    const int t = threadIdx.x, x = t & 63, y = t >> 6;
+    for (int64_t tid = blockIdx.x; tid < tiles; tid += gridDim.x) {
+        int lo = 0, hi = nd - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (descs[mid].tile_start <= tid) lo = mid; else hi = mid - 1;
+        }
+        const TransDesc& D = descs[lo];
+        int64_t r = tid - D.tile_start;
+        const int i0 = (int)(r % D.t0) * 64;
+        r /= D.t0;
+        const int i1 = (int)(r % D.t1) * 64;
+        r /= D.t1;
+        int64_t so = 0, dof = 0;
+        for (int k = 0; k < 4; ++k) {
+            if (D.bdim[k] <= 1) continue;
+            const int64_t ik = r % D.bdim[k];
+            r /= D.bdim[k];
+            so += ik * D.bs[k];
+            dof += ik * D.bd[k];
+        }
+        const double* src = D.src + so;
+        double* dst = D.dst + dof;
+        __syncthreads();
+        for (int j = y; j < 64; j += 4) {          // j along n1, x along n0
+            const int a = i0 + x, b = i1 + j;
+            if (a < D.n0 && b < D.n1) tile[j][x] = src[a + (int64_t)b * D.ss1];
+        }
+        __syncthreads();
+        for (int j = y; j < 64; j += 4) {          // j along n0, x along n1
+            const int a = i0 + j, b = i1 + x;
+            if (a < D.n0 && b < D.n1) dst[b + (int64_t)a * D.ds0] = tile[x][j];
+        }
+    }
+}
+
 // --------------------------------------------------------------- FP64 GEMM
-// 64x64 output tile per 256-thread workgroup; 4 waves as 2x2 of 32x32; each
-// wave issues 2x2 v_mfma_f64_16x16x4_f64 per k-step of 4.  Operand lane
+// Output tile BM x BN per 256-thread workgroup; 4 waves as 2 x 2, each wave
+// owning (BM/2) x (BN/2) = MI x NI blocks of v_mfma_f64_16x16x4_f64 per
+// k-step of 4.  The k order of every output element is the same for all
+// tile shapes, so results do not depend on the shape chosen.  Operand lane
 // layout of the 16x16x4 MFMAs (measured on gfx950): A[i=l%16][k=l/16],
 // B[k=l/16][j=l%16]; result register r of lane l holds
 //   f64: D[i = l/16 + 4r][j = l%16]   (rows interleaved)
 //   f32: D[i = 4(l/16) + r][j = l%16] (rows contiguous).
-constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int BK = 16;
 
 __device__ __forceinline__ int find_desc_gemm(const GemmDesc* d, int n, int64_t t) {
     int lo = 0, hi = n - 1;
@@ -75,6 +123,7 @@ struct TileCoord {
     int m0, n0, kbeg, kend, kz;
 };
 
+template <int BM, int BN>
 __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local) {
     TileCoord c;
     int tm = (int)(local % g.tiles_m);
@@ -91,86 +140,97 @@ __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local
     return c;
 }
 
-__device__ __forceinline__ void load_a(const GemmDesc& g, int m0, int k0, int t, double (&ra)[4]) {
-    if (!g.ta) {  // A[i + k*lda], M x K
-        int i = m0 + (t & 63), kk = t >> 6;
+// global -> registers: the tile's BM x BK slab of op(A) (RA values per thread)
+template <int BM>
+__device__ __forceinline__ void load_a(const GemmDesc& g, int m0, int k0, int t,
+                                       double (&ra)[BM / 16]) {
+    constexpr int R = BM / 16;
+    if (!g.ta) {  // A[i + k*lda], M x K: thread -> (i, k-group)
+        const int i = m0 + (t % BM), kk = t / BM;      // kk < 256/BM
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int k = k0 + kk + 4 * r;
+        for (int r = 0; r < R; ++r) {
+            const int k = k0 + kk + (256 / BM) * r;
             ra[r] = (i < g.M && k < g.K) ? g.A[(int64_t)i + (int64_t)k * g.lda] : 0.0;
         }
     } else {      // stored K x M: A[k + i*lda]
-        int kk = t & 15, i0 = t >> 4;
+        const int kk = t & 15, i0 = t >> 4;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int i = m0 + i0 + 16 * r, k = k0 + kk;
+        for (int r = 0; r < R; ++r) {
+            const int i = m0 + i0 + 16 * r, k = k0 + kk;
             ra[r] = (i < g.M && k < g.K) ? g.A[(int64_t)k + (int64_t)i * g.lda] : 0.0;
         }
     }
 }
 
-__device__ __forceinline__ void store_a(const GemmDesc& g, int t, const double (&ra)[4],
+template <int BM>
+__device__ __forceinline__ void store_a(const GemmDesc& g, int t, const double (&ra)[BM / 16],
                                         double (*As)[BM + 1]) {
+    constexpr int R = BM / 16;
     if (!g.ta) {
-        int i = t & 63, kk = t >> 6;
+        const int i = t % BM, kk = t / BM;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) As[kk + 4 * r][i] = ra[r];
+        for (int r = 0; r < R; ++r) As[kk + (256 / BM) * r][i] = ra[r];
     } else {
-        int kk = t & 15, i0 = t >> 4;
+        const int kk = t & 15, i0 = t >> 4;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) As[kk][i0 + 16 * r] = ra[r];
+        for (int r = 0; r < R; ++r) As[kk][i0 + 16 * r] = ra[r];
     }
 }
 
-__device__ __forceinline__ void load_b(const GemmDesc& g, int n0, int k0, int t, double (&rb)[4]) {
+template <int BN>
+__device__ __forceinline__ void load_b(const GemmDesc& g, int n0, int k0, int t,
+                                       double (&rb)[BN / 16]) {
+    constexpr int R = BN / 16;
     if (!g.tb) {  // B[k + j*ldb], K x N
-        int kk = t & 15, j0 = t >> 4;
+        const int kk = t & 15, j0 = t >> 4;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int j = n0 + j0 + 16 * r, k = k0 + kk;
+        for (int r = 0; r < R; ++r) {
+            const int j = n0 + j0 + 16 * r, k = k0 + kk;
             rb[r] = (j < g.N && k < g.K) ? g.B[(int64_t)k + (int64_t)j * g.ldb] : 0.0;
         }
     } else {      // stored N x K: B[j + k*ldb]
-        int j = n0 + (t & 63), kk = t >> 6;
+        const int j = n0 + (t % BN), kk = t / BN;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int k = k0 + kk + 4 * r;
+        for (int r = 0; r < R; ++r) {
+            const int k = k0 + kk + (256 / BN) * r;
             rb[r] = (j < g.N && k < g.K) ? g.B[(int64_t)j + (int64_t)k * g.ldb] : 0.0;
         }
     }
 }
 
-__device__ __forceinline__ void store_b(const GemmDesc& g, int t, const double (&rb)[4],
+template <int BN>
+__device__ __forceinline__ void store_b(const GemmDesc& g, int t, const double (&rb)[BN / 16],
                                         double (*Bs)[BN + 1]) {
+    constexpr int R = BN / 16;
     if (!g.tb) {
-        int kk = t & 15, j0 = t >> 4;
+        const int kk = t & 15, j0 = t >> 4;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Bs[kk][j0 + 16 * r] = rb[r];
+        for (int r = 0; r < R; ++r) Bs[kk][j0 + 16 * r] = rb[r];
     } else {
-        int j = t & 63, kk = t >> 6;
+        const int j = t % BN, kk = t / BN;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Bs[kk + 4 * r][j] = rb[r];
+        for (int r = 0; r < R; ++r) Bs[kk + (256 / BN) * r][j] = rb[r];
     }
 }
 
-template <bool F64_LAYOUT>
+template <bool F64_LAYOUT, int MI, int NI>
 __device__ __forceinline__ void write_tile(const GemmDesc& g, const TileCoord& tc, int lane,
-                                           int wm, int wn, const dbl4 (&acc)[2][2]) {
+                                           int wm0, int wn0, const dbl4 (&acc)[MI][NI]) {
     const bool part = g.ksplit > 1;
     double* out = part ? g.work + (int64_t)tc.kz * g.M * g.N : g.C;
     const int64_t ld = part ? g.M : g.ldc;
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
+    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-            int col = tc.n0 + wn * 32 + ni * 16 + (lane & 15);
+        for (int ni = 0; ni < NI; ++ni) {
+            const int col = tc.n0 + wn0 + ni * 16 + (lane & 15);
             if (col >= g.N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int ri = F64_LAYOUT ? (lane >> 4) + 4 * r : (lane >> 4) * 4 + r;
-                int row = tc.m0 + wm * 32 + mi * 16 + ri;
+                const int row = tc.m0 + wm0 + mi * 16 + ri;
                 if (row >= g.M) continue;
-                double v = acc[mi][ni][r];
+                const double v = acc[mi][ni][r];
                 double* p = out + (int64_t)row + (int64_t)col * ld;
                 if (part) *p = v;
                 else *p = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * *p;
@@ -178,45 +238,52 @@ __device__ __forceinline__ void write_tile(const GemmDesc& g, const TileCoord& t
         }
 }
 
+template <int BM, int BN>
 __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ descs, int nd) {
+    constexpr int MI = BM / 32, NI = BN / 32;
     __shared__ double As[BK][BM + 1];
     __shared__ double Bs[BK][BN + 1];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w & 1, wn = w >> 1;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm0 = (w & 1) * (BM / 2), wn0 = (w >> 1) * (BN / 2);
     const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
-    const TileCoord tc = tile_coord(g, (int64_t)blockIdx.x - g.tile_start);
-    dbl4 acc[2][2];
+    if (g.skip && *g.skip) return;
+    const TileCoord tc = tile_coord<BM, BN>(g, (int64_t)blockIdx.x - g.tile_start);
+    dbl4 acc[MI][NI];
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
+    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
-    double ra[4], rb[4];
+        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+    double ra[BM / 16], rb[BN / 16];
     if (tc.kbeg < tc.kend) {
-        load_a(g, tc.m0, tc.kbeg, t, ra);
-        load_b(g, tc.n0, tc.kbeg, t, rb);
+        load_a<BM>(g, tc.m0, tc.kbeg, t, ra);
+        load_b<BN>(g, tc.n0, tc.kbeg, t, rb);
     }
     for (int k0 = tc.kbeg; k0 < tc.kend; k0 += BK) {
-        store_a(g, t, ra, As);
-        store_b(g, t, rb, Bs);
+        store_a<BM>(g, t, ra, As);
+        store_b<BN>(g, t, rb, Bs);
         __syncthreads();
         if (k0 + BK < tc.kend) {
-            load_a(g, tc.m0, k0 + BK, t, ra);
-            load_b(g, tc.n0, k0 + BK, t, rb);
+            load_a<BM>(g, tc.m0, k0 + BK, t, ra);
+            load_b<BN>(g, tc.n0, k0 + BK, t, rb);
         }
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int k = ks * 4 + (lane >> 4);
-            double a0 = As[k][wm * 32 + (lane & 15)];
-            double a1 = As[k][wm * 32 + 16 + (lane & 15)];
-            double b0 = Bs[k][wn * 32 + (lane & 15)];
-            double b1 = Bs[k][wn * 32 + 16 + (lane & 15)];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            double a[MI], b[NI];
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi) a[mi] = As[k][wm0 + mi * 16 + (lane & 15)];
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni) b[ni] = Bs[k][wn0 + ni * 16 + (lane & 15)];
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[ni], acc[mi][ni],
+                                                                       0, 0, 0);
         }
         __syncthreads();
     }
-    write_tile<true>(g, tc, lane, wm, wn, acc);
+    write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, acc);
 }
 
 // ------------------------------------------------ FP32 MFMA, hi/lo split
@@ -225,11 +292,12 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
 // (v_mfma_f32_16x16x4_f32) in separate accumulators, then flushed into FP64
 // registers, so FP32 rounding only acts within one 16-deep k-tile.
 __global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict__ descs, int nd) {
-    __shared__ float Ah[BK][BM + 1], Al[BK][BM + 1];
-    __shared__ float Bh[BK][BN + 1], Bl[BK][BN + 1];
+    __shared__ float Ah[BK][65], Al[BK][65];
+    __shared__ float Bh[BK][65], Bl[BK][65];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w & 1, wn = w >> 1;
     const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
-    const TileCoord tc = tile_coord(g, (int64_t)blockIdx.x - g.tile_start);
+    if (g.skip && *g.skip) return;
+    const TileCoord tc = tile_coord<64, 64>(g, (int64_t)blockIdx.x - g.tile_start);
     dbl4 acc[2][2];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
@@ -237,8 +305,8 @@ __global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict
         for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
     double ra[4], rb[4];
     for (int k0 = tc.kbeg; k0 < tc.kend; k0 += BK) {
-        load_a(g, tc.m0, k0, t, ra);
-        load_b(g, tc.n0, k0, t, rb);
+        load_a<64>(g, tc.m0, k0, t, ra);
+        load_b<64>(g, tc.n0, k0, t, rb);
         // split and stage
         {
             float ah[4], al[4], bh[4], bl[4];
@@ -298,7 +366,7 @@ __global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict
                     acc[mi][ni][r] += (double)hh[mi][ni][r] + (double)cr[mi][ni][r];
         __syncthreads();
     }
-    write_tile<false>(g, tc, lane, wm, wn, acc);
+    write_tile<false, 2, 2>(g, tc, lane, wm * 32, wn * 32, acc);
 }
 
 // sum ksplit partials in fixed order, apply alpha/beta
@@ -313,6 +381,7 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const GemmDesc* __restrict_
             if (starts[mid] <= e) lo = mid; else hi = mid - 1;
         }
         const GemmDesc& g = descs[lo];
+        if (g.skip && *g.skip) continue;
         int64_t r = e - starts[lo];
         int64_t mn = (int64_t)g.M * g.N;
         double s = 0.0;
@@ -379,114 +448,136 @@ __global__ __launch_bounds__(256) void k_normalize(const TraceProb* __restrict__
 }
 
 // ------------------------------------------------------------- Jacobi eigen
-// One workgroup per symmetric n x n matrix (n <= LMAX).  Parallel cyclic
-// Jacobi with the circle-method tournament ordering: each round rotates n/2
-// disjoint pairs (p,q) simultaneously, G <- J^T G J, W <- W J.  A pair is
-// skipped when |g_pq| <= tol sqrt(|g_pp g_qq|) (relative criterion, which
-// keeps small eigenvalues of graded Gram matrices relatively accurate).  The
-// rotated 2x2 block is written from the closed form (g_pp - t g_pq,
-// g_qq + t g_pq, 0).  The input is symmetrised on load ((G + G^T)/2).
-// Output: eigenvalues by |lambda| descending, eigenvectors as columns of W
-// in the same order.
-__global__ __launch_bounds__(256) void k_jacobi(const JacobiDesc* __restrict__ descs, int nd,
-                                                double tol, int max_sweeps) {
+// One 1024-thread workgroup per symmetric n x n matrix (n <= LMAX).
+// Parallel cyclic Jacobi with the circle-method tournament ordering: each
+// round rotates the n/2 disjoint pairs (p_i, q_i) simultaneously,
+// G <- J^T G J, W <- W J.  A pair is rotated only while
+// |g_pq| > tol sqrt(|g_pp g_qq|) (relative criterion, which keeps the small
+// eigenvalues of graded Gram matrices relatively accurate).  A round is two
+// phases: (1) one thread per pair computes its rotation (the tournament
+// positions advance in registers); (2) every 2x2 block
+// B_ij = G[{p_i,q_i},{p_j,q_j}] becomes J_i^T B_ij J_j (diagonal blocks from
+// the closed form g_pp - t g_pq, g_qq + t g_pq, 0) and every row of W is
+// rotated.  A sweep starts with a parallel scan of all off-diagonal pairs and
+// the solve stops when none violates the criterion.  The input is
+// symmetrised on load ((G + G^T)/2).  Output: eigenvalues by |lambda|
+// descending, eigenvectors as columns of W in the same order.
+constexpr int JT = 1024;
+
+__global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ descs, int nd,
+                                               double tol, int max_sweeps) {
     extern __shared__ double lds[];
     const JacobiDesc D = descs[blockIdx.x];
+    if (D.skip && *D.skip) return;
     const int n = D.n;
     const int np = n + (n & 1);
-    double* G = lds;                 // np x np, column-major (G[i + j*np])
-    double* W = lds + np * np;
-    double* cs = W + np * np;        // c, s, t per pair (3 * np/2)
-    double* g0 = cs + 3 * (LMAX / 2);   // gpp, gqq, gpq per pair
-    int* act = (int*)(g0 + 3 * (LMAX / 2));
-    int* anyrot = act + LMAX / 2;
-    const int t = threadIdx.x, nt = blockDim.x;
-    for (int e = t; e < np * np; e += nt) {
-        int i = e % np, j = e / np;
-        G[e] = (i < n && j < n)
-                   ? 0.5 * (D.G[i + (int64_t)j * D.ldg] + D.G[j + (int64_t)i * D.ldg])
-                   : 0.0;
-        W[e] = (i == j) ? 1.0 : 0.0;
+    const int ld = np + 1;              // padded column stride (bank spread)
+    double* G = lds;                    // np x np, G[i + j*ld]
+    double* W = G + np * ld;
+    double* rc = W + np * ld;           // c, s, t per pair
+    double* rs = rc + LMAX / 2;
+    double* rt = rs + LMAX / 2;
+    int* P = (int*)(rt + LMAX / 2);     // pair rows
+    int* Q = P + LMAX / 2;
+    int* act = Q + LMAX / 2;
+    int* pos = act + LMAX / 2;          // output order (LMAX ints)
+    const int t = threadIdx.x;
+    const float inv_np = 1.0f / (float)np;
+    for (int e = t; e < np * np; e += JT) {
+        const int j = (int)(((float)e + 0.5f) * inv_np), i = e - j * np;
+        G[i + j * ld] = (i < n && j < n)
+                            ? 0.5 * (D.G[i + (int64_t)j * D.ldg] + D.G[j + (int64_t)i * D.ldg])
+                            : 0.0;
+        W[i + j * ld] = (i == j) ? 1.0 : 0.0;
     }
-    __syncthreads();
     const int half = np / 2, m = np - 1;
+    const float inv_half = 1.0f / (float)half;
     int sweeps = 0;
     for (int sw = 0; sw < max_sweeps; ++sw) {
-        if (t == 0) *anyrot = 0;
-        __syncthreads();
+        // convergence scan over all pairs p < q
+        int viol = 0;
+        for (int e = t; e < np * np; e += JT) {
+            const int q = (int)(((float)e + 0.5f) * inv_np), p = e - q * np;
+            if (p < q) {
+                const double gpq = G[p + q * ld];
+                viol |= (gpq != 0.0) &&
+                        (fabs(gpq) > tol * sqrt(fabs(G[p + p * ld] * G[q + q * ld])));
+            }
+        }
+        if (!__syncthreads_or(viol)) break;
+        ++sweeps;
+        int a = t, b = np - 1 - t;      // tournament positions of pair t, round 0
         for (int rnd = 0; rnd < m; ++rnd) {
+            int on = 0;
             if (t < half) {
-                int a = (t == 0) ? 0 : 1 + ((t - 1 + rnd) % m);
-                int x = np - 1 - t;
-                int b = 1 + ((x - 1 + rnd) % m);
-                int p = min(a, b), q = max(a, b);
-                double gpp = G[p + p * np], gqq = G[q + q * np], gpq = G[p + q * np];
-                int on = (gpq != 0.0) && (fabs(gpq) > tol * sqrt(fabs(gpp * gqq)));
+                const int p = min(a, b), q = max(a, b);
+                const double gpp = G[p + p * ld], gqq = G[q + q * ld], gpq = G[p + q * ld];
+                on = (gpq != 0.0) && (fabs(gpq) > tol * sqrt(fabs(gpp * gqq)));
                 double c = 1.0, s = 0.0, tt = 0.0;
                 if (on) {
-                    double tau = (gqq - gpp) / (2.0 * gpq);
+                    const double tau = (gqq - gpp) / (2.0 * gpq);
                     tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                     c = 1.0 / sqrt(1.0 + tt * tt);
                     s = tt * c;
-                    *anyrot = 1;
                 }
-                act[t] = on;
-                cs[3 * t] = c; cs[3 * t + 1] = s; cs[3 * t + 2] = tt;
-                g0[3 * t] = gpp; g0[3 * t + 1] = gqq; g0[3 * t + 2] = gpq;
+                P[t] = p; Q[t] = q; act[t] = on;
+                rc[t] = c; rs[t] = s; rt[t] = tt;
+                if (t > 0) a = (a == m) ? 1 : a + 1;
+                b = (b == m) ? 1 : b + 1;
             }
-            __syncthreads();
-            // rows p, q of G (all columns)
-            for (int e = t; e < half * np; e += nt) {
-                int i = e / np, j = e % np;
-                if (!act[i]) continue;
-                int a = (i == 0) ? 0 : 1 + ((i - 1 + rnd) % m);
-                int b = 1 + ((np - 1 - i - 1 + rnd) % m);
-                int p = min(a, b), q = max(a, b);
-                double c = cs[3 * i], s = cs[3 * i + 1];
-                double gp = G[p + j * np], gq = G[q + j * np];
-                G[p + j * np] = c * gp - s * gq;
-                G[q + j * np] = s * gp + c * gq;
-            }
-            __syncthreads();
-            // columns p, q of G and W (all rows); closed form on the 2x2 block
-            for (int e = t; e < half * np; e += nt) {
-                int i = e / np, j = e % np;
-                if (!act[i]) continue;
-                int a = (i == 0) ? 0 : 1 + ((i - 1 + rnd) % m);
-                int b = 1 + ((np - 1 - i - 1 + rnd) % m);
-                int p = min(a, b), q = max(a, b);
-                double c = cs[3 * i], s = cs[3 * i + 1];
-                if (j == p || j == q) {
-                    double tt = cs[3 * i + 2];
-                    double gpp = g0[3 * i], gqq = g0[3 * i + 1], gpq = g0[3 * i + 2];
-                    if (j == p) { G[p + p * np] = gpp - tt * gpq; G[q + p * np] = 0.0; }
-                    else        { G[q + q * np] = gqq + tt * gpq; G[p + q * np] = 0.0; }
-                } else {
-                    double gp = G[j + p * np], gq = G[j + q * np];
-                    G[j + p * np] = c * gp - s * gq;
-                    G[j + q * np] = s * gp + c * gq;
+            if (!__syncthreads_or(on)) continue;
+            for (int e = t; e < half * half; e += JT) {
+                const int j = (int)(((float)e + 0.5f) * inv_half), i = e - j * half;
+                const int ai = act[i], aj = act[j];
+                if (!ai && !aj) continue;
+                const int pi = P[i], qi = Q[i];
+                if (i == j) {
+                    const double tt = rt[i], gpq = G[pi + qi * ld];
+                    G[pi + pi * ld] -= tt * gpq;
+                    G[qi + qi * ld] += tt * gpq;
+                    G[pi + qi * ld] = 0.0;
+                    G[qi + pi * ld] = 0.0;
+                    continue;
                 }
-                double wp = W[j + p * np], wq = W[j + q * np];
-                W[j + p * np] = c * wp - s * wq;
-                W[j + q * np] = s * wp + c * wq;
+                const int pj = P[j], qj = Q[j];
+                const double ci = rc[i], si = rs[i], cj = rc[j], sj = rs[j];
+                const double b00 = G[pi + pj * ld], b01 = G[pi + qj * ld];
+                const double b10 = G[qi + pj * ld], b11 = G[qi + qj * ld];
+                const double x00 = ci * b00 - si * b10, x01 = ci * b01 - si * b11;   // J_i^T B
+                const double x10 = si * b00 + ci * b10, x11 = si * b01 + ci * b11;
+                G[pi + pj * ld] = cj * x00 - sj * x01;                               // (.) J_j
+                G[pi + qj * ld] = sj * x00 + cj * x01;
+                G[qi + pj * ld] = cj * x10 - sj * x11;
+                G[qi + qj * ld] = sj * x10 + cj * x11;
+            }
+            for (int e = t; e < half * np; e += JT) {
+                const int j = (int)(((float)e + 0.5f) * inv_np), r = e - j * np;
+                if (!act[j]) continue;
+                const int pj = P[j], qj = Q[j];
+                const double cj = rc[j], sj = rs[j];
+                const double wp = W[r + pj * ld], wq = W[r + qj * ld];
+                W[r + pj * ld] = cj * wp - sj * wq;
+                W[r + qj * ld] = sj * wp + cj * wq;
             }
             __syncthreads();
         }
-        ++sweeps;
-        int done = (*anyrot == 0);
-        __syncthreads();
-        if (done) break;
     }
-    // sort by |lambda| descending (stable by index) and write out
-    for (int i = t; i < n; i += nt) {
-        double li = G[i + i * np], ai = fabs(li);
-        int pos = 0;
+    // order by |lambda| descending (stable by index) and write out
+    if (t < n) {
+        const double li = G[t + t * ld], ai = fabs(li);
+        int k = 0;
         for (int j = 0; j < n; ++j) {
-            double aj = fabs(G[j + j * np]);
-            pos += (aj > ai) || (aj == ai && j < i);
+            const double aj = fabs(G[j + j * ld]);
+            k += (aj > ai) || (aj == ai && j < t);
         }
-        D.lam[pos] = li;
-        for (int r = 0; r < n; ++r) D.W[r + (int64_t)pos * D.ldw] = W[r + i * np];
+        pos[t] = k;
+        D.lam[k] = li;
+    }
+    __syncthreads();
+    const float inv_n = 1.0f / (float)n;
+    for (int e = t; e < n * n; e += JT) {
+        const int i = (int)(((float)e + 0.5f) * inv_n), r = e - i * n;
+        D.W[r + (int64_t)pos[i] * D.ldw] = W[r + i * ld];
     }
     if (t == 0 && D.sweeps) *D.sweeps = sweeps;
 }
@@ -494,20 +585,21 @@ __global__ __launch_bounds__(256) void k_jacobi(const JacobiDesc* __restrict__ d
 // ------------------------------------------------- column norms / normalise
 // sigma_j = ||Y[:, j]||; U[:, j] = Y[:, j] / sigma_j, or 0 when
 // sigma_j <= rtol * max_j sigma_j (numerical null direction).  One
-// workgroup per problem; wave w owns columns j = w (mod 4) and reduces with a
-// fixed butterfly, so the result does not depend on scheduling.
+// workgroup per problem; wave w owns columns j = w (mod 16) and reduces with
+// a fixed butterfly, so the result does not depend on scheduling.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_colnorm(const ColDesc* __restrict__ descs, int nd,
-                                                 double rtol) {
+__global__ __launch_bounds__(1024) void k_colnorm(const ColDesc* __restrict__ descs, int nd,
+                                                  double rtol) {
     __shared__ double sig[LMAX];
     const ColDesc D = descs[blockIdx.x];
+    if (D.skip && *D.skip) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    for (int j = w; j < D.ncol; j += 4) {
+    for (int j = w; j < D.ncol; j += 16) {
         const double* y = D.Y + (int64_t)j * D.ldy;
         double acc = 0.0;
         for (int i = lane; i < D.nrow; i += 64) acc += y[i] * y[i];
@@ -517,7 +609,7 @@ __global__ __launch_bounds__(256) void k_colnorm(const ColDesc* __restrict__ des
     __syncthreads();
     double smax = 0.0;
     for (int j = 0; j < D.ncol; ++j) smax = fmax(smax, sig[j]);
-    for (int j = w; j < D.ncol; j += 4) {
+    for (int j = w; j < D.ncol; j += 16) {
         const double sj = sig[j];
         const bool live = sj > rtol * smax && sj > 0.0;
         const double inv = live ? 1.0 / sj : 0.0;
@@ -526,6 +618,31 @@ __global__ __launch_bounds__(256) void k_colnorm(const ColDesc* __restrict__ des
         for (int i = lane; i < D.nrow; i += 64) u[i] = y[i] * inv;
         if (lane == 0) D.sigma[j] = live ? sj : 0.0;
     }
+}
+
+// stopping rule of the block iteration, one thread per problem
+__global__ void k_conv(const ConvDesc* __restrict__ descs, int nd, double null_rtol,
+                       double target, int min_iters, int max_iters) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nd) return;
+    const ConvDesc D = descs[p];
+    if (*D.done) return;
+    const int it = *D.it + 1;
+    const double* sv = D.sigma;
+    int kk = 0;
+    for (int i = 0; i < D.kmax; ++i)
+        if (sv[i] > null_rtol * sv[0]) kk = i + 1;
+    double r = 0.0;
+    if (kk > 0 && sv[kk - 1] > 0.0) {
+        r = sv[D.l - 1] / sv[kk - 1];
+        if (D.squared) r *= r;
+        r = fmin(r, 1.0);
+    }
+    double prod = *D.prod;
+    if (it >= min_iters) prod *= r;
+    *D.it = it;
+    *D.prod = prod;
+    if ((it >= min_iters && prod < target) || it >= max_iters) *D.done = 1;
 }
 
 // in-place (M + M^T)/2 of square blocks (project_hermitian)
@@ -603,6 +720,72 @@ __global__ __launch_bounds__(256) void k_frob(const ColDesc* __restrict__ descs,
     if (threadIdx.x == 0) D.sigma[0] = s;
 }
 
+// ================================================================ profiling
+namespace {
+struct ProfState {
+    bool on = false;
+    struct Rec { hipEvent_t a, b; int kind; double flops, bytes; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> spare;
+    ProfTotals tot[PK_COUNT];
+    hipEvent_t ev() {
+        if (!spare.empty()) { hipEvent_t e = spare.back(); spare.pop_back(); return e; }
+        hipEvent_t e;
+        TNR_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void drain() {
+        for (auto& r : recs) {
+            TNR_HIP(hipEventSynchronize(r.b));
+            float ms = 0;
+            TNR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            tot[r.kind].launches++;
+            tot[r.kind].ms += ms;
+            tot[r.kind].flops += r.flops;
+            tot[r.kind].bytes += r.bytes;
+            spare.push_back(r.a);
+            spare.push_back(r.b);
+        }
+        recs.clear();
+    }
+};
+ProfState& prof() {
+    static ProfState p;
+    return p;
+}
+}  // namespace
+
+void prof_enable(bool on) {
+    ProfState& p = prof();
+    p.drain();
+    p.on = on;
+    for (auto& t : p.tot) t = ProfTotals();
+}
+
+bool prof_enabled() { return prof().on; }
+
+ProfScope::ProfScope(hipStream_t s, int k, double f, double b)
+    : st(s), kind(k), slot(-1), flops(f), bytes(b) {
+    ProfState& p = prof();
+    if (!p.on) return;
+    if (p.recs.size() > 4096) p.drain();
+    ProfState::Rec r{p.ev(), p.ev(), k, f, b};
+    TNR_HIP(hipEventRecord(r.a, st));
+    slot = (int)p.recs.size();
+    p.recs.push_back(r);
+}
+
+ProfScope::~ProfScope() {
+    if (slot < 0) return;
+    (void)hipEventRecord(prof().recs[slot].b, st);
+}
+
+ProfTotals prof_read(int kind) {
+    ProfState& p = prof();
+    p.drain();
+    return (kind >= 0 && kind < PK_COUNT) ? p.tot[kind] : ProfTotals();
+}
+
 // =================================================================== launch
 static int grid_for(int64_t total, int cap = 4096) {
     int64_t g = (total + 255) / 256;
@@ -613,22 +796,39 @@ static int grid_for(int64_t total, int cap = 4096) {
 
 void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t total) {
     if (total <= 0 || nd <= 0) return;
+    ProfScope ps(st, PK_COPY, 0.0, 16.0 * (double)total);
     hipLaunchKernelGGL(k_copy, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd, total);
     TNR_HIP(hipGetLastError());
 }
 
-void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32) {
+void launch_transpose(hipStream_t st, const TransDesc* d_descs, int nd, int64_t tiles,
+                      int64_t elems) {
     if (tiles <= 0 || nd <= 0) return;
+    ProfScope ps(st, PK_COPY, 0.0, 16.0 * (double)elems);
+    const int grid = (int)std::min<int64_t>(tiles, 4096);
+    hipLaunchKernelGGL(k_transpose, dim3(grid), dim3(256), 0, st, d_descs, nd, tiles);
+    TNR_HIP(hipGetLastError());
+}
+
+void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32,
+                 double flops, int shape) {
+    if (tiles <= 0 || nd <= 0) return;
+    ProfScope ps(st, split32 ? PK_GEMM_SPLIT32 : PK_GEMM_F64, flops, 0.0);
     if (split32)
         hipLaunchKernelGGL(k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
+    else if (shape == GEMM_128x128)
+        hipLaunchKernelGGL((k_gemm_f64<128, 128>), dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
+    else if (shape == GEMM_128x64)
+        hipLaunchKernelGGL((k_gemm_f64<128, 64>), dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
     else
-        hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
+        hipLaunchKernelGGL((k_gemm_f64<64, 64>), dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 
 void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const int64_t* d_starts,
                         int64_t total) {
     if (total <= 0 || nd <= 0) return;
+    ProfScope ps(st, PK_GEMM_REDUCE, 0.0, 0.0);
     hipLaunchKernelGGL(k_gemm_reduce, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd,
                        d_starts, total);
     TNR_HIP(hipGetLastError());
@@ -637,6 +837,7 @@ void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const i
 void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const TraceProb* d_probs,
                             int nprob, int64_t max_size) {
     if (nprob <= 0) return;
+    ProfScope ps(st, PK_TRACE, 0.0, 0.0);
     hipLaunchKernelGGL(k_trace_partial, dim3(TRACE_PARTS, nprob), dim3(256), 0, st, d_descs,
                        d_probs);
     TNR_HIP(hipGetLastError());
@@ -646,9 +847,9 @@ void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const Trac
 }
 
 size_t jacobi_lds_bytes(int n) {
-    int np = n + (n & 1);
-    return (size_t)2 * np * np * sizeof(double) + 6 * (LMAX / 2) * sizeof(double) +
-           (LMAX / 2 + 4) * sizeof(int);
+    const size_t np = n + (n & 1);
+    return 2 * np * (np + 1) * sizeof(double) + 3 * (LMAX / 2) * sizeof(double) +
+           (3 * (LMAX / 2) + LMAX) * sizeof(int);
 }
 
 void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, double tol,
@@ -661,38 +862,53 @@ void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, 
                                     (int)jacobi_lds_bytes(LMAX)));
         attr = true;
     }
-    hipLaunchKernelGGL(k_jacobi, dim3(nd), dim3(256), jacobi_lds_bytes(nmax), st, d_descs, nd, tol,
+    ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
+    hipLaunchKernelGGL(k_jacobi, dim3(nd), dim3(JT), jacobi_lds_bytes(nmax), st, d_descs, nd, tol,
                        max_sweeps);
     TNR_HIP(hipGetLastError());
 }
 
 void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, double rtol) {
     if (nd <= 0) return;
-    hipLaunchKernelGGL(k_colnorm, dim3(nd), dim3(256), 0, st, d_descs, nd, rtol);
+    ProfScope ps(st, PK_COLNORM, 0.0, 0.0);
+    hipLaunchKernelGGL(k_colnorm, dim3(nd), dim3(1024), 0, st, d_descs, nd, rtol);
     TNR_HIP(hipGetLastError());
 }
 
 void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax) {
     if (nd <= 0) return;
     int64_t tot = (int64_t)nmax * nmax;
+    ProfScope ps(st, PK_MISC, 0.0, 0.0);
     hipLaunchKernelGGL(k_symmetrize, dim3(grid_for(tot, 1024), nd), dim3(256), 0, st, d_descs, nd);
+    TNR_HIP(hipGetLastError());
+}
+
+void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rtol, double target,
+                 int min_iters, int max_iters) {
+    if (nd <= 0) return;
+    ProfScope ps(st, PK_MISC, 0.0, 0.0);
+    hipLaunchKernelGGL(k_conv, dim3((nd + 63) / 64), dim3(64), 0, st, d_descs, nd, null_rtol, target,
+                       min_iters, max_iters);
     TNR_HIP(hipGetLastError());
 }
 
 void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd) {
     if (nd <= 0) return;
+    ProfScope ps(st, PK_MISC, 0.0, 0.0);
     hipLaunchKernelGGL(k_frob, dim3(nd), dim3(256), 0, st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 
 void launch_randfill(hipStream_t st, const FillDesc* d_descs, int nd, int64_t total) {
     if (total <= 0 || nd <= 0) return;
+    ProfScope ps(st, PK_FILL, 0.0, 0.0);
     hipLaunchKernelGGL(k_randfill, dim3(grid_for(total, 4096)), dim3(256), 0, st, d_descs, nd, total);
     TNR_HIP(hipGetLastError());
 }
 
 void launch_gather(hipStream_t st, const GatherDesc* d_descs, int nd, int64_t total) {
     if (total <= 0 || nd <= 0) return;
+    ProfScope ps(st, PK_GATHER, 0.0, 16.0 * (double)total);
     hipLaunchKernelGGL(k_gather, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd, total);
     TNR_HIP(hipGetLastError());
 }

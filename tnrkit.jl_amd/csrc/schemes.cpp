@@ -25,24 +25,39 @@ uint64_t problem_seed(int m, int n, int c, int tag) {
     return mix64(((uint64_t)m << 40) ^ ((uint64_t)n << 16) ^ ((uint64_t)c << 8) ^ (uint64_t)tag);
 }
 
-BondVec make_bond(Ctx& ctx, int N, const int* dim, const std::vector<double>& vals) {
-    BondVec b;
-    b.N = N;
-    int o = 0;
-    for (int c = 0; c < N; ++c) {
-        b.dim[c] = dim[c];
-        b.off[c] = o;
-        o += dim[c];
+// Bond vectors of many schemes in one device store, one upload.
+std::vector<BondVec> make_bonds(Ctx& ctx, const std::vector<int>& Ns,
+                                const std::vector<const int*>& dims,
+                                const std::vector<const std::vector<double>*>& vals) {
+    std::vector<BondVec> out(dims.size());
+    std::vector<double> all;
+    for (size_t b = 0; b < dims.size(); ++b) {
+        BondVec& v = out[b];
+        const int N = Ns[b];
+        v.N = N;
+        int o = 0;
+        for (int c = 0; c < N; ++c) {
+            v.dim[c] = dims[b][c];
+            v.off[c] = o;
+            o += dims[b][c];
+        }
+        TNR_REQUIRE((size_t)o == vals[b]->size(), TNR_ERR_INTERNAL, "bond vector size mismatch");
+        v.host = *vals[b];
+        v.base = all.size();
+        all.insert(all.end(), vals[b]->begin(), vals[b]->end());
     }
-    TNR_REQUIRE((size_t)o == vals.size(), TNR_ERR_INTERNAL, "bond vector size mismatch");
-    b.host = vals;
-    b.buf = Buf(&ctx.pool, std::max<size_t>(1, vals.size()) * sizeof(double));
-    if (!vals.empty()) {
-        void* tmp = ctx.stage->put(ctx.stream, vals.data(), vals.size() * sizeof(double));
-        TNR_HIP(hipMemcpyAsync(b.buf.d(), tmp, vals.size() * sizeof(double),
+    auto store = std::make_shared<Buf>(&ctx.pool, std::max<size_t>(1, all.size()) * sizeof(double));
+    if (!all.empty()) {
+        void* tmp = ctx.stage->put(ctx.stream, all.data(), all.size() * sizeof(double));
+        TNR_HIP(hipMemcpyAsync(store->d(), tmp, all.size() * sizeof(double),
                                hipMemcpyDeviceToDevice, ctx.stream));
     }
-    return b;
+    for (auto& v : out) v.store = store;
+    return out;
+}
+
+BondVec make_bond(Ctx& ctx, int N, const int* dim, const std::vector<double>& vals) {
+    return std::move(make_bonds(ctx, {N}, {dim}, {&vals})[0]);
 }
 
 // s^a where s >= tol, else s unchanged (src/schemes/btrg.jl:50-58)
@@ -69,7 +84,8 @@ using Weigh = std::function<void(size_t t, const std::vector<double>&, std::vect
 // chi largest values kept across blocks (stable, descending), then
 // U * diag(wu) and diag(wv) * V^T as block tensors.
 std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
-                                   const std::vector<int>& chis, int tag, const Weigh& weigh) {
+                                   const std::vector<int>& chis, const std::vector<int>& tags,
+                                   const Weigh& weigh) {
     std::vector<SvdIn> in;
     std::vector<std::pair<int, int>> owner;
     for (size_t t = 0; t < Ts.size(); ++t) {
@@ -82,7 +98,7 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
             p.M = Ts[t]->d() + L.off[c];
             p.m = R; p.n = C; p.ld = R;
             p.kmax = std::min(chis[t], std::min(R, C));
-            p.seed = problem_seed(R, C, c, tag);
+            p.seed = problem_seed(R, C, c, tags[t]);
             in.push_back(p);
             owner.emplace_back((int)t, c);
         }
@@ -91,6 +107,7 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
     svd_grouped(ctx, in, out);
     std::vector<Factor> F(Ts.size());
     std::vector<GatherDesc> gd;
+    std::vector<double> wall;    // all weight vectors, uploaded once
     size_t p0 = 0;
     for (size_t t = 0; t < Ts.size(); ++t) {
         const Layout& L = Ts[t]->L;
@@ -125,8 +142,11 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
         }
         std::vector<double> wu, wv;
         weigh(t, f.s, wu, wv);
-        const double* dwu = (const double*)ctx.stage->put(ctx.stream, wu.data(), wu.size() * 8);
-        const double* dwv = (const double*)ctx.stage->put(ctx.stream, wv.data(), wv.size() * 8);
+        // device addresses are patched in after the single upload below
+        const size_t owu = wall.size();
+        wall.insert(wall.end(), wu.begin(), wu.end());
+        const size_t owv = wall.size();
+        wall.insert(wall.end(), wv.begin(), wv.end());
         // layouts of the factors
         Leg nl_u, nl_v;
         for (int c = 0; c < N; ++c) {
@@ -148,16 +168,22 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
                         TNR_ERR_INTERNAL, "factor layout mismatch");
             GatherDesc gu{};
             gu.in = out[p].U.d(); gu.out = f.U.d() + f.U.L.off[c];
-            gu.cw = dwu + off[c];
+            gu.cw = reinterpret_cast<const double*>(owu + off[c]);   // index, patched below
             gu.rows = in[p].m; gu.cols = kc[p]; gu.ldi = out[p].ldu; gu.ldo = in[p].m;
             gd.push_back(gu);
             GatherDesc gv{};
             gv.in = out[p].V.d(); gv.out = f.V.d() + f.V.L.off[c];
-            gv.rw = dwv + off[c];
+            gv.rw = reinterpret_cast<const double*>(owv + off[c]);
             gv.rows = kc[p]; gv.cols = in[p].n; gv.ldi = out[p].ldv; gv.ldo = kc[p]; gv.trans = 1;
             gd.push_back(gv);
         }
         p0 = p1;
+    }
+    const double* dw = static_cast<const double*>(
+        ctx.stage->put(ctx.stream, wall.data(), std::max<size_t>(1, wall.size()) * sizeof(double)));
+    for (auto& g : gd) {
+        if (g.cw) g.cw = dw + reinterpret_cast<size_t>(g.cw);
+        if (g.rw) g.rw = dw + reinterpret_cast<size_t>(g.rw);
     }
     gather_many(ctx, gd);
     return F;
@@ -296,17 +322,19 @@ void step_trg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& c
         for (size_t i = 0; i < s.size(); ++i) wu[i] = std::sqrt(s[i]);
         wv = wu;
     };
-    std::vector<const BT*> Ts;
-    for (auto* s : G) Ts.push_back(&s->T);
-    // A, B = SVD12(T) (src/utility/projectors.jl:211-217)
-    auto F1 = svd_trunc_many(ctx, Ts, chis, 11, sqrtw);
     // Tp = transpose(T, ((2, 4), (1, 3)))
     std::vector<PermJob> pj;
     for (auto* s : G) pj.push_back(PermJob{&s->T, {1, 3, 0, 2}, 2, false, {}});
     auto Tp = permute_many(ctx, pj);
-    std::vector<const BT*> Tps;
-    for (auto& t : Tp) Tps.push_back(&t);
-    auto F2 = svd_trunc_many(ctx, Tps, chis, 12, sqrtw);
+    // A, B = SVD12(T); C, D = SVD12(Tp) (src/utility/projectors.jl:211-217),
+    // both factorizations of every scheme in one grouped solve
+    std::vector<const BT*> Ts;
+    std::vector<int> ch, tags;
+    for (size_t i = 0; i < n; ++i) { Ts.push_back(&G[i]->T); ch.push_back(chis[i]); tags.push_back(11); }
+    for (size_t i = 0; i < n; ++i) { Ts.push_back(&Tp[i]); ch.push_back(chis[i]); tags.push_back(12); }
+    auto F = svd_trunc_many(ctx, Ts, ch, tags, sqrtw);
+    std::vector<Factor> F1(std::make_move_iterator(F.begin()), std::make_move_iterator(F.begin() + n));
+    std::vector<Factor> F2(std::make_move_iterator(F.begin() + n), std::make_move_iterator(F.end()));
     // T[-1 -2; -3 -4] := D[-2; 1 2] * B[-1; 4 1] * C[4 3; -3] * A[3 2; -4]
     std::vector<ContractJob> cj;
     for (size_t i = 0; i < n; ++i) {
@@ -334,29 +362,30 @@ void step_btrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& 
     const size_t n = G.size();
     std::vector<const BT*> Ts;
     for (auto* s : G) Ts.push_back(&s->T);
-    std::vector<std::vector<double>> sb1(n), sb2(n);
-    auto w_for = [&](std::vector<std::vector<double>>& sb) {
-        return [&G, &sb](size_t t, const std::vector<double>& s, std::vector<double>& wu,
-                         std::vector<double>& wv) {
-            const double k = G[t]->k;
-            wu.resize(s.size());
-            sb[t].resize(s.size());
-            for (size_t i = 0; i < s.size(); ++i) {
-                wu[i] = pseudopow(s[i], (1.0 - k) / 2.0);   // S_a
-                sb[t][i] = pseudopow(s[i], k);               // S_b
-            }
-            wv = wu;
-        };
-    };
     // U, S, V = svd_trunc(T): A = U S_a, B = S_a V, S1' = S_b
-    auto F1 = svd_trunc_many(ctx, Ts, chis, 21, w_for(sb1));
     // U, S, V = svd_trunc(permute(T, ((3, 1), (4, 2)))): C, D, S2'
+    // (both factorizations of every scheme in one grouped solve)
     std::vector<PermJob> pj;
     for (auto* s : G) pj.push_back(PermJob{&s->T, {2, 0, 3, 1}, 2, false, {}});
     auto P = permute_many(ctx, pj);
-    std::vector<const BT*> Ps;
-    for (auto& t : P) Ps.push_back(&t);
-    auto F2 = svd_trunc_many(ctx, Ps, chis, 22, w_for(sb2));
+    std::vector<const BT*> all(Ts);
+    std::vector<int> ch(chis), tags(n, 21);
+    for (size_t i = 0; i < n; ++i) { all.push_back(&P[i]); ch.push_back(chis[i]); tags.push_back(22); }
+    std::vector<std::vector<double>> sb(2 * n);
+    auto weigh = [&G, &sb, n](size_t t, const std::vector<double>& s, std::vector<double>& wu,
+                              std::vector<double>& wv) {
+        const double k = G[t % n]->k;
+        wu.resize(s.size());
+        sb[t].resize(s.size());
+        for (size_t i = 0; i < s.size(); ++i) {
+            wu[i] = pseudopow(s[i], (1.0 - k) / 2.0);   // S_a
+            sb[t][i] = pseudopow(s[i], k);               // S_b
+        }
+        wv = wu;
+    };
+    auto F = svd_trunc_many(ctx, all, ch, tags, weigh);
+    std::vector<Factor> F1(std::make_move_iterator(F.begin()), std::make_move_iterator(F.begin() + n));
+    std::vector<Factor> F2(std::make_move_iterator(F.begin() + n), std::make_move_iterator(F.end()));
     // T[-1 -2; -3 -4] := D[-1; 4 7] S1[1; 7] B[-2; 1 3] S2[3; 2]
     //                    C[8 2; -4] S1[8; 5] A[6 5; -3] S2[4; 6]
     // with diagonal S1 (vertical) and S2 (horizontal) absorbed as leg weights:
@@ -391,8 +420,18 @@ void step_btrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& 
         log_factor(s, F1[i].kept_min, F1[i].dropped_max, F1[i].iters, F1[i].conv);
         log_factor(s, F2[i].kept_min, F2[i].dropped_max, F2[i].iters, F2[i].conv);
         s.T = std::move(Tn[i]);
-        s.S1 = make_bond(ctx, s.N, F1[i].dim, sb1[i]);
-        s.S2 = make_bond(ctx, s.N, F2[i].dim, sb2[i]);
+    }
+    std::vector<int> Ns;
+    std::vector<const int*> dims;
+    std::vector<const std::vector<double>*> vals;
+    for (size_t i = 0; i < n; ++i) {
+        Ns.push_back(G[i]->N); dims.push_back(F1[i].dim); vals.push_back(&sb[i]);
+        Ns.push_back(G[i]->N); dims.push_back(F2[i].dim); vals.push_back(&sb[n + i]);
+    }
+    auto bonds = make_bonds(ctx, Ns, dims, vals);
+    for (size_t i = 0; i < n; ++i) {
+        G[i]->S1 = std::move(bonds[2 * i]);
+        G[i]->S2 = std::move(bonds[2 * i + 1]);
     }
 }
 

@@ -1,4 +1,10 @@
 // Grouped truncated SVD / symmetric eigensolver (see solver.h).
+//
+// The block iteration is prepared once per call: every GEMM / Jacobi /
+// column-norm descriptor of one iteration is uploaded once and reused by
+// each iteration; a device kernel applies the stopping rule per problem and
+// raises a flag that makes every later launch skip that problem, so the
+// host only looks at the flags between chunks of iterations.
 #include "solver.h"
 
 #include <algorithm>
@@ -13,10 +19,9 @@ SolverCfg& solver_cfg() {
 
 namespace {
 
-// One two-pass Gram/Jacobi Rayleigh-Ritz job: rotates the columns of Y
-// (rows x l) to the Ritz basis of span(Y), normalises them into U and
-// writes the column norms to sigma.  Wtot (optional) receives the total
-// l x l rotation.
+// One Gram/Jacobi Rayleigh-Ritz job: rotates the columns of Y (rows x l) to
+// the Ritz basis of span(Y), normalises them into U and writes the column
+// norms to sigma.  Wtot (two-pass jobs only) receives the total rotation.
 struct RR {
     double* Y;
     int rows, l, ldy;
@@ -30,53 +35,97 @@ struct RR {
     double* U;
     int ldu;
     double* sigma;
+    const int* skip = nullptr;
 };
 
-void rr_grouped(Ctx& ctx, const std::vector<RR>& jobs) {
-    if (jobs.empty()) return;
-    const SolverCfg& cfg = solver_cfg();
-    int lmax = 0;
-    for (auto& j : jobs) lmax = std::max(lmax, j.l);
-    std::vector<GemmJob> g;
-    std::vector<JacobiDesc> jd;
-    auto gram = [&](double* Y, const RR& j) {
-        GemmJob q{Y, Y, j.G, j.l, j.l, j.rows, Y == j.Y ? j.ldy : j.rows,
-                  Y == j.Y ? j.ldy : j.rows, j.l, 1, 0};
-        return q;
-    };
-    for (int pass = 0; pass < 2; ++pass) {
-        g.clear();
-        jd.clear();
-        for (auto& j : jobs) g.push_back(gram(pass == 0 ? j.Y : j.Y2, j));
-        gemm_grouped(ctx, g);
-        for (auto& j : jobs) {
-            JacobiDesc d;
-            d.G = j.G;
-            d.W = pass == 0 ? j.W : j.W2;
-            d.lam = j.lam;
-            d.sweeps = j.sweeps;
-            d.n = j.l;
-            d.ldg = j.l;
-            d.ldw = j.l;
-            jd.push_back(d);
-        }
-        JacobiDesc* djd = ctx.stage->put(ctx.stream, jd);
-        launch_jacobi(ctx.stream, djd, (int)jd.size(), lmax, cfg.jacobi_tol, cfg.jacobi_sweeps);
-        g.clear();
-        for (auto& j : jobs) {
-            if (pass == 0)  // Y2 = Y W
-                g.push_back(GemmJob{j.Y, j.W, j.Y2, j.rows, j.l, j.l, j.ldy, j.l, j.rows, 0, 0});
-            else            // Y = Y2 W2
-                g.push_back(GemmJob{j.Y2, j.W2, j.Y, j.rows, j.l, j.l, j.rows, j.l, j.ldy, 0, 0});
-            if (pass == 1 && j.Wtot)
-                g.push_back(GemmJob{j.W, j.W2, j.Wtot, j.l, j.l, j.l, j.l, j.l, j.l, 0, 0});
-        }
-        gemm_grouped(ctx, g);
+// `passes` = 1 inside the block iteration (the next iteration corrects
+// what one pass leaves), 2 for the final Ritz extraction and direct solves.
+struct RRPlan {
+    int passes = 1, lmax = 0;
+    GemmPlan gram[2], rot[2];
+    std::vector<JacobiDesc> jac[2];
+    std::vector<ColDesc> col;
+    size_t o_jac[2] = {0, 0}, o_col = 0;
+    const JacobiDesc* d_jac[2] = {nullptr, nullptr};
+    const ColDesc* d_col = nullptr;
+};
+
+RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes) {
+    TNR_REQUIRE(passes == 1 || passes == 2, TNR_ERR_INTERNAL, "one or two passes");
+    RRPlan P;
+    P.passes = passes;
+    for (auto& j : jobs) {
+        TNR_REQUIRE(passes == 2 || !j.Wtot, TNR_ERR_INTERNAL, "total rotation needs two passes");
+        P.lmax = std::max(P.lmax, j.l);
     }
-    std::vector<ColDesc> cd;
-    for (auto& j : jobs) cd.push_back(ColDesc{j.Y, j.U, j.sigma, j.rows, j.l, j.ldy, j.ldu});
-    ColDesc* dcd = ctx.stage->put(ctx.stream, cd);
-    launch_colnorm(ctx.stream, dcd, (int)cd.size(), cfg.null_rtol);
+    for (int pass = 0; pass < passes; ++pass) {
+        std::vector<GemmJob> gj, rj;
+        for (auto& j : jobs) {
+            double* Y = pass == 0 ? j.Y : j.Y2;
+            const int ld = pass == 0 ? j.ldy : j.rows;
+            GemmJob g{Y, Y, j.G, j.l, j.l, j.rows, ld, ld, j.l, 1, 0};   // G = Y^T Y
+            g.skip = j.skip;
+            gj.push_back(g);
+            GemmJob r = pass == 0
+                            ? GemmJob{j.Y, j.W, j.Y2, j.rows, j.l, j.l, j.ldy, j.l, j.rows, 0, 0}
+                            : GemmJob{j.Y2, j.W2, j.Y, j.rows, j.l, j.l, j.rows, j.l, j.ldy, 0, 0};
+            r.skip = j.skip;
+            rj.push_back(r);
+            if (pass == 1 && j.Wtot) {
+                GemmJob w{j.W, j.W2, j.Wtot, j.l, j.l, j.l, j.l, j.l, j.l, 0, 0};
+                w.skip = j.skip;
+                rj.push_back(w);
+            }
+            P.jac[pass].push_back(JacobiDesc{j.G, pass == 0 ? j.W : j.W2, j.lam, j.sweeps, j.l, j.l,
+                                             j.l, j.skip});
+        }
+        P.gram[pass] = gemm_plan(ctx, gj);
+        P.rot[pass] = gemm_plan(ctx, rj);
+    }
+    // rotated columns: Y2 after one pass, Y after two
+    for (auto& j : jobs)
+        P.col.push_back(passes == 1 ? ColDesc{j.Y2, j.U, j.sigma, j.rows, j.l, j.rows, j.ldu, j.skip}
+                                    : ColDesc{j.Y, j.U, j.sigma, j.rows, j.l, j.ldy, j.ldu, j.skip});
+    return P;
+}
+
+void rr_stage(RRPlan& P, DescArena& a) {
+    for (int pass = 0; pass < P.passes; ++pass) {
+        gemm_stage(P.gram[pass], a);
+        gemm_stage(P.rot[pass], a);
+        P.o_jac[pass] = a.add(P.jac[pass]);
+    }
+    P.o_col = a.add(P.col);
+}
+
+void rr_bind(RRPlan& P, const DescArena& a) {
+    for (int pass = 0; pass < P.passes; ++pass) {
+        gemm_bind(P.gram[pass], a);
+        gemm_bind(P.rot[pass], a);
+        P.d_jac[pass] = a.at<JacobiDesc>(P.o_jac[pass]);
+    }
+    P.d_col = a.at<ColDesc>(P.o_col);
+}
+
+void rr_issue(Ctx& ctx, const RRPlan& P) {
+    const SolverCfg& cfg = solver_cfg();
+    for (int pass = 0; pass < P.passes; ++pass) {
+        gemm_issue(ctx, P.gram[pass]);
+        launch_jacobi(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmax, cfg.jacobi_tol,
+                      cfg.jacobi_sweeps);
+        gemm_issue(ctx, P.rot[pass]);
+    }
+    launch_colnorm(ctx.stream, P.d_col, (int)P.col.size(), cfg.null_rtol);
+}
+
+void rr_grouped(Ctx& ctx, const std::vector<RR>& jobs, int passes) {
+    if (jobs.empty()) return;
+    RRPlan P = rr_plan(ctx, jobs, passes);
+    DescArena a;
+    rr_stage(P, a);
+    a.commit(ctx);
+    rr_bind(P, a);
+    rr_issue(ctx, P);
 }
 
 void randfill(Ctx& ctx, const std::vector<FillDesc>& fills) {
@@ -96,27 +145,41 @@ int block_size(int kmax) {
     return std::max(l, std::min(LMAX, kmax + 2));
 }
 
-// per-problem convergence bookkeeping of the block iteration
-struct Conv {
-    int it = 0;
-    double prod = 1.0;
-    bool done = false;
-    void update(const double* sv, int l, int kmax, bool squared) {
-        const SolverCfg& cfg = solver_cfg();
-        ++it;
-        int kk = 0;
-        for (int i = 0; i < kmax; ++i)
-            if (sv[i] > cfg.null_rtol * sv[0]) kk = i + 1;
-        double r = 0.0;
-        if (kk > 0 && sv[kk - 1] > 0.0) {
-            r = sv[l - 1] / sv[kk - 1];
-            if (squared) r *= r;
-            r = std::min(r, 1.0);
-        }
-        if (it >= cfg.min_iters) prod *= r;
-        if ((it >= cfg.min_iters && prod < cfg.conv_target) || it >= cfg.max_iters) done = true;
+// Per-call device state of the stopping rule: it, done (ints), prod (double).
+struct ConvState {
+    Buf ints, prods;
+    int P = 0;
+    ConvState(Ctx& ctx, int P_) : P(P_) {
+        ints = Buf(&ctx.pool, (size_t)2 * P * sizeof(int));
+        prods = Buf(&ctx.pool, (size_t)P * sizeof(double));
+        std::vector<int> zi(2 * P, 0);
+        std::vector<double> one(P, 1.0);
+        void* a = ctx.stage->put(ctx.stream, zi.data(), zi.size() * sizeof(int));
+        void* b = ctx.stage->put(ctx.stream, one.data(), one.size() * sizeof(double));
+        TNR_HIP(hipMemcpyAsync(ints.d(), a, zi.size() * sizeof(int), hipMemcpyDeviceToDevice,
+                               ctx.stream));
+        TNR_HIP(hipMemcpyAsync(prods.d(), b, one.size() * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx.stream));
     }
+    int* it(int p) const { return ints.as<int>() + p; }
+    int* done(int p) const { return ints.as<int>() + P + p; }
+    double* prod(int p) const { return prods.d() + p; }
 };
+
+// Issue `issue_one` chunk by chunk until the device marks every problem done.
+template <class F>
+void iterate(Ctx& ctx, const ConvState& cs, F&& issue_one) {
+    const SolverCfg& cfg = solver_cfg();
+    std::vector<int> done(cs.P);
+    int total = 0;
+    const int chunk = 3;
+    while (total < cfg.max_iters) {
+        for (int c = 0; c < chunk; ++c) issue_one();
+        total += chunk;
+        ctx.fetch(done.data(), cs.done(0), cs.P * sizeof(int));
+        if (std::all_of(done.begin(), done.end(), [](int d) { return d != 0; })) break;
+    }
+}
 
 }  // namespace
 
@@ -124,6 +187,7 @@ struct Conv {
 void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& out) {
     out.clear();
     out.resize(in.size());
+    const SolverCfg& cfg = solver_cfg();
     std::vector<int> direct, iter;
     for (size_t p = 0; p < in.size(); ++p) {
         const SvdIn& s = in[p];
@@ -136,13 +200,12 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             iter.push_back((int)p);
         }
     }
-    // ---------------- direct problems
+    // ---------------- direct problems: Gram/Jacobi twice on the smaller side
     if (!direct.empty()) {
-        struct DW { Buf Y, Y2, G, W, W2, Wt, lam, sig; Buf sw; };
+        struct DW { Buf Y, Y2, G, W, W2, Wt, lam, sig, sw; };
         std::vector<DW> w(direct.size());
         std::vector<GatherDesc> gd;
         std::vector<RR> rr;
-        int64_t start = 0;
         for (size_t i = 0; i < direct.size(); ++i) {
             const SvdIn& s = in[direct[i]];
             const bool tr = s.n > s.m;
@@ -159,15 +222,13 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             x.sw = Buf(&ctx.pool, 64);
             GatherDesc g{};
             g.in = s.M; g.out = x.Y.d(); g.rows = rows; g.cols = l;
-            g.ldi = s.ld; g.ldo = rows; g.trans = tr ? 1 : 0; g.start = start;
-            start += (int64_t)rows * l;
+            g.ldi = s.ld; g.ldo = rows; g.trans = tr ? 1 : 0;
             gd.push_back(g);
             rr.push_back(RR{x.Y.d(), rows, l, rows, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
                             x.lam.d(), x.sw.as<int>(), x.Wt.d(), x.Y.d(), rows, x.sig.d()});
         }
-        GatherDesc* dgd = ctx.stage->put(ctx.stream, gd);
-        launch_gather(ctx.stream, dgd, (int)gd.size(), start);
-        rr_grouped(ctx, rr);
+        gather_many(ctx, gd);
+        rr_grouped(ctx, rr, 2);
         for (size_t i = 0; i < direct.size(); ++i) {
             const SvdIn& s = in[direct[i]];
             SvdOut& o = out[direct[i]];
@@ -185,15 +246,18 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             o.iters = 0;
         }
     }
-    // ---------------- iterative problems
+    // ---------------- iterative problems: block subspace iteration
     if (!iter.empty()) {
-        const size_t P = iter.size();
-        struct IW { Buf V, Z, Y, U, Y2, G, W, W2, Wt, lam, sigU, sw; int l; Conv cv; };
+        const int P = (int)iter.size();
+        struct IW { Buf V, Z, Y, U, Y2, G, W, W2, Wt, lam, sigU, sw; int l; };
         std::vector<IW> w(P);
-        Buf sigV(&ctx.pool, P * LMAX * 8);
+        Buf sigV(&ctx.pool, (size_t)P * LMAX * 8);
+        ConvState cs(ctx, P);
         std::vector<FillDesc> fills;
-        std::vector<RR> rr;
-        for (size_t i = 0; i < P; ++i) {
+        std::vector<RR> r0, rY, rZ, rF;
+        std::vector<GemmJob> jMV, jMtU, jF, jV;
+        std::vector<ConvDesc> conv;
+        for (int i = 0; i < P; ++i) {
             const SvdIn& s = in[iter[i]];
             IW& x = w[i];
             x.l = block_size(s.kmax);
@@ -210,77 +274,70 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             x.lam = Buf(&ctx.pool, (size_t)l * 8);
             x.sigU = Buf(&ctx.pool, (size_t)l * 8);
             x.sw = Buf(&ctx.pool, 64);
+            const int* skip = cs.done(i);
+            double* sv = sigV.d() + (size_t)i * LMAX;
             fills.push_back(FillDesc{x.V.d(), s.n, l, s.n, s.seed, 0});
-            rr.push_back(RR{x.V.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
-                            x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n,
-                            sigV.d() + i * LMAX});
+            r0.push_back(RR{x.V.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
+                            x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
+            GemmJob mv{s.M, x.V.d(), x.Y.d(), s.m, l, s.n, s.ld, s.n, s.m, 0, 0};   // Y = M V
+            mv.skip = skip;
+            jMV.push_back(mv);
+            rY.push_back(RR{x.Y.d(), s.m, l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
+                            x.sw.as<int>(), nullptr, x.U.d(), s.m, x.sigU.d(), skip});
+            GemmJob mtu{s.M, x.U.d(), x.Z.d(), s.n, l, s.m, s.ld, s.m, s.n, 1, 0};  // Z = M^T U
+            mtu.skip = skip;
+            jMtU.push_back(mtu);
+            rZ.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
+                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip});
+            conv.push_back(ConvDesc{sv, l, s.kmax, 1, cs.it(i), cs.prod(i), cs.done(i)});
+            // Ritz extraction from Y = M V (two passes), V <- V Wtot
+            jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, l, s.n, s.ld, s.n, s.m, 0, 0});
+            rF.push_back(RR{x.Y.d(), s.m, l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
+                            x.sw.as<int>(), x.Wt.d(), x.U.d(), s.m, x.sigU.d()});
+            jV.push_back(GemmJob{x.V.d(), x.Wt.d(), x.Z.d(), s.n, l, l, s.n, l, s.n, 0, 0});
         }
         randfill(ctx, fills);
-        rr_grouped(ctx, rr);
-        std::vector<double> sv(P * LMAX);
-        std::vector<int> active(P);
-        for (size_t i = 0; i < P; ++i) active[i] = (int)i;
-        while (!active.empty()) {
-            std::vector<GemmJob> g;
-            std::vector<RR> r1, r2;
-            for (int i : active) {
-                const SvdIn& s = in[iter[i]];
-                IW& x = w[i];
-                g.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, x.l, s.n, s.ld, s.n, s.m, 0, 0});
-                r1.push_back(RR{x.Y.d(), s.m, x.l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
-                                x.lam.d(), x.sw.as<int>(), nullptr, x.U.d(), s.m, x.sigU.d()});
-            }
-            gemm_grouped(ctx, g);
-            rr_grouped(ctx, r1);
-            g.clear();
-            for (int i : active) {
-                const SvdIn& s = in[iter[i]];
-                IW& x = w[i];
-                g.push_back(GemmJob{s.M, x.U.d(), x.Z.d(), s.n, x.l, s.m, s.ld, s.m, s.n, 1, 0});
-                r2.push_back(RR{x.Z.d(), s.n, x.l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
-                                x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n,
-                                sigV.d() + i * LMAX});
-            }
-            gemm_grouped(ctx, g);
-            rr_grouped(ctx, r2);
-            ctx.fetch(sv.data(), sigV.d(), sv.size() * 8);
-            std::vector<int> still;
-            for (int i : active) {
-                w[i].cv.update(&sv[i * LMAX], w[i].l, in[iter[i]].kmax, true);
-                if (!w[i].cv.done) still.push_back(i);
-            }
-            active.swap(still);
-        }
-        // Ritz extraction from Y = M V
-        std::vector<GemmJob> g;
-        std::vector<RR> r;
-        for (size_t i = 0; i < P; ++i) {
-            const SvdIn& s = in[iter[i]];
-            IW& x = w[i];
-            g.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, x.l, s.n, s.ld, s.n, s.m, 0, 0});
-            r.push_back(RR{x.Y.d(), s.m, x.l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
-                           x.lam.d(), x.sw.as<int>(), x.Wt.d(), x.U.d(), s.m, x.sigU.d()});
-        }
-        gemm_grouped(ctx, g);
-        rr_grouped(ctx, r);
-        g.clear();
-        for (size_t i = 0; i < P; ++i) {
-            const SvdIn& s = in[iter[i]];
-            IW& x = w[i];
-            g.push_back(GemmJob{x.V.d(), x.Wt.d(), x.Z.d(), s.n, x.l, x.l, s.n, x.l, s.n, 0, 0});
-        }
-        gemm_grouped(ctx, g);
-        for (size_t i = 0; i < P; ++i) {
+        rr_grouped(ctx, r0, 1);
+        GemmPlan pMV = gemm_plan(ctx, jMV), pMtU = gemm_plan(ctx, jMtU);
+        RRPlan pY = rr_plan(ctx, rY, 1), pZ = rr_plan(ctx, rZ, 1);
+        DescArena a;
+        gemm_stage(pMV, a);
+        gemm_stage(pMtU, a);
+        rr_stage(pY, a);
+        rr_stage(pZ, a);
+        const size_t o_conv = a.add(conv);
+        a.commit(ctx);
+        gemm_bind(pMV, a);
+        gemm_bind(pMtU, a);
+        rr_bind(pY, a);
+        rr_bind(pZ, a);
+        const ConvDesc* d_conv = a.at<ConvDesc>(o_conv);
+        iterate(ctx, cs, [&] {
+            gemm_issue(ctx, pMV);
+            rr_issue(ctx, pY);
+            gemm_issue(ctx, pMtU);
+            rr_issue(ctx, pZ);
+            launch_conv(ctx.stream, d_conv, P, cfg.null_rtol, cfg.conv_target, cfg.min_iters,
+                        cfg.max_iters);
+        });
+        gemm_grouped(ctx, jF);
+        rr_grouped(ctx, rF, 2);
+        gemm_grouped(ctx, jV);
+        std::vector<int> its(P);
+        std::vector<double> prods(P);
+        ctx.fetch(its.data(), cs.it(0), P * sizeof(int));
+        ctx.fetch(prods.data(), cs.prod(0), P * sizeof(double));
+        for (int i = 0; i < P; ++i) {
             SvdOut& o = out[iter[i]];
             IW& x = w[i];
             o.U = std::move(x.U); o.ldu = in[iter[i]].m;
             o.V = std::move(x.Z); o.ldv = in[iter[i]].n;
             o.s = std::move(x.sigU);
-            o.iters = x.cv.it;
-            o.conv = x.cv.prod;
+            o.iters = its[i];
+            o.conv = prods[i];
         }
     }
-    // singular values to the host (one transfer per problem, all queued)
+    // singular values to the host (all queued, one synchronisation)
     for (size_t p = 0; p < in.size(); ++p) {
         out[p].s_host.resize(in[p].kmax);
         TNR_HIP(hipMemcpyAsync(out[p].s_host.data(), out[p].s.d(), in[p].kmax * 8,
@@ -317,23 +374,25 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             o.ncols = s.n;
             o.direct = true;
             sw[i] = Buf(&ctx.pool, 64);
-            JacobiDesc d{s.M, o.U.d(), o.lam.d(), sw[i].as<int>(), s.n, s.ld, s.n};
-            jd.push_back(d);
+            jd.push_back(JacobiDesc{s.M, o.U.d(), o.lam.d(), sw[i].as<int>(), s.n, s.ld, s.n});
             nmax = std::max(nmax, s.n);
         }
         JacobiDesc* djd = ctx.stage->put(ctx.stream, jd);
         launch_jacobi(ctx.stream, djd, (int)jd.size(), nmax, cfg.jacobi_tol, cfg.jacobi_sweeps);
     }
     if (!iter.empty()) {
-        const size_t P = iter.size();
-        struct IW { Buf V, Y, Y2, G, W, W2, lam, sw; int l; Conv cv; };
+        const int P = (int)iter.size();
+        struct IW { Buf V, Y, Y2, G, W, W2, lam, sw; int l; };
         std::vector<IW> w(P);
-        Buf sigV(&ctx.pool, P * LMAX * 8);
+        Buf sigV(&ctx.pool, (size_t)P * LMAX * 8);
+        ConvState cs(ctx, P);
         std::vector<FillDesc> fills;
-        std::vector<RR> rr;
+        std::vector<RR> r0, rY;
         std::vector<ColDesc> sym;
+        std::vector<GemmJob> jMV, jF, jH;
+        std::vector<ConvDesc> conv;
         int nmax = 0;
-        for (size_t i = 0; i < P; ++i) {
+        for (int i = 0; i < P; ++i) {
             const EighIn& s = in[iter[i]];
             IW& x = w[i];
             x.l = block_size(s.kmax);
@@ -346,59 +405,48 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             x.W2 = Buf(&ctx.pool, (size_t)l * l * 8);
             x.lam = Buf(&ctx.pool, (size_t)l * 8);
             x.sw = Buf(&ctx.pool, 64);
+            const int* skip = cs.done(i);
+            double* sv = sigV.d() + (size_t)i * LMAX;
             fills.push_back(FillDesc{x.V.d(), s.n, l, s.n, s.seed, 0});
-            rr.push_back(RR{x.V.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
-                            x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n,
-                            sigV.d() + i * LMAX});
+            r0.push_back(RR{x.V.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
+                            x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
             sym.push_back(ColDesc{s.M, nullptr, nullptr, s.n, s.n, s.ld, 0});
+            GemmJob mv{s.M, x.V.d(), x.Y.d(), s.n, l, s.n, s.ld, s.n, s.n, 0, 0};   // Y = M V
+            mv.skip = skip;
+            jMV.push_back(mv);
+            rY.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
+                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip});
+            conv.push_back(ConvDesc{sv, l, s.kmax, 0, cs.it(i), cs.prod(i), cs.done(i)});
+            jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, l, s.n, s.ld, s.n, s.n, 0, 0});
+            jH.push_back(GemmJob{x.V.d(), x.Y.d(), x.G.d(), l, l, s.n, s.n, s.n, l, 1, 0});
             nmax = std::max(nmax, s.n);
         }
         ColDesc* dsym = ctx.stage->put(ctx.stream, sym);
         launch_symmetrize(ctx.stream, dsym, (int)sym.size(), nmax);
         randfill(ctx, fills);
-        rr_grouped(ctx, rr);
-        std::vector<double> sv(P * LMAX);
-        std::vector<int> active(P);
-        for (size_t i = 0; i < P; ++i) active[i] = (int)i;
-        while (!active.empty()) {
-            std::vector<GemmJob> g;
-            std::vector<RR> r1;
-            for (int i : active) {
-                const EighIn& s = in[iter[i]];
-                IW& x = w[i];
-                g.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, x.l, s.n, s.ld, s.n, s.n, 0, 0});
-                r1.push_back(RR{x.Y.d(), s.n, x.l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
-                                x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n,
-                                sigV.d() + i * LMAX});
-            }
-            gemm_grouped(ctx, g);
-            rr_grouped(ctx, r1);
-            ctx.fetch(sv.data(), sigV.d(), sv.size() * 8);
-            std::vector<int> still;
-            for (int i : active) {
-                w[i].cv.update(&sv[i * LMAX], w[i].l, in[iter[i]].kmax, false);
-                if (!w[i].cv.done) still.push_back(i);
-            }
-            active.swap(still);
-        }
+        rr_grouped(ctx, r0, 1);
+        GemmPlan pMV = gemm_plan(ctx, jMV);
+        RRPlan pY = rr_plan(ctx, rY, 1);
+        DescArena a;
+        gemm_stage(pMV, a);
+        rr_stage(pY, a);
+        const size_t o_conv = a.add(conv);
+        a.commit(ctx);
+        gemm_bind(pMV, a);
+        rr_bind(pY, a);
+        const ConvDesc* d_conv = a.at<ConvDesc>(o_conv);
+        iterate(ctx, cs, [&] {
+            gemm_issue(ctx, pMV);
+            rr_issue(ctx, pY);
+            launch_conv(ctx.stream, d_conv, P, cfg.null_rtol, cfg.conv_target, cfg.min_iters,
+                        cfg.max_iters);
+        });
         // Ritz pairs: H = V^T (M V), eigendecompose, U = V W
-        std::vector<GemmJob> g;
-        for (size_t i = 0; i < P; ++i) {
-            const EighIn& s = in[iter[i]];
-            IW& x = w[i];
-            g.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, x.l, s.n, s.ld, s.n, s.n, 0, 0});
-        }
-        gemm_grouped(ctx, g);
-        g.clear();
-        for (size_t i = 0; i < P; ++i) {
-            const EighIn& s = in[iter[i]];
-            IW& x = w[i];
-            g.push_back(GemmJob{x.V.d(), x.Y.d(), x.G.d(), x.l, x.l, s.n, s.n, s.n, x.l, 1, 0});
-        }
-        gemm_grouped(ctx, g);
+        gemm_grouped(ctx, jF);
+        gemm_grouped(ctx, jH);
         std::vector<JacobiDesc> jd;
         int lmax = 0;
-        for (size_t i = 0; i < P; ++i) {
+        for (int i = 0; i < P; ++i) {
             IW& x = w[i];
             EighOut& o = out[iter[i]];
             o.lam = Buf(&ctx.pool, (size_t)x.l * 8);
@@ -407,19 +455,25 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         }
         JacobiDesc* djd = ctx.stage->put(ctx.stream, jd);
         launch_jacobi(ctx.stream, djd, (int)jd.size(), lmax, cfg.jacobi_tol, cfg.jacobi_sweeps);
-        g.clear();
-        for (size_t i = 0; i < P; ++i) {
+        std::vector<GemmJob> jU;
+        for (int i = 0; i < P; ++i) {
             const EighIn& s = in[iter[i]];
             IW& x = w[i];
             EighOut& o = out[iter[i]];
             o.U = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
             o.ncols = x.l;
             o.direct = false;
-            o.iters = x.cv.it;
-            o.conv = x.cv.prod;
-            g.push_back(GemmJob{x.V.d(), x.W.d(), o.U.d(), s.n, x.l, x.l, s.n, x.l, s.n, 0, 0});
+            jU.push_back(GemmJob{x.V.d(), x.W.d(), o.U.d(), s.n, x.l, x.l, s.n, x.l, s.n, 0, 0});
         }
-        gemm_grouped(ctx, g);
+        gemm_grouped(ctx, jU);
+        std::vector<int> its(P);
+        std::vector<double> prods(P);
+        ctx.fetch(its.data(), cs.it(0), P * sizeof(int));
+        ctx.fetch(prods.data(), cs.prod(0), P * sizeof(double));
+        for (int i = 0; i < P; ++i) {
+            out[iter[i]].iters = its[i];
+            out[iter[i]].conv = prods[i];
+        }
     }
     for (size_t p = 0; p < in.size(); ++p) {
         out[p].lam_host.resize(out[p].ncols);

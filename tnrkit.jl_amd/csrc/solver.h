@@ -56,7 +56,7 @@ struct SolverCfg {
     double jacobi_tol = 1e-15;   // relative off-diagonal threshold
     int jacobi_sweeps = 40;
     double null_rtol = 1e-14;    // columns below this x sigma_max are null
-    double conv_target = 1e-20;  // accumulated contraction target
+    double conv_target = 1e-14;  // accumulated contraction of the top-k subspace angle
     int min_iters = 2;
     int max_iters = 80;
 };
