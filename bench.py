@@ -13,9 +13,12 @@ Reported beside the throughput:
   <= 1e-9), and that path's own throughput;
 * roofline of the dominant kernel class (device time from HIP events on the
   library's stream, algorithmic flops / bytes per launch);
+* latency p50/p99 from a seeded Poisson stream of BTRG solves (Z2 Ising,
+  Z3 Potts, Z6 clock; chi 8-64) through the continuous-batching server;
 * cpu_baseline: the FP64 numpy oracle (OpenBLAS dgemm / LAPACK gesdd, the
-  library family TensorKit reaches) on the host cores, over a bounded
-  sample of the same workload, scaled to free energies per second.
+  library family TensorKit reaches) as a beta scan, one single-threaded
+  process per host core, over a bounded sample of the same workload, scaled
+  to free energies per second.
 """
 from __future__ import annotations
 
@@ -80,16 +83,12 @@ def roofline_from(prof, precision):
             "share_of_device_time": g["ms"] / total_ms if total_ms else None}
 
 
-def cpu_baseline(chi, maxiter, beta, budget_s):
-    """FP64 oracle on the host: time steps until the budget is spent, then
-    scale the steady-state step time to one free energy (maxiter steps)."""
+def _cpu_run(args):
+    """One oracle BTRG run on one core: time steps until the budget is spent
+    and extrapolate the remaining steps at the steady-state step time."""
+    chi, maxiter, beta, budget_s = args
     from oracle import models as om
     from oracle import schemes as osch
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:
-        threads = os.cpu_count() or 1
     sc = osch.BTRG(om.classical_ising_trivial(beta))
     sc.finalize()
     times = []
@@ -102,16 +101,190 @@ def cpu_baseline(chi, maxiter, beta, budget_s):
         if time.perf_counter() - t_all > budget_s:
             break
     n = len(times)
-    if n >= maxiter:
-        per_fe = sum(times)
-        sample = f"one full BTRG chi={chi} run ({maxiter} steps)"
-    else:
-        steady = float(np.median(times[-3:]))
-        per_fe = sum(times) + steady * (maxiter - n)
-        sample = (f"first {n} of {maxiter} BTRG chi={chi} steps, remaining steps at the "
-                  f"steady-state step time {steady:.2f} s")
-    return {"value": 1.0 / per_fe, "unit": "free energies/s", "cores": threads, "kind": "port",
-            "sample": sample + f"; {time.perf_counter() - t_all:.1f} s measured"}
+    per_fe = sum(times) + (float(np.median(times[-3:])) * (maxiter - n) if n < maxiter else 0.0)
+    return per_fe, n
+
+
+def cpu_baseline(chi, maxiter, betas, budget_s):
+    """FP64 oracle (numpy -> OpenBLAS dgemm / LAPACK gesdd) as a beta scan:
+    one single-threaded process per core, each on its own beta, bounded to
+    budget_s; throughput = sum over processes of 1 / (time per free energy)."""
+    import multiprocessing as mp
+    procs = max(1, min(16, os.cpu_count() or 1, len(betas)))
+    saved = {k: os.environ.get(k) for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS",
+                                             "MKL_NUM_THREADS")}
+    for k in saved:
+        os.environ[k] = "1"
+    try:
+        with mp.get_context("spawn").Pool(procs) as pool:
+            res = pool.map(_cpu_run, [(chi, maxiter, float(betas[i]), budget_s)
+                                      for i in range(procs)])
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    value = sum(1.0 / r[0] for r in res)
+    steps = min(r[1] for r in res)
+    sample = (f"{procs} single-threaded oracle BTRG chi={chi} runs on distinct betas, "
+              f"{steps}+ of {maxiter} steps each timed within {budget_s:.0f} s, the rest "
+              f"extrapolated at the steady-state step time")
+    return {"value": value, "unit": "free energies/s", "cores": procs, "kind": "port",
+            "sample": sample}
+
+
+def btrg_model_flops(chi, maxiter):
+    """Algorithmic FP64 flops of one BTRG free energy on the Trivial Ising
+    tensor: bond schedule D0 = 2, D' = min(chi, D^2); per step the optimal
+    pairwise contraction 2(chi'^4 D^2 + 2 D^3 chi'^2 + 4 chi' D^3) plus two
+    dense truncated SVDs of n = D^2 at 8n^3/3 + 8n^2 chi' each."""
+    D, total = 2, 0.0
+    for _ in range(max(maxiter, 1)):
+        cp = min(chi, D * D)
+        n = D * D
+        total += 2.0 * (cp ** 4 * D ** 2 + 2 * D ** 3 * cp ** 2 + 4 * cp * D ** 3)
+        total += 2.0 * (8.0 * n ** 3 / 3.0 + 8.0 * n * n * cp)
+        D = cp
+    return total
+
+
+def _timed(fn):
+    t0 = time.perf_counter()
+    r = fn()
+    return r, time.perf_counter() - t0
+
+
+def workload_c2(tk, maxiter):
+    """TRG and HOTRG on Trivial Ising at beta_c, chi 32 and 48: plain FP64
+    against split-FP32 contractions, per-step norm and free-energy error."""
+    cases = []
+    T = tk.classical_ising(None, symmetry="Trivial")
+    fo = tk.f_onsager
+    for name, sf in (("TRG", 2.0), ("HOTRG", 4.0)):
+        for chi in (32, 48):
+            res = {}
+            for prec in (tk.FP64, tk.SPLIT32):
+                sc = getattr(tk, name)(T, precision=prec)
+                d, dt = _timed(lambda: tk.run(sc, tk.truncrank(chi), tk.maxiter(maxiter),
+                                              verbosity=0))
+                sc.close()
+                res[prec] = (np.asarray(d), dt)
+            d64, t64 = res[tk.FP64]
+            d32, t32 = res[tk.SPLIT32]
+            f64 = tk.free_energy(d64, tk.ising_betac, scalefactor=sf)
+            f32 = tk.free_energy(d32, tk.ising_betac, scalefactor=sf)
+            cases.append({"scheme": name, "chi": chi, "maxiter": maxiter,
+                          "fp64_s_per_run": t64, "split32_s_per_run": t32,
+                          "steps_per_s_fp64": maxiter / t64,
+                          "max_rel_norm_diff_split_vs_fp64": float(np.max(np.abs(d32 - d64) /
+                                                                         np.abs(d64))),
+                          "rel_df_split_vs_fp64": abs(f32 - f64) / abs(f64),
+                          "rel_df_fp64_vs_onsager": abs(f64 - fo) / abs(fo)})
+    return {"workload": "C2: TRG/HOTRG Trivial Ising beta_c, chi 32/48, plain FP64 vs split",
+            "cases": cases}
+
+
+def workload_c3(tk, maxiter):
+    """BTRG with Z2 Ising at beta_c and Z3 Potts at ln(1+sqrt 3), chi = 64."""
+    cases = []
+    for label, T, beta in (("Z2 Ising", tk.classical_ising(None, symmetry="Z2"), tk.ising_betac),
+                           ("Z3 Potts", tk.classical_potts(3), tk.potts_betac(3))):
+        sc = tk.BTRG(T)
+        d, dt = _timed(lambda: tk.run(sc, tk.truncrank(64), tk.maxiter(maxiter), verbosity=0))
+        sc.close()
+        f = tk.free_energy(d, beta)
+        c = {"model": label, "chi": 64, "maxiter": maxiter, "s_per_run": dt,
+             "steps_per_s": maxiter / dt, "f": f}
+        if label == "Z2 Ising":
+            c["rel_df_vs_onsager"] = abs(f - tk.f_onsager) / abs(tk.f_onsager)
+        # the same two runs batched together
+        cases.append(c)
+    reqs = [tk.Request("BTRG", tk.classical_ising(None, symmetry="Z2"), 64, maxiter),
+            tk.Request("BTRG", tk.classical_potts(3), 64, maxiter)]
+    _, dt = _timed(lambda: tk.run_batch(reqs))
+    return {"workload": "C3: BTRG Z2 Ising beta_c and Z3 Potts ln(1+sqrt3), chi 64",
+            "cases": cases, "both_batched_s": dt}
+
+
+def workload_c4(tk, n, maxiter, check):
+    """512 BTRG runs, beta ~ U[0.35, 0.55], chi ~ U{16, 24, 32, 48}."""
+    rng = np.random.default_rng(SEED + 4)
+    betas = rng.uniform(0.35, 0.55, n)
+    chis = rng.choice([16, 24, 32, 48], n)
+    reqs = [tk.Request("BTRG", tk.classical_ising(float(b), symmetry="Trivial"), int(c), maxiter)
+            for b, c in zip(betas, chis)]
+    out = {"workload": f"C4: {n} BTRG runs, beta U[0.35,0.55], chi U{{16,24,32,48}}, "
+                       f"maxiter {maxiter}", "seed": SEED + 4}
+    fs = {}
+    for name, prec in (("fp64", tk.FP64), ("split32", tk.SPLIT32)):
+        tk.run_batch(reqs[:8], prec)
+        res, dt = _timed(lambda: tk.run_batch(reqs, prec))
+        fs[name] = [tk.free_energy(r, float(b)) for r, b in zip(res, betas)]
+        out[name] = {"free_energies_per_s": n / dt, "s": dt}
+    idx = list(range(min(check, n)))
+    errs = {"fp64": [], "split32": []}
+    t_plain = 0.0
+    for i in idx:
+        sc = tk.BTRG(reqs[i].T)
+        d, dt = _timed(lambda: tk.run(sc, tk.truncrank(int(chis[i])), tk.maxiter(maxiter),
+                                      verbosity=0))
+        t_plain += dt
+        sc.close()
+        fp = tk.free_energy(d, float(betas[i]))
+        for k in errs:
+            errs[k].append(abs(fs[k][i] - fp) / abs(fp))
+    out["plain_fp64_one_at_a_time_per_s"] = len(idx) / t_plain
+    for k in errs:
+        out[k]["max_rel_df_vs_plain_fp64"] = max(errs[k])
+    return out
+
+
+def serving_stream(tk, seed, n_req, rate, precision):
+    """Poisson stream of BTRG solves through the continuous-batching server.
+
+    Request mix: Z2 Ising beta ~ U[0.35, 0.55], Z3 Potts beta ~ U[0.8, 1.2],
+    Z6 clock beta ~ U[0.5, 1.5]; chi ~ U{8, 16, 32, 64}; maxiter ~ U[20, 40].
+    Latency of a request = completion seen by poll - its arrival time (both
+    on the host clock, so queueing is included)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_req):
+        m = int(rng.integers(3))
+        if m == 0:
+            T = tk.classical_ising(float(rng.uniform(0.35, 0.55)), symmetry="Z2")
+        elif m == 1:
+            T = tk.classical_potts(3, float(rng.uniform(0.8, 1.2)))
+        else:
+            T = tk.classical_clock(6, float(rng.uniform(0.5, 1.5)))
+        chi = int(rng.choice([8, 16, 32, 64]))
+        reqs.append(tk.Request("BTRG", T, chi, int(rng.integers(20, 41))))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    srv = tk.Server(0, 1024, precision)
+    lat = np.zeros(n_req)
+    inflight = {}
+    nxt, ndone = 0, 0
+    t0 = time.perf_counter()
+    while ndone < n_req:
+        now = time.perf_counter() - t0
+        while nxt < n_req and arrivals[nxt] <= now:
+            inflight[srv.submit(reqs[nxt])] = nxt
+            nxt += 1
+        if not inflight:
+            time.sleep(max(0.0, arrivals[nxt] - now))
+            continue
+        srv.pump()
+        t_poll = time.perf_counter() - t0
+        for rid in list(inflight):
+            if srv.poll(rid) is not None:
+                i = inflight.pop(rid)
+                lat[i] = t_poll - arrivals[i]
+                ndone += 1
+    span = time.perf_counter() - t0 - arrivals[0]
+    srv.close()
+    return {"requests": n_req, "rate_per_s": rate, "seed": seed, "solves_per_s": n_req / span,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mix": "BTRG; Z2 Ising / Z3 Potts / Z6 clock; chi in {8,16,32,64}; maxiter 20-40"}
 
 
 def main():
@@ -127,16 +300,34 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
+    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4"], default="scan",
+                    help="scan = the headline beta scan; c2-c4 print a report line of their own")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent runs would shard by request across GPUs)", file=sys.stderr)
         sys.exit(2)
+    if a.workload != "scan":
+        import __graft_entry__
+        tk = __graft_entry__.package()
+        if a.workload == "c2":
+            r = workload_c2(tk, a.maxiter)
+        elif a.workload == "c3":
+            r = workload_c3(tk, a.maxiter)
+        else:
+            r = workload_c4(tk, 512 if a.batch == 128 else a.batch, a.maxiter, a.check)
+        print(json.dumps(r))
+        return
+    rng = np.random.default_rng(SEED)
+    betas = rng.uniform(0.35, 0.55, a.batch)
+    # the CPU leg runs first, in fresh single-threaded processes, before this
+    # process touches the GPU
+    cpu = None if a.no_cpu else cpu_baseline(a.chi, a.maxiter, betas, a.cpu_seconds)
     import __graft_entry__
     tk = __graft_entry__.package()
     prec = tk.FP64 if a.precision == "fp64" else tk.SPLIT32
-    rng = np.random.default_rng(SEED)
-    betas = rng.uniform(0.35, 0.55, a.batch)
     reqs = [tk.Request("BTRG", tk.classical_ising(float(b), symmetry="Trivial"), a.chi, a.maxiter)
             for b in betas]
 
@@ -178,15 +369,21 @@ def main():
                      "pass": bool(errs) and max(errs) <= 1e-9},
         "plain_fp64_one_at_a_time": {"value": plain_value, "unit": "free energies/s"},
         "speedup_vs_plain": value / plain_value if plain_value else None,
-        "latency_s": {"p50": ms_per_step / 1e3, "p99": ms_per_step / 1e3,
-                      "note": "every run of a batch completes with the batch"},
+        "batch_latency_s": ms_per_step / 1e3,
+        "model_flops_per_free_energy": btrg_model_flops(a.chi, a.maxiter),
+        "end_to_end_model_tflops": btrg_model_flops(a.chi, a.maxiter) * value / 1e12,
+        "end_to_end_frac_fp64_peak": btrg_model_flops(a.chi, a.maxiter) * value / PEAK["fp64_mfma"],
     }
+    if a.serve_requests > 0:
+        sv = serving_stream(tk, SEED + 1, a.serve_requests, a.serve_rate, prec)
+        out["latency_s"] = {"p50": sv["p50_s"], "p99": sv["p99_s"], "source": "serving stream"}
+        out["serving"] = sv
     if not a.no_profile:
         prof = profile_batch(tk, reqs, prec)
         out["roofline"] = roofline_from(prof, a.precision)
         out["kernel_time_ms"] = {k: round(v["ms"], 3) for k, v in prof.items() if v["launches"]}
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a.chi, a.maxiter, float(betas[0]), a.cpu_seconds)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

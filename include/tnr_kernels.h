@@ -54,6 +54,12 @@ int tnr_kernel_jacobi(int count, int n, const double* G, double* W,
  * launches were asked to do (GEMM: 2MNK summed over the grouped problems;
  * copies: bytes read + written). */
 int tnr_profile_enable(int on);
+/* GEMM tile-shape autotuner: on = 1 (default; TNR_AUTOTUNE=0 in the
+ * environment starts it off) times each tile shape on the first launches of
+ * every launch signature and then keeps the fastest; on = 0 uses the static
+ * rule.  Every call forgets what was learned.  Tile shapes never change
+ * results: each output element is summed in the same order under all. */
+int tnr_autotune_enable(int on);
 /* hipDeviceSynchronize() on the current device (timing brackets). */
 int tnr_device_sync(void);
 int tnr_profile_read(int kind, int64_t* launches, double* ms, double* flops,

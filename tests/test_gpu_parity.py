@@ -114,3 +114,17 @@ def test_batch_mixed_symmetries_equals_single_runs(tk):
     for r, b in zip(reqs, batch):
         single = gpu_run(tk, tk.BTRG, r.T, r.chi, r.maxiter)
         assert rel(single, b) < 1e-12
+
+
+def test_lanes_do_not_change_results(tk, monkeypatch):
+    """The server spreads runs over 1-4 lanes (stream + host thread each);
+    every run's norms are bitwise the same for any lane count."""
+    reqs = [tk.Request("BTRG", tk.classical_ising(0.36 + 0.03 * i, symmetry="Trivial"), 16, 6)
+            for i in range(5)]
+    reqs.append(tk.Request("HOTRG", tk.classical_ising(0.44, symmetry="Z2"), 8, 4))
+    out = {}
+    for lanes in ("1", "2", "3"):
+        monkeypatch.setenv("TNR_LANES", lanes)
+        out[lanes] = tk.run_batch(reqs)
+    for a, b, c in zip(out["1"], out["2"], out["3"]):
+        assert np.array_equal(a, b) and np.array_equal(a, c)

@@ -75,6 +75,7 @@ SIGNATURES = {
     "tnr_kernel_eigh": (C.c_int, [C.c_int, _DP, C.c_int, _DP, _DP, _IP]),
     "tnr_kernel_jacobi": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP, _IP, _DP]),
     "tnr_profile_enable": (C.c_int, [C.c_int]),
+    "tnr_autotune_enable": (C.c_int, [C.c_int]),
     "tnr_device_sync": (C.c_int, []),
     "tnr_profile_read": (C.c_int, [C.c_int, _I64P, _DP, _DP, _DP]),
 }

@@ -35,7 +35,7 @@ def build(verbose: bool = False) -> str:
 
     def compile_one(src):
         obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-        flags = CXXFLAGS + (HIPFLAGS if src.endswith(".hip") else [])
+        flags = CXXFLAGS + HIPFLAGS   # host-only sources too: one device target, gfx950
         cmd = [hipcc] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
         out = _run(cmd)
         if verbose and out.strip():

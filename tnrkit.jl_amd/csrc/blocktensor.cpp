@@ -2,7 +2,13 @@
 #include "blocktensor.h"
 
 #include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <limits>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <sstream>
 
@@ -358,10 +364,140 @@ static int pick_shape(const std::vector<GemmJob>& jobs, bool split32) {
     return GEMM_64x64;
 }
 
+
+// ------------------------------------------------------ tile-shape autotuner
+// Every output element is summed in the same order under each tile shape,
+// so the choice changes speed only, never results.  Launches are keyed by a
+// coarse signature (device, operand transposes, log2 of the largest M, N,
+// K, of the problem count and of the flops).  The first occurrences of a
+// key are timed once per shape (HIP events around the launch, read back
+// later without waiting); once every shape has TUNE_SAMPLES timings the
+// fastest is used for that key.  Until then the heuristic above decides.
+namespace {
+constexpr int TUNE_SAMPLES = 2;
+using TuneKey = std::array<int, 9>;
+struct TuneTrial {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int shape = 0;
+};
+struct TuneEntry {
+    float ms[3] = {0, 0, 0};
+    int samples[3] = {0, 0, 0};
+    int requested[3] = {0, 0, 0};
+    std::vector<TuneTrial> pending;
+    int best = -1;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+int g_tune_on = -1;   // -1: read TNR_AUTOTUNE on first use
+
+int ilog2(double v) { return v < 1.0 ? 0 : (int)std::floor(std::log2(v)); }
+
+bool tune_enabled() {
+    if (g_tune_on < 0) {
+        const char* e = std::getenv("TNR_AUTOTUNE");
+        g_tune_on = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_tune_on == 1;
+}
+
+void tune_poll(TuneEntry& t) {
+    for (size_t i = 0; i < t.pending.size();) {
+        TuneTrial& tr = t.pending[i];
+        if (hipEventQuery(tr.e1) != hipSuccess) { ++i; continue; }
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, tr.e0, tr.e1) == hipSuccess) {
+            const int s = tr.shape;
+            t.ms[s] = t.samples[s] ? std::min(t.ms[s], ms) : ms;
+            ++t.samples[s];
+        } else {
+            --t.requested[tr.shape];   // measure again
+        }
+        (void)hipEventDestroy(tr.e0);
+        (void)hipEventDestroy(tr.e1);
+        t.pending.erase(t.pending.begin() + (long)i);
+    }
+    bool all = true;
+    for (int s = 0; s < 3; ++s) all = all && t.samples[s] >= TUNE_SAMPLES;
+    if (all) {
+        int best = 0;
+        for (int s = 1; s < 3; ++s)
+            if (t.ms[s] < t.ms[best]) best = s;
+        t.best = best;
+    }
+}
+}  // namespace
+
+void gemm_autotune_enable(int on) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_on = on ? 1 : 0;
+    for (auto& kv : g_tune)
+        for (auto& tr : kv.second.pending) {
+            (void)hipEventSynchronize(tr.e1);
+            (void)hipEventDestroy(tr.e0);
+            (void)hipEventDestroy(tr.e1);
+        }
+    g_tune.clear();
+}
+
+// Shape for a launch and whether this launch is a timing trial.
+static int tuned_shape(const Ctx& ctx, const std::vector<GemmJob>& jobs, int heuristic,
+                       TuneKey* key_out, bool* trial) {
+    *trial = false;
+    if (!tune_enabled()) return heuristic;
+    int mM = 0, mN = 0, mK = 0, cnt = 0, ta = 0, tb = 0;
+    double fl = 0;
+    for (const GemmJob& j : jobs) {
+        if (j.M <= 0 || j.N <= 0) continue;
+        if (!cnt) { ta = j.ta; tb = j.tb; }
+        mM = std::max(mM, j.M); mN = std::max(mN, j.N); mK = std::max(mK, j.K);
+        fl += 2.0 * j.M * j.N * j.K;
+        ++cnt;
+    }
+    if (!cnt) return heuristic;
+    const TuneKey key{ctx.device, ta, tb, ilog2(mM), ilog2(mN), ilog2(mK), ilog2(cnt),
+                      ilog2(fl), 0};
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneEntry& t = g_tune[key];
+    if (t.best >= 0) return t.best;
+    tune_poll(t);
+    if (t.best >= 0) return t.best;
+    for (int s = 2; s >= 0; --s)
+        if (t.requested[s] < TUNE_SAMPLES) {
+            ++t.requested[s];
+            *key_out = key;
+            *trial = true;
+            return s;
+        }
+    return heuristic;   // trials in flight
+}
+
+static void tune_record_begin(Ctx& ctx, GemmPlan& p, TuneTrial& tr) {
+    TNR_HIP(hipEventCreate(&tr.e0));
+    TNR_HIP(hipEventCreate(&tr.e1));
+    tr.shape = p.shape;
+    TNR_HIP(hipEventRecord(tr.e0, ctx.stream));
+}
+
+static void tune_record_end(Ctx& ctx, const GemmPlan& p, TuneTrial& tr) {
+    TNR_HIP(hipEventRecord(tr.e1, ctx.stream));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(p.tune_key);
+    if (it == g_tune.end()) {   // the tuner was reset meanwhile
+        (void)hipEventSynchronize(tr.e1);
+        (void)hipEventDestroy(tr.e0);
+        (void)hipEventDestroy(tr.e1);
+        return;
+    }
+    it->second.pending.push_back(tr);
+}
+
 GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape) {
     GemmPlan P;
     P.split32 = split32;
-    P.shape = (force_shape >= 0 && !split32) ? force_shape : pick_shape(jobs, split32);
+    if (force_shape >= 0 && !split32) P.shape = force_shape;
+    else if (split32) P.shape = GEMM_64x64;
+    else P.shape = tuned_shape(ctx, jobs, pick_shape(jobs, false), &P.tune_key, &P.trial);
     const int BMs = P.shape == GEMM_64x64 ? 64 : 128;
     const int BNs = P.shape == GEMM_128x128 ? 128 : 64;
     size_t work_elems = 0;
@@ -416,10 +552,15 @@ void gemm_bind(GemmPlan& p, const DescArena& arena) {
     p.d_starts = arena.at<int64_t>(p.off_starts);
 }
 
-void gemm_issue(Ctx& ctx, const GemmPlan& p) {
+void gemm_issue(Ctx& ctx, GemmPlan& p) {
     if (p.descs.empty()) return;
+    TuneTrial tr;
+    const bool trial = p.trial;
+    p.trial = false;   // one timing per plan
+    if (trial) tune_record_begin(ctx, p, tr);
     launch_gemm(ctx.stream, p.d_descs, (int)p.descs.size(), p.tiles, p.split32, p.flops,
                 p.shape);
+    if (trial) tune_record_end(ctx, p, tr);
     if (!p.red.empty())
         launch_gemm_reduce(ctx.stream, p.d_red, (int)p.red.size(), p.d_starts, p.red_total);
 }
@@ -641,6 +782,40 @@ void gather_many(Ctx& ctx, std::vector<GatherDesc> descs) {
     if (d.empty()) return;
     GatherDesc* dd = ctx.stage->put(ctx.stream, d);
     launch_gather(ctx.stream, dd, (int)d.size(), start);
+}
+
+void readback_vectors(Ctx& ctx, const std::vector<const double*>& src,
+                      const std::vector<int>& cnt, const std::vector<std::vector<double>*>& dst) {
+    size_t total = 0;
+    for (size_t p = 0; p < src.size(); ++p) {
+        dst[p]->resize(std::max(cnt[p], 0));
+        total += (size_t)std::max(cnt[p], 0);
+    }
+    if (!total) return;
+    Buf packed(&ctx.pool, total * sizeof(double));
+    std::vector<GatherDesc> gd;
+    size_t o = 0;
+    for (size_t p = 0; p < src.size(); ++p) {
+        if (cnt[p] <= 0) continue;
+        GatherDesc g{};
+        g.in = src[p];
+        g.out = packed.d() + o;
+        g.rows = cnt[p];
+        g.cols = 1;
+        g.ldi = cnt[p];
+        g.ldo = cnt[p];
+        gd.push_back(g);
+        o += (size_t)cnt[p];
+    }
+    gather_many(ctx, gd);
+    std::vector<double> h(total);
+    ctx.fetch(h.data(), packed.d(), total * sizeof(double));
+    o = 0;
+    for (size_t p = 0; p < src.size(); ++p) {
+        if (cnt[p] <= 0) continue;
+        std::copy(h.begin() + o, h.begin() + o + cnt[p], dst[p]->begin());
+        o += (size_t)cnt[p];
+    }
 }
 
 // ---------------------------------------------------------- ABI transfer

@@ -176,6 +176,8 @@ struct GemmPlan {
     double flops = 0;
     bool split32 = false;
     int shape = 0;                     // GemmShape of the launch
+    bool trial = false;                // next issue is timed for the autotuner
+    std::array<int, 9> tune_key{};
     Buf work;
     size_t off_descs = 0, off_red = 0, off_starts = 0;
     const GemmDesc* d_descs = nullptr;
@@ -187,7 +189,10 @@ GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32 = fa
                    int force_shape = -1);
 void gemm_stage(GemmPlan& p, DescArena& arena);          // before arena.commit
 void gemm_bind(GemmPlan& p, const DescArena& arena);     // after arena.commit
-void gemm_issue(Ctx& ctx, const GemmPlan& p);
+void gemm_issue(Ctx& ctx, GemmPlan& p);
+// on = 0 freezes tile shapes to the static rule; any call clears what the
+// autotuner has learned
+void gemm_autotune_enable(int on);
 
 // n = |sum T[x1, x2, x2, x1] w1[x1] w2[x2]| (legs 1<->4, 2<->3), written to
 // the device slot; then T /= n, all on the device; grouped over tensors.
@@ -201,6 +206,10 @@ void trace_normalize_many(Ctx& ctx, const std::vector<TraceJob>& jobs);
 
 // Grouped strided matrix copies with optional row/column weights.
 void gather_many(Ctx& ctx, std::vector<GatherDesc> descs);
+// cnt[p] doubles at device src[p] into *dst[p]: packed on the device, one
+// copy to the host, one synchronisation
+void readback_vectors(Ctx& ctx, const std::vector<const double*>& src,
+                      const std::vector<int>& cnt, const std::vector<std::vector<double>*>& dst);
 
 // Upload / download in the ABI format (ncod = 2, 4 legs, sector dims per
 // space label).

@@ -1,13 +1,17 @@
 // extern "C" entry points of libtnr_hip.so (include/tnr.h).  No exception
 // crosses this boundary: every call maps failures to a status code and keeps
 // the message for tnr_last_error().
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
 #include <new>
+#include <thread>
 
 #include "schemes.h"
 
@@ -258,88 +262,184 @@ struct Slot {
 
 }  // namespace
 
-struct tnr_server {
+// A server spreads its resident schemes over a few lanes.  Each lane has its
+// own stream, memory pool and staging ring and is driven by its own host
+// thread during a pump, so one lane's host work (planning, the truncation
+// cut between the SVD and the contractions) and its kernels overlap the
+// other lanes' kernels.  Every computation is local to one scheme, so the
+// lane a scheme lands in changes nothing in its results.
+struct Lane {
     Ctx ctx;
+    std::vector<std::unique_ptr<Slot>> active;
+    double load = 0;   // estimated work of the resident schemes
+};
+
+static double work_estimate(const Req& r) {
+    const double c = (double)r.chi;
+    return (double)steps_of(r.maxiter) * c * c * c * c * c * c / (double)r.zn;
+}
+
+static int default_lanes() {
+    const char* e = std::getenv("TNR_LANES");
+    int n = e ? std::atoi(e) : 2;
+    return std::max(1, std::min(n, 4));
+}
+
+struct tnr_server {
+    std::vector<std::unique_ptr<Lane>> lanes;
     size_t budget = 0, in_use = 0;
     int max_slots = 1;
     int64_t next_id = 1;
     std::map<int64_t, std::unique_ptr<Req>> reqs;
     std::deque<Req*> queue;
-    std::vector<std::unique_ptr<Slot>> active;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
+    explicit tnr_server(int nlanes, int precision) {
+        for (int i = 0; i < nlanes; ++i) {
+            lanes.emplace_back(new Lane());
+            lanes.back()->ctx.precision = precision;
+        }
+    }
+
+    int resident() const {
+        int n = 0;
+        for (auto& l : lanes) n += (int)l->active.size();
+        return n;
+    }
+
+    size_t pool_in_use() const {
+        size_t n = 0;
+        for (auto& l : lanes) n += l->ctx.pool.in_use();
+        return n;
+    }
+
     void admit() {
-        std::vector<Slot*> fresh;
-        while (!queue.empty() && (int)active.size() < max_slots) {
+        std::vector<std::vector<Slot*>> fresh(lanes.size());
+        int res = resident();
+        while (!queue.empty() && res < max_slots) {
             Req* r = queue.front();
-            if (!active.empty() && in_use + r->est > budget) break;   // wait for memory
+            if (res > 0 && in_use + r->est > budget) break;   // wait for memory
             queue.pop_front();
+            size_t li = 0;
+            for (size_t i = 1; i < lanes.size(); ++i)
+                if (lanes[i]->load < lanes[li]->load) li = i;
+            Lane& L = *lanes[li];
             std::unique_ptr<Slot> sl(new Slot());
             sl->r = r;
             sl->nsteps = steps_of(r->maxiter);
             const int n = sl->nsteps + (r->fb ? 1 : 0);
             try {
-                BT T = upload(ctx, r->zn, r->dims.data(), r->data.data(), r->data.size());
-                scheme_init(ctx, sl->s, r->kind, std::move(T), r->k);
-                sl->norms = Buf(&ctx.pool, (size_t)n * sizeof(double));
+                BT T = upload(L.ctx, r->zn, r->dims.data(), r->data.data(), r->data.size());
+                scheme_init(L.ctx, sl->s, r->kind, std::move(T), r->k);
+                sl->norms = Buf(&L.ctx.pool, (size_t)n * sizeof(double));
             } catch (const Error& e) {
                 r->status = e.code;
                 r->t_done = now_s(t0);
                 continue;
             }
             in_use += r->est;
-            fresh.push_back(sl.get());
-            active.push_back(std::move(sl));
+            L.load += work_estimate(*r);
+            fresh[li].push_back(sl.get());
+            L.active.push_back(std::move(sl));
+            ++res;
         }
-        // finalize at the beginning, grouped over the newly admitted
+        // finalize at the beginning, grouped over the newly admitted of a lane
+        for (size_t li = 0; li < lanes.size(); ++li) {
+            std::vector<Scheme*> g;
+            std::vector<double*> slots;
+            for (Slot* sl : fresh[li])
+                if (sl->r->fb) {
+                    g.push_back(&sl->s);
+                    slots.push_back(sl->norms.d() + sl->written++);
+                }
+            if (!g.empty()) finalize_many(lanes[li]->ctx, g, slots);
+        }
+    }
+
+    // one coarse-graining step of every resident scheme of a lane
+    static void step_lane(Lane& L) {
+        if (L.active.empty()) return;
         std::vector<Scheme*> g;
+        std::vector<int> chis;
         std::vector<double*> slots;
-        for (Slot* sl : fresh)
-            if (sl->r->fb) {
-                g.push_back(&sl->s);
-                slots.push_back(sl->norms.d() + sl->written++);
-            }
-        if (!g.empty()) finalize_many(ctx, g, slots);
+        for (auto& sl : L.active) {
+            g.push_back(&sl->s);
+            chis.push_back(sl->r->chi);
+        }
+        step_many(L.ctx, g, chis);
+        for (auto& sl : L.active) {
+            slots.push_back(sl->norms.d() + sl->written++);
+            sl->done++;
+        }
+        finalize_many(L.ctx, g, slots);
+    }
+
+    void step_all() {
+        std::vector<Lane*> busy;
+        for (auto& l : lanes)
+            if (!l->active.empty()) busy.push_back(l.get());
+        if (busy.size() <= 1) {
+            if (!busy.empty()) step_lane(*busy[0]);
+            return;
+        }
+        std::vector<std::thread> th;
+        std::vector<std::exception_ptr> err(busy.size());
+        for (size_t i = 1; i < busy.size(); ++i)
+            th.emplace_back([&, i] {
+                try {
+                    TNR_HIP(hipSetDevice(busy[i]->ctx.device));
+                    step_lane(*busy[i]);
+                } catch (...) {
+                    err[i] = std::current_exception();
+                }
+            });
+        try {
+            step_lane(*busy[0]);
+        } catch (...) {
+            err[0] = std::current_exception();
+        }
+        for (auto& t : th) t.join();
+        for (auto& e : err)
+            if (e) std::rethrow_exception(e);
     }
 
     void retire() {
-        std::vector<std::unique_ptr<Slot>> keep;
-        for (auto& sl : active) {
-            if (sl->done < sl->nsteps) {
-                keep.push_back(std::move(sl));
-                continue;
+        for (auto& lp : lanes) {
+            Lane& L = *lp;
+            std::vector<std::unique_ptr<Slot>> keep;
+            for (auto& sl : L.active) {
+                if (sl->done < sl->nsteps) {
+                    keep.push_back(std::move(sl));
+                    continue;
+                }
+                Req* r = sl->r;
+                r->norms.resize(sl->written);
+                L.ctx.fetch(r->norms.data(), sl->norms.d(), sl->written * sizeof(double));
+                r->status = TNR_OK;
+                for (double v : r->norms)
+                    if (!std::isfinite(v) || v == 0.0) r->status = TNR_ERR_NONFINITE;
+                r->t_done = now_s(t0);
+                in_use -= r->est;
+                L.load -= work_estimate(*r);
             }
-            Req* r = sl->r;
-            r->norms.resize(sl->written);
-            ctx.fetch(r->norms.data(), sl->norms.d(), sl->written * sizeof(double));
-            r->status = TNR_OK;
-            for (double v : r->norms)
-                if (!std::isfinite(v) || v == 0.0) r->status = TNR_ERR_NONFINITE;
-            r->t_done = now_s(t0);
-            in_use -= r->est;
+            L.active.swap(keep);
+            if (L.active.empty()) L.load = 0;
         }
-        active.swap(keep);
     }
 
     int pump() {
         admit();
-        if (!active.empty()) {
-            std::vector<Scheme*> g;
-            std::vector<int> chis;
-            std::vector<double*> slots;
-            for (auto& sl : active) {
-                g.push_back(&sl->s);
-                chis.push_back(sl->r->chi);
-            }
-            step_many(ctx, g, chis);
-            for (auto& sl : active) {
-                slots.push_back(sl->norms.d() + sl->written++);
-                sl->done++;
-            }
-            finalize_many(ctx, g, slots);
-        }
+        step_all();
         retire();
-        return (int)(active.size() + queue.size());
+        return resident() + (int)queue.size();
+    }
+
+    void sync_all() {
+        for (auto& l : lanes) l->ctx.sync();
+    }
+
+    void clear() {
+        for (auto& l : lanes) l->active.clear();
     }
 };
 
@@ -352,8 +452,7 @@ int tnr_server_create(tnr_server** out, size_t hbm_budget_bytes, int max_slots, 
         if (max_slots < 1) return fail(TNR_ERR_ARG, "max_slots must be >= 1");
         if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
             return fail(TNR_ERR_ARG, "unknown precision");
-        std::unique_ptr<tnr_server> s(new tnr_server());
-        s->ctx.precision = precision;
+        std::unique_ptr<tnr_server> s(new tnr_server(default_lanes(), precision));
         size_t fr = 0, tot = 0;
         TNR_HIP(hipMemGetInfo(&fr, &tot));
         s->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(0.8 * (double)fr);
@@ -418,8 +517,8 @@ int tnr_server_poll(tnr_server* s, int64_t id, double* norms, int* n_out, double
 int tnr_server_stats(tnr_server* s, size_t* hbm_in_use, int* resident, int* queued) {
     return guarded([&] {
         if (!s) return fail(TNR_ERR_ARG, "null handle");
-        if (hbm_in_use) *hbm_in_use = s->ctx.pool.in_use();
-        if (resident) *resident = (int)s->active.size();
+        if (hbm_in_use) *hbm_in_use = s->pool_in_use();
+        if (resident) *resident = s->resident();
         if (queued) *queued = (int)s->queue.size();
         return TNR_OK;
     });
@@ -428,8 +527,8 @@ int tnr_server_stats(tnr_server* s, size_t* hbm_in_use, int* resident, int* queu
 int tnr_server_destroy(tnr_server* s) {
     return guarded([&] {
         if (s) {
-            s->ctx.sync();
-            s->active.clear();
+            s->sync_all();
+            s->clear();
         }
         delete s;
         return TNR_OK;
@@ -449,7 +548,7 @@ int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision, double* no
             if (st != TNR_OK) return st;
         }
         int active = 1;
-        while (active > 0) s->pump(), active = (int)(s->active.size() + s->queue.size());
+        while (active > 0) active = s->pump();
         size_t o = 0;
         for (int i = 0; i < nruns; ++i) {
             Req& r = *s->reqs[ids[i]];
@@ -460,7 +559,7 @@ int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision, double* no
             std::memcpy(norms + o, r.norms.data(), r.norms.size() * sizeof(double));
             o += r.norms.size();
         }
-        s->active.clear();
+        s->clear();
         return TNR_OK;
     });
 }

@@ -119,8 +119,13 @@ struct Ctx {
     Ctx();
     ~Ctx();
     void sync();
-    // device -> host copy that waits for the stream
+    // device -> host copy that waits for the stream (through a pinned
+    // bounce buffer, so the DMA engine writes host memory directly)
     void fetch(void* dst, const void* src, size_t bytes);
+
+  private:
+    void* bounce_ = nullptr;
+    size_t bounce_cap_ = 0;
 };
 
 void ensure_device();   // throws TNR_ERR_NODEVICE if no gfx950 GPU

@@ -72,6 +72,13 @@ int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta, const
     });
 }
 
+int tnr_autotune_enable(int on) {
+    return kguard([&] {
+        gemm_autotune_enable(on);
+        return TNR_OK;
+    });
+}
+
 int tnr_kernel_svd(int m, int n, const double* M, int k, double* U, double* s, double* V,
                    int* iters) {
     return kguard([&] {

@@ -156,8 +156,10 @@ struct ProfScope {
     ProfScope(hipStream_t st, int kind, double flops, double bytes);
     ~ProfScope();
     hipStream_t st;
-    int kind, slot;
+    int kind;
+    bool active;
     double flops, bytes;
+    hipEvent_t ea = nullptr, eb = nullptr;
 };
 struct ProfTotals { int64_t launches = 0; double ms = 0, flops = 0, bytes = 0; };
 ProfTotals prof_read(int kind);   // synchronises the device

@@ -9,6 +9,8 @@
 // run and alone vs batched.
 #include "kernels.h"
 
+#include <mutex>
+
 #include <algorithm>
 #include <vector>
 
@@ -723,18 +725,19 @@ __global__ __launch_bounds__(256) void k_frob(const ColDesc* __restrict__ descs,
 // ================================================================ profiling
 namespace {
 struct ProfState {
+    std::mutex mu;
     bool on = false;
     struct Rec { hipEvent_t a, b; int kind; double flops, bytes; };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> spare;
     ProfTotals tot[PK_COUNT];
-    hipEvent_t ev() {
+    hipEvent_t ev() {   // with mu held
         if (!spare.empty()) { hipEvent_t e = spare.back(); spare.pop_back(); return e; }
         hipEvent_t e;
         TNR_HIP(hipEventCreate(&e));
         return e;
     }
-    void drain() {
+    void drain() {      // with mu held
         for (auto& r : recs) {
             TNR_HIP(hipEventSynchronize(r.b));
             float ms = 0;
@@ -757,31 +760,48 @@ ProfState& prof() {
 
 void prof_enable(bool on) {
     ProfState& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
     p.drain();
     p.on = on;
     for (auto& t : p.tot) t = ProfTotals();
 }
 
-bool prof_enabled() { return prof().on; }
-
-ProfScope::ProfScope(hipStream_t s, int k, double f, double b)
-    : st(s), kind(k), slot(-1), flops(f), bytes(b) {
+bool prof_enabled() {
     ProfState& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    return p.on;
+}
+
+// Launches may come from several host threads (one stream each): the
+// events live in the scope object and the record is filed at its end.
+ProfScope::ProfScope(hipStream_t s, int k, double f, double b)
+    : st(s), kind(k), active(false), flops(f), bytes(b) {
+    ProfState& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
     if (!p.on) return;
-    if (p.recs.size() > 4096) p.drain();
-    ProfState::Rec r{p.ev(), p.ev(), k, f, b};
-    TNR_HIP(hipEventRecord(r.a, st));
-    slot = (int)p.recs.size();
-    p.recs.push_back(r);
+    ea = p.ev();
+    eb = p.ev();
+    active = true;
+    TNR_HIP(hipEventRecord(ea, st));
 }
 
 ProfScope::~ProfScope() {
-    if (slot < 0) return;
-    (void)hipEventRecord(prof().recs[slot].b, st);
+    if (!active) return;
+    (void)hipEventRecord(eb, st);
+    ProfState& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    p.recs.push_back(ProfState::Rec{ea, eb, kind, flops, bytes});
+    if (p.recs.size() > 4096) {
+        try {
+            p.drain();
+        } catch (...) {
+        }
+    }
 }
 
 ProfTotals prof_read(int kind) {
     ProfState& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
     p.drain();
     return (kind >= 0 && kind < PK_COUNT) ? p.tot[kind] : ProfTotals();
 }
@@ -855,13 +875,11 @@ size_t jacobi_lds_bytes(int n) {
 void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, double tol,
                    int max_sweeps) {
     if (nd <= 0) return;
-    static bool attr = false;
-    if (!attr) {
-        TNR_HIP(hipFuncSetAttribute((const void*)k_jacobi,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)jacobi_lds_bytes(LMAX)));
-        attr = true;
-    }
+    // once per process (thread-safe static initialisation)
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (int)jacobi_lds_bytes(LMAX));
+    TNR_HIP(attr);
     ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
     hipLaunchKernelGGL(k_jacobi, dim3(nd), dim3(JT), jacobi_lds_bytes(nmax), st, d_descs, nd, tol,
                        max_sweeps);

@@ -1,4 +1,5 @@
 // Device pool, staging ring and execution context.
+#include <algorithm>
 #include <cstring>
 
 #include "common.h"
@@ -128,14 +129,25 @@ Ctx::~Ctx() {
         stage.reset();
         (void)hipStreamDestroy(stream);
     }
+    if (bounce_) (void)hipHostFree(bounce_);
 }
 
 void Ctx::sync() { TNR_HIP(hipStreamSynchronize(stream)); }
 
 void Ctx::fetch(void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
-    TNR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    if (bytes > bounce_cap_) {
+        TNR_HIP(hipStreamSynchronize(stream));
+        if (bounce_) TNR_HIP(hipHostFree(bounce_));
+        bounce_ = nullptr;
+        bounce_cap_ = 0;
+        const size_t cap = std::max<size_t>(bytes, 1 << 16);
+        TNR_HIP(hipHostMalloc(&bounce_, cap, hipHostMallocDefault));
+        bounce_cap_ = cap;
+    }
+    TNR_HIP(hipMemcpyAsync(bounce_, src, bytes, hipMemcpyDeviceToHost, stream));
     TNR_HIP(hipStreamSynchronize(stream));
+    std::memcpy(dst, bounce_, bytes);
 }
 
 }  // namespace tnr

@@ -107,7 +107,7 @@ void rr_bind(RRPlan& P, const DescArena& a) {
     P.d_col = a.at<ColDesc>(P.o_col);
 }
 
-void rr_issue(Ctx& ctx, const RRPlan& P) {
+void rr_issue(Ctx& ctx, RRPlan& P) {
     const SolverCfg& cfg = solver_cfg();
     for (int pass = 0; pass < P.passes; ++pass) {
         gemm_issue(ctx, P.gram[pass]);
@@ -337,13 +337,16 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             o.conv = prods[i];
         }
     }
-    // singular values to the host (all queued, one synchronisation)
+    // singular values to the host: packed on the device, one copy
+    std::vector<const double*> src(in.size());
+    std::vector<int> cnt(in.size());
     for (size_t p = 0; p < in.size(); ++p) {
-        out[p].s_host.resize(in[p].kmax);
-        TNR_HIP(hipMemcpyAsync(out[p].s_host.data(), out[p].s.d(), in[p].kmax * 8,
-                               hipMemcpyDeviceToHost, ctx.stream));
+        src[p] = out[p].s.d();
+        cnt[p] = in[p].kmax;
     }
-    ctx.sync();
+    std::vector<std::vector<double>*> dst(in.size());
+    for (size_t p = 0; p < in.size(); ++p) dst[p] = &out[p].s_host;
+    readback_vectors(ctx, src, cnt, dst);
 }
 
 // ---------------------------------------------------------------- eigh
@@ -475,12 +478,15 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             out[iter[i]].conv = prods[i];
         }
     }
+    std::vector<const double*> lsrc(in.size());
+    std::vector<int> lcnt(in.size());
+    std::vector<std::vector<double>*> ldst(in.size());
     for (size_t p = 0; p < in.size(); ++p) {
-        out[p].lam_host.resize(out[p].ncols);
-        TNR_HIP(hipMemcpyAsync(out[p].lam_host.data(), out[p].lam.d(), out[p].ncols * 8,
-                               hipMemcpyDeviceToHost, ctx.stream));
+        lsrc[p] = out[p].lam.d();
+        lcnt[p] = out[p].ncols;
+        ldst[p] = &out[p].lam_host;
     }
-    ctx.sync();
+    readback_vectors(ctx, lsrc, lcnt, ldst);
 }
 
 }  // namespace tnr
