@@ -155,9 +155,11 @@ bool prof_enabled();
 struct ProfScope {
     ProfScope(hipStream_t st, int kind, double flops, double bytes);
     ~ProfScope();
+    void end();                       // record the closing event now
     hipStream_t st;
     int kind;
     bool active;
+    bool ended = false;
     double flops, bytes;
     hipEvent_t ea = nullptr, eb = nullptr;
 };

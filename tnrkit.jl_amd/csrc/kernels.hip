@@ -476,13 +476,10 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
     const int ld = np + 1;              // padded column stride (bank spread)
     double* G = lds;                    // np x np, G[i + j*ld]
     double* W = G + np * ld;
-    double* rc = W + np * ld;           // c, s, t per pair
-    double* rs = rc + LMAX / 2;
-    double* rt = rs + LMAX / 2;
-    int* P = (int*)(rt + LMAX / 2);     // pair rows
-    int* Q = P + LMAX / 2;
-    int* act = Q + LMAX / 2;
-    int* pos = act + LMAX / 2;          // output order (LMAX ints)
+    double2* rcs = (double2*)(W + np * ld);      // (c, s) per pair
+    double* rt = (double*)(rcs + LMAX / 2);      // t per pair
+    int4* pq = (int4*)(rt + LMAX / 2);           // (p, q, active, -) per pair
+    int* pos = (int*)(pq + LMAX / 2);            // output order (LMAX ints)
     const int t = threadIdx.x;
     const float inv_np = 1.0f / (float)np;
     for (int e = t; e < np * np; e += JT) {
@@ -493,7 +490,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
         W[i + j * ld] = (i == j) ? 1.0 : 0.0;
     }
     const int half = np / 2, m = np - 1;
-    const float inv_half = 1.0f / (float)half;
+    const int ntri = half * (half + 1) / 2;      // pair blocks (i <= j)
     int sweeps = 0;
     for (int sw = 0; sw < max_sweeps; ++sw) {
         // convergence scan over all pairs p < q
@@ -522,17 +519,23 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                     c = 1.0 / sqrt(1.0 + tt * tt);
                     s = tt * c;
                 }
-                P[t] = p; Q[t] = q; act[t] = on;
-                rc[t] = c; rs[t] = s; rt[t] = tt;
+                pq[t] = make_int4(p, q, on, 0);
+                rcs[t] = make_double2(c, s);
+                rt[t] = tt;
                 if (t > 0) a = (a == m) ? 1 : a + 1;
                 b = (b == m) ? 1 : b + 1;
             }
             if (!__syncthreads_or(on)) continue;
-            for (int e = t; e < half * half; e += JT) {
-                const int j = (int)(((float)e + 0.5f) * inv_half), i = e - j * half;
-                const int ai = act[i], aj = act[j];
-                if (!ai && !aj) continue;
-                const int pi = P[i], qi = Q[i];
+            // G <- J^T G J on the upper pair blocks; the lower ones are their
+            // transposes, so G stays exactly symmetric
+            for (int e = t; e < ntri; e += JT) {
+                int j = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+                while (j * (j + 1) / 2 > e) --j;
+                while ((j + 1) * (j + 2) / 2 <= e) ++j;
+                const int i = e - j * (j + 1) / 2;         // i <= j
+                const int4 Pi = pq[i], Pj = pq[j];
+                if (!Pi.z && !Pj.z) continue;
+                const int pi = Pi.x, qi = Pi.y;
                 if (i == j) {
                     const double tt = rt[i], gpq = G[pi + qi * ld];
                     G[pi + pi * ld] -= tt * gpq;
@@ -541,25 +544,29 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                     G[qi + pi * ld] = 0.0;
                     continue;
                 }
-                const int pj = P[j], qj = Q[j];
-                const double ci = rc[i], si = rs[i], cj = rc[j], sj = rs[j];
+                const int pj = Pj.x, qj = Pj.y;
+                const double2 ri = rcs[i], rj = rcs[j];
+                const double ci = ri.x, si = ri.y, cj = rj.x, sj = rj.y;
                 const double b00 = G[pi + pj * ld], b01 = G[pi + qj * ld];
                 const double b10 = G[qi + pj * ld], b11 = G[qi + qj * ld];
                 const double x00 = ci * b00 - si * b10, x01 = ci * b01 - si * b11;   // J_i^T B
                 const double x10 = si * b00 + ci * b10, x11 = si * b01 + ci * b11;
-                G[pi + pj * ld] = cj * x00 - sj * x01;                               // (.) J_j
-                G[pi + qj * ld] = sj * x00 + cj * x01;
-                G[qi + pj * ld] = cj * x10 - sj * x11;
-                G[qi + qj * ld] = sj * x10 + cj * x11;
+                const double y00 = cj * x00 - sj * x01, y01 = sj * x00 + cj * x01;   // (.) J_j
+                const double y10 = cj * x10 - sj * x11, y11 = sj * x10 + cj * x11;
+                G[pi + pj * ld] = y00; G[pj + pi * ld] = y00;
+                G[pi + qj * ld] = y01; G[qj + pi * ld] = y01;
+                G[qi + pj * ld] = y10; G[pj + qi * ld] = y10;
+                G[qi + qj * ld] = y11; G[qj + qi * ld] = y11;
             }
+            const float inv_np_w = inv_np;
             for (int e = t; e < half * np; e += JT) {
-                const int j = (int)(((float)e + 0.5f) * inv_np), r = e - j * np;
-                if (!act[j]) continue;
-                const int pj = P[j], qj = Q[j];
-                const double cj = rc[j], sj = rs[j];
-                const double wp = W[r + pj * ld], wq = W[r + qj * ld];
-                W[r + pj * ld] = cj * wp - sj * wq;
-                W[r + qj * ld] = sj * wp + cj * wq;
+                const int j = (int)(((float)e + 0.5f) * inv_np_w), r = e - j * np;
+                const int4 Pj = pq[j];
+                if (!Pj.z) continue;
+                const double2 rj = rcs[j];
+                const double wp = W[r + Pj.x * ld], wq = W[r + Pj.y * ld];
+                W[r + Pj.x * ld] = rj.x * wp - rj.y * wq;
+                W[r + Pj.y * ld] = rj.y * wp + rj.x * wq;
             }
             __syncthreads();
         }
@@ -785,9 +792,16 @@ ProfScope::ProfScope(hipStream_t s, int k, double f, double b)
     TNR_HIP(hipEventRecord(ea, st));
 }
 
+void ProfScope::end() {
+    if (active && !ended) {
+        (void)hipEventRecord(eb, st);
+        ended = true;
+    }
+}
+
 ProfScope::~ProfScope() {
     if (!active) return;
-    (void)hipEventRecord(eb, st);
+    end();
     ProfState& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
     p.recs.push_back(ProfState::Rec{ea, eb, kind, flops, bytes});
@@ -868,8 +882,8 @@ void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const Trac
 
 size_t jacobi_lds_bytes(int n) {
     const size_t np = n + (n & 1);
-    return 2 * np * (np + 1) * sizeof(double) + 3 * (LMAX / 2) * sizeof(double) +
-           (3 * (LMAX / 2) + LMAX) * sizeof(int);
+    return 2 * np * (np + 1) * sizeof(double) + (LMAX / 2) * (sizeof(double) * 3) +
+           (LMAX / 2) * sizeof(int) * 4 + LMAX * sizeof(int);
 }
 
 void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, double tol,
@@ -884,6 +898,29 @@ void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, 
     hipLaunchKernelGGL(k_jacobi, dim3(nd), dim3(JT), jacobi_lds_bytes(nmax), st, d_descs, nd, tol,
                        max_sweeps);
     TNR_HIP(hipGetLastError());
+    if (ps.active) {
+        // profiling only: the work a launch did depends on the sweeps each
+        // problem took, read back after the launch (the timing events are
+        // already recorded)
+        ps.end();
+        TNR_HIP(hipStreamSynchronize(st));
+        std::vector<JacobiDesc> hd(nd);
+        TNR_HIP(hipMemcpy(hd.data(), d_descs, nd * sizeof(JacobiDesc), hipMemcpyDeviceToHost));
+        double fl = 0.0, by = 0.0;
+        for (const JacobiDesc& d : hd) {
+            int sk = 0, sw = 0;
+            if (d.skip) TNR_HIP(hipMemcpy(&sk, d.skip, sizeof(int), hipMemcpyDeviceToHost));
+            if (sk) continue;
+            if (d.sweeps) TNR_HIP(hipMemcpy(&sw, d.sweeps, sizeof(int), hipMemcpyDeviceToHost));
+            const double n = d.n;
+            // per sweep: (n-1) rounds of n/2 rotations, each 18n flops on
+            // G (two rows, two columns) and W (two columns)
+            fl += (double)sw * (n - 1.0) * (n / 2.0) * 18.0 * n;
+            by += (2.0 * n * n + n) * sizeof(double);
+        }
+        ps.flops = fl;
+        ps.bytes = by;
+    }
 }
 
 void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, double rtol) {

@@ -107,6 +107,7 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
     svd_grouped(ctx, in, out);
     std::vector<Factor> F(Ts.size());
     std::vector<GatherDesc> gd;
+    std::vector<int64_t> gw;     // per descriptor: offset of its weights in wall
     std::vector<double> wall;    // all weight vectors, uploaded once
     size_t p0 = 0;
     for (size_t t = 0; t < Ts.size(); ++t) {
@@ -166,24 +167,26 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
             TNR_REQUIRE(f.U.L.R[c] == in[p].m && f.U.L.C[c] == kc[p] && f.V.L.R[c] == kc[p] &&
                             f.V.L.C[c] == in[p].n,
                         TNR_ERR_INTERNAL, "factor layout mismatch");
+            // U * diag(wu): column weights; diag(wv) * V^T: row weights
             GatherDesc gu{};
             gu.in = out[p].U.d(); gu.out = f.U.d() + f.U.L.off[c];
-            gu.cw = reinterpret_cast<const double*>(owu + off[c]);   // index, patched below
             gu.rows = in[p].m; gu.cols = kc[p]; gu.ldi = out[p].ldu; gu.ldo = in[p].m;
             gd.push_back(gu);
+            gw.push_back((int64_t)(owu + off[c]));
             GatherDesc gv{};
             gv.in = out[p].V.d(); gv.out = f.V.d() + f.V.L.off[c];
-            gv.rw = reinterpret_cast<const double*>(owv + off[c]);
             gv.rows = kc[p]; gv.cols = in[p].n; gv.ldi = out[p].ldv; gv.ldo = kc[p]; gv.trans = 1;
             gd.push_back(gv);
+            gw.push_back((int64_t)(owv + off[c]));
         }
         p0 = p1;
     }
+    if (gd.empty()) return F;
     const double* dw = static_cast<const double*>(
         ctx.stage->put(ctx.stream, wall.data(), std::max<size_t>(1, wall.size()) * sizeof(double)));
-    for (auto& g : gd) {
-        if (g.cw) g.cw = dw + reinterpret_cast<size_t>(g.cw);
-        if (g.rw) g.rw = dw + reinterpret_cast<size_t>(g.rw);
+    for (size_t i = 0; i < gd.size(); ++i) {
+        if (gd[i].trans) gd[i].rw = dw + gw[i];   // V side
+        else gd[i].cw = dw + gw[i];               // U side
     }
     gather_many(ctx, gd);
     return F;

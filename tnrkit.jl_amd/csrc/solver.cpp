@@ -9,11 +9,41 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 namespace tnr {
 
+// TNR_SOLVER_CFG="key=value,..." overrides the defaults (keys: tol_iter,
+// sweeps_iter, block_mult, block_extra, conv_target, max_iters)
+static SolverCfg make_cfg() {
+    SolverCfg c;
+    const char* e = std::getenv("TNR_SOLVER_CFG");
+    if (!e) return c;
+    std::string s(e);
+    size_t i = 0;
+    while (i < s.size()) {
+        size_t j = s.find(',', i);
+        if (j == std::string::npos) j = s.size();
+        const std::string kv = s.substr(i, j - i);
+        const size_t eq = kv.find('=');
+        if (eq != std::string::npos) {
+            const std::string k = kv.substr(0, eq);
+            const double v = std::atof(kv.c_str() + eq + 1);
+            if (k == "tol_iter") c.jacobi_tol_iter = v;
+            else if (k == "sweeps_iter") c.jacobi_sweeps_iter = (int)v;
+            else if (k == "block_mult") c.block_mult = v;
+            else if (k == "block_extra") c.block_extra = (int)v;
+            else if (k == "conv_target") c.conv_target = v;
+            else if (k == "max_iters") c.max_iters = (int)v;
+        }
+        i = j + 1;
+    }
+    return c;
+}
+
 SolverCfg& solver_cfg() {
-    static SolverCfg c;
+    static SolverCfg c = make_cfg();
     return c;
 }
 
@@ -111,8 +141,10 @@ void rr_issue(Ctx& ctx, RRPlan& P) {
     const SolverCfg& cfg = solver_cfg();
     for (int pass = 0; pass < P.passes; ++pass) {
         gemm_issue(ctx, P.gram[pass]);
-        launch_jacobi(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmax, cfg.jacobi_tol,
-                      cfg.jacobi_sweeps);
+        const bool in_loop = P.passes == 1;
+        launch_jacobi(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmax,
+                      in_loop ? cfg.jacobi_tol_iter : cfg.jacobi_tol,
+                      in_loop ? cfg.jacobi_sweeps_iter : cfg.jacobi_sweeps);
         gemm_issue(ctx, P.rot[pass]);
     }
     launch_colnorm(ctx.stream, P.d_col, (int)P.col.size(), cfg.null_rtol);
@@ -140,9 +172,12 @@ void randfill(Ctx& ctx, const std::vector<FillDesc>& fills) {
     launch_randfill(ctx.stream, d, (int)f.size(), start);
 }
 
+// block width of the subspace iteration: max(block_mult * k, k + block_extra),
+// at most LMAX
 int block_size(int kmax) {
-    int l = std::min(LMAX, 2 * kmax);
-    return std::max(l, std::min(LMAX, kmax + 2));
+    const SolverCfg& cfg = solver_cfg();
+    const int l = std::max((int)std::ceil(cfg.block_mult * kmax), kmax + cfg.block_extra);
+    return std::min(LMAX, std::max(l, kmax + 2 <= LMAX ? kmax + 2 : LMAX));
 }
 
 // Per-call device state of the stopping rule: it, done (ints), prod (double).
@@ -254,7 +289,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
         Buf sigV(&ctx.pool, (size_t)P * LMAX * 8);
         ConvState cs(ctx, P);
         std::vector<FillDesc> fills;
-        std::vector<RR> r0, rY, rZ, rF;
+        std::vector<RR> r0, rY, rZ, rV, rF;
         std::vector<GemmJob> jMV, jMtU, jF, jV;
         std::vector<ConvDesc> conv;
         for (int i = 0; i < P; ++i) {
@@ -290,6 +325,8 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             rZ.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip});
             conv.push_back(ConvDesc{sv, l, s.kmax, 1, cs.it(i), cs.prod(i), cs.done(i)});
+            rV.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
+                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
             // Ritz extraction from Y = M V (two passes), V <- V Wtot
             jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, l, s.n, s.ld, s.n, s.m, 0, 0});
             rF.push_back(RR{x.Y.d(), s.m, l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
@@ -320,6 +357,10 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             launch_conv(ctx.stream, d_conv, P, cfg.null_rtol, cfg.conv_target, cfg.min_iters,
                         cfg.max_iters);
         });
+        // the loop orthonormalises only to its looser Jacobi threshold: V is
+        // rebuilt from the last Z = M^T U at full accuracy before the final
+        // extraction relies on V^T V = I
+        rr_grouped(ctx, rV, 2);
         gemm_grouped(ctx, jF);
         rr_grouped(ctx, rF, 2);
         gemm_grouped(ctx, jV);
@@ -390,7 +431,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         Buf sigV(&ctx.pool, (size_t)P * LMAX * 8);
         ConvState cs(ctx, P);
         std::vector<FillDesc> fills;
-        std::vector<RR> r0, rY;
+        std::vector<RR> r0, rY, rV;
         std::vector<ColDesc> sym;
         std::vector<GemmJob> jMV, jF, jH;
         std::vector<ConvDesc> conv;
@@ -420,6 +461,8 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             rY.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip});
             conv.push_back(ConvDesc{sv, l, s.kmax, 0, cs.it(i), cs.prod(i), cs.done(i)});
+            rV.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
+                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
             jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, l, s.n, s.ld, s.n, s.n, 0, 0});
             jH.push_back(GemmJob{x.V.d(), x.Y.d(), x.G.d(), l, l, s.n, s.n, s.n, l, 1, 0});
             nmax = std::max(nmax, s.n);
@@ -444,6 +487,9 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             launch_conv(ctx.stream, d_conv, P, cfg.null_rtol, cfg.conv_target, cfg.min_iters,
                         cfg.max_iters);
         });
+        // V at full orthonormality from the last Y = M V (the loop uses a
+        // looser Jacobi threshold)
+        rr_grouped(ctx, rV, 2);
         // Ritz pairs: H = V^T (M V), eigendecompose, U = V W
         gemm_grouped(ctx, jF);
         gemm_grouped(ctx, jH);

@@ -1,0 +1,95 @@
+"""Turn rocprofv3 CSV output into the short summaries kept under profiles/.
+
+  python tnrkit.jl_amd/prof/summarize.py stats KERNEL_STATS_CSV --command CMD > profiles/X.md
+  python tnrkit.jl_amd/prof/summarize.py traffic FETCH_CSV WRITE_CSV CAL_FETCH_CSV CAL_WRITE_CSV \
+      --command CMD > profiles/traffic_X.json
+
+traffic: FETCH_SIZE / WRITE_SIZE are KiB per dispatch.  Each is scaled by the
+ratio of the known byte count of a plain device copy (copy_calibration.py:
+256 MiB read, 256 MiB written) to what the same counter reads for it.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+import re
+
+COPY_KIB = 256 * 1024
+
+
+def short(name):
+    m = re.match(r"(?:void )?tnr::(k_\w+)(<[^>]*>)?", name)
+    return (m.group(1) + (m.group(2) or "")) if m else name.split("(")[0][:48]
+
+
+def stats(path, command):
+    rows = list(csv.DictReader(open(path)))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["# rocprofv3 kernel statistics", "", "Command:", "", "```", command, "```", "",
+           "| kernel | calls | total ms | avg us | min us | max us | % |",
+           "|---|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        out.append("| %s | %s | %.3f | %.1f | %.1f | %.1f | %.2f |" % (
+            short(r["Name"]), r["Calls"], float(r["TotalDurationNs"]) / 1e6,
+            float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3,
+            100.0 * float(r["TotalDurationNs"]) / total))
+    out.append("")
+    out.append("Total kernel time: %.1f ms" % (total / 1e6))
+    return "\n".join(out)
+
+
+def _per_kernel(path):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        d[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return d
+
+
+def _copy_factor(path):
+    v = [x for k, xs in _per_kernel(path).items() if "copyBuffer" in k for x in xs]
+    return COPY_KIB / (sum(v) / len(v))
+
+
+def traffic(fetch, write, cal_fetch, cal_write, command):
+    sf, sw = _copy_factor(cal_fetch), _copy_factor(cal_write)
+    # the two passes are separate runs: average each counter over its own
+    # dispatches (a tile-shape autotuner may split GEMM launches differently)
+    per = collections.defaultdict(lambda: [0, 0.0, 0, 0.0])
+    for col, data in ((0, _per_kernel(fetch)), (2, _per_kernel(write))):
+        for name, xs in data.items():
+            for key in {short(name), short(name).split("<")[0]}:
+                per[key][col] += len(xs)
+                per[key][col + 1] += sum(xs)
+    kernels = {}
+    for key, (nf, f, nw, w) in per.items():
+        if key.startswith("__amd") or nf == 0 or nw == 0:
+            continue
+        fb, wb = f / nf * sf * 1024, w / nw * sw * 1024
+        kernels[key] = {"dispatches": nf, "fetch_bytes_per_launch": fb,
+                        "write_bytes_per_launch": wb, "bytes_per_launch": fb + wb}
+    return json.dumps({"command": command, "fetch_scale": sf, "write_scale": sw,
+                       "calibration": "256 MiB device copy (copy_calibration.py)",
+                       "kernels": kernels}, indent=1, sort_keys=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("--command", required=True)
+    t = sub.add_parser("traffic")
+    for x in ("fetch", "write", "cal_fetch", "cal_write"):
+        t.add_argument(x)
+    t.add_argument("--command", required=True)
+    a = ap.parse_args()
+    if a.mode == "stats":
+        print(stats(a.csv, a.command))
+    else:
+        print(traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.command))
+
+
+if __name__ == "__main__":
+    main()
