@@ -17,9 +17,11 @@
 extern "C" {
 #endif
 
-/* C = alpha op(A) op(B) + beta C, column-major FP64.  mode: 0 = FP64 MFMA
- * with the tile shape the engine would choose, 1 = FP32-MFMA hi/lo split
- * kernel, 2 / 3 / 4 = FP64 MFMA with 64x64 / 128x64 / 128x128 tiles.
+/* C = alpha op(A) op(B) + beta C, column-major FP64.  mode & 7: 0 = FP64
+ * MFMA with the tile shape the engine would choose, 1 = FP32-MFMA hi/lo
+ * split kernel, 2 / 3 / 4 = FP64 MFMA with 64x64 / 128x64 / 128x128 tiles;
+ * mode | 8 never splits K over workgroups, mode | 16 always does (for K of
+ * two or more 256-deep chunks); the default decides by launch size.
  * repeat >= 1 launches the product that many times on device-resident
  * copies; *ms_out (optional) receives the device time of the last launch
  * from HIP events. */

@@ -58,6 +58,25 @@ def test_gemm_tile_shapes_agree_bitwise(tk, M, N, K, ta, tb):
     assert np.max(np.abs(outs[0] - ref)) <= 1e-14 * (np.sqrt(K) + 1) * 8
 
 
+@pytest.mark.parametrize("M,N,K,ta,tb", [
+    (64, 64, 1024, 1, 0), (1024, 64, 1024, 0, 0), (100, 37, 777, 1, 1), (300, 130, 2049, 0, 1)])
+def test_gemm_split_k_does_not_change_bits(tk, M, N, K, ta, tb):
+    """K is summed in 256-deep chunks added in order whether one workgroup
+    runs all chunks or each chunk has its own workgroup (then k_gemm_reduce
+    adds them): split and unsplit launches agree bit for bit, for every tile
+    shape."""
+    rng = np.random.default_rng(M + 7 * N + 13 * K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    Cm = rng.standard_normal((M, N))
+    outs = [gemm(tk, A, B, Cm, ta, tb, 1.1, 0.3, split=shape | flag)
+            for shape in (2, 3, 4) for flag in (8, 16)]
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    ref = 1.1 * ((A.T if ta else A) @ (B.T if tb else B)) + 0.3 * Cm
+    assert np.max(np.abs(outs[0] - ref)) <= 1e-14 * (np.sqrt(K) + 1) * 8
+
+
 @pytest.mark.parametrize("M,N,K", [(128, 128, 256), (200, 100, 1000)])
 def test_gemm_split32(tk, M, N, K):
     rng = np.random.default_rng(K)

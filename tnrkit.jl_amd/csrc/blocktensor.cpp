@@ -327,19 +327,11 @@ void DescArena::commit(Ctx& ctx) {
     base_ = static_cast<char*>(ctx.stage->put(ctx.stream, host_.data(), host_.size()));
 }
 
-// Split-K factor from the problem's own shape (64 x 64 tile count), so a
-// problem is summed in the same order alone or inside any batch and under
-// any tile shape.
-static int split_k_of(const GemmJob& j) {
-    const int64_t mn = (int64_t)((j.M + 63) / 64) * ((j.N + 63) / 64);
-    const int ktiles = (j.K + GEMM_BK - 1) / GEMM_BK;
-    int ks = 1;
-    if (mn < 128 && ktiles >= 32) {
-        ks = (int)std::min<int64_t>(ktiles / 16, std::max<int64_t>(1, 256 / mn));
-        ks = std::max(1, std::min(ks, 64));
-    }
-    return ks;
-}
+// K chunking: a function of K alone (every chunk is summed from zero and
+// the chunk sums are added in order), so whether a launch splits K over
+// workgroups changes no bits and can depend on the whole launch.
+static int kchunk_of(int K) { return K >= 512 ? 256 : std::max(GEMM_BK, (K + 15) / 16 * 16); }
+static int nchunks_of(int K) { return K >= 512 ? (K + 255) / 256 : 1; }
 
 // Tile shape of one grouped launch: the largest tile that still gives the
 // chip >= 4 workgroups per CU and pads the problems by at most 15 % more
@@ -352,7 +344,7 @@ static int pick_shape(const std::vector<GemmJob>& jobs, bool split32) {
     int64_t tiles[3] = {0, 0, 0};
     for (const GemmJob& j : jobs) {
         if (j.M <= 0 || j.N <= 0) continue;
-        const int ks = split_k_of(j);
+        const int ks = nchunks_of(j.K);
         for (int s = 0; s < 3; ++s) {
             const int64_t tm = (j.M + bm[s] - 1) / bm[s], tn = (j.N + bn[s] - 1) / bn[s];
             tiles[s] += tm * tn * ks;
@@ -492,7 +484,8 @@ static void tune_record_end(Ctx& ctx, const GemmPlan& p, TuneTrial& tr) {
     it->second.pending.push_back(tr);
 }
 
-GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape) {
+GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape,
+                   int force_split) {
     GemmPlan P;
     P.split32 = split32;
     if (force_shape >= 0 && !split32) P.shape = force_shape;
@@ -500,6 +493,13 @@ GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int
     else P.shape = tuned_shape(ctx, jobs, pick_shape(jobs, false), &P.tune_key, &P.trial);
     const int BMs = P.shape == GEMM_64x64 ? 64 : 128;
     const int BNs = P.shape == GEMM_128x128 ? 128 : 64;
+    // split K over workgroups only when the launch is too small to fill
+    // the chip without it (results are the same either way)
+    int64_t whole = 0;
+    for (const GemmJob& j : jobs)
+        if (j.M > 0 && j.N > 0)
+            whole += (int64_t)((j.M + BMs - 1) / BMs) * ((j.N + BNs - 1) / BNs);
+    const bool split = force_split >= 0 ? force_split != 0 : whole < 512;
     size_t work_elems = 0;
     P.descs.reserve(jobs.size());
     for (const GemmJob& j : jobs) {
@@ -516,7 +516,8 @@ GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int
         g.skip = j.skip;
         g.tiles_m = (j.M + BMs - 1) / BMs;
         g.tiles_n = (j.N + BNs - 1) / BNs;
-        const int ks = split_k_of(j);
+        g.kchunk = kchunk_of(j.K);
+        const int ks = split ? nchunks_of(j.K) : 1;
         const int64_t mn = (int64_t)g.tiles_m * g.tiles_n;
         g.ksplit = ks;
         g.tile_start = P.tiles;
@@ -565,8 +566,9 @@ void gemm_issue(Ctx& ctx, GemmPlan& p) {
         launch_gemm_reduce(ctx.stream, p.d_red, (int)p.red.size(), p.d_starts, p.red_total);
 }
 
-void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape) {
-    GemmPlan p = gemm_plan(ctx, jobs, split32, force_shape);
+void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape,
+                  int force_split) {
+    GemmPlan p = gemm_plan(ctx, jobs, split32, force_shape, force_split);
     if (p.descs.empty()) return;
     DescArena arena;
     gemm_stage(p, arena);

@@ -147,7 +147,7 @@ struct GemmJob {
     const int* skip = nullptr;       // device flag: problem finished
 };
 void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32 = false,
-                  int force_shape = -1);
+                  int force_shape = -1, int force_split = -1);
 
 // Descriptor arrays gathered on the host and uploaded to the device in one
 // copy (through the staging ring); every launch queued before the ring's
@@ -184,9 +184,10 @@ struct GemmPlan {
     const GemmDesc* d_red = nullptr;
     const int64_t* d_starts = nullptr;
 };
-// force_shape >= 0 fixes the FP64 tile shape (GemmShape) instead of choosing it
+// force_shape >= 0 fixes the FP64 tile shape (GemmShape) instead of choosing
+// it; force_split 0 / 1 turns splitting K over workgroups off / on instead
 GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32 = false,
-                   int force_shape = -1);
+                   int force_shape = -1, int force_split = -1);
 void gemm_stage(GemmPlan& p, DescArena& arena);          // before arena.commit
 void gemm_bind(GemmPlan& p, const DescArena& arena);     // after arena.commit
 void gemm_issue(Ctx& ctx, GemmPlan& p);

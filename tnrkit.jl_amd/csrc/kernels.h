@@ -44,7 +44,13 @@ struct GemmDesc {
     int32_t ta, tb;               // 1 = operand stored transposed
     double alpha, beta;
     int32_t tiles_m, tiles_n;
-    int32_t ksplit;               // >1: partial products into `work`
+    // K is summed in chunks of kchunk (a function of K alone): each chunk
+    // is accumulated from zero and the chunk sums are added in order, so
+    // ksplit = 1 (one workgroup runs all chunks) and ksplit = #chunks (one
+    // chunk per workgroup, partials added in order by k_gemm_reduce) give
+    // identical bits
+    int32_t kchunk;
+    int32_t ksplit;               // 1 or #chunks: partial products into `work`
     int64_t tile_start;
     double* work;                 // ksplit partials, each M*N (ld = M)
     const int* skip;              // optional: problem finished, do nothing

@@ -134,10 +134,13 @@ __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local
     int kz = (int)(rest / g.tiles_n);
     c.m0 = tm * BM;
     c.n0 = tn * BN;
-    int ktiles = (g.K + BK - 1) / BK;
-    int per = (ktiles + g.ksplit - 1) / g.ksplit;
-    c.kbeg = kz * per * BK;
-    c.kend = min(g.K, (kz + 1) * per * BK);
+    if (g.ksplit > 1) {           // one K chunk per workgroup
+        c.kbeg = kz * g.kchunk;
+        c.kend = min(g.K, (kz + 1) * g.kchunk);
+    } else {
+        c.kbeg = 0;
+        c.kend = g.K;
+    }
     c.kz = kz;
     return c;
 }
@@ -250,11 +253,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
     const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
     if (g.skip && *g.skip) return;
     const TileCoord tc = tile_coord<BM, BN>(g, (int64_t)blockIdx.x - g.tile_start);
-    dbl4 acc[MI][NI];
+    dbl4 acc[MI][NI], tot[MI][NI];
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int ni = 0; ni < NI; ++ni) {
+            acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+            tot[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
     double ra[BM / 16], rb[BN / 16];
     if (tc.kbeg < tc.kend) {
         load_a<BM>(g, tc.m0, tc.kbeg, t, ra);
@@ -283,9 +289,20 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
                     acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[ni], acc[mi][ni],
                                                                        0, 0, 0);
         }
+        // end of a K chunk: add its sum to the running total (chunk order)
+        if ((k0 + BK) % g.kchunk == 0 || k0 + BK >= tc.kend) {
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) tot[mi][ni][r] += acc[mi][ni][r];
+                    acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+                }
+        }
         __syncthreads();
     }
-    write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, acc);
+    write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, tot);
 }
 
 // ------------------------------------------------ FP32 MFMA, hi/lo split
@@ -300,11 +317,14 @@ __global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict
     const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
     if (g.skip && *g.skip) return;
     const TileCoord tc = tile_coord<64, 64>(g, (int64_t)blockIdx.x - g.tile_start);
-    dbl4 acc[2][2];
+    dbl4 acc[2][2], tot[2][2];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int ni = 0; ni < 2; ++ni) {
+            acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+            tot[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
     double ra[4], rb[4];
     for (int k0 = tc.kbeg; k0 < tc.kend; k0 += BK) {
         load_a<64>(g, tc.m0, k0, t, ra);
@@ -366,9 +386,19 @@ __global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     acc[mi][ni][r] += (double)hh[mi][ni][r] + (double)cr[mi][ni][r];
+        if ((k0 + BK) % g.kchunk == 0 || k0 + BK >= tc.kend) {
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) tot[mi][ni][r] += acc[mi][ni][r];
+                    acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+                }
+        }
         __syncthreads();
     }
-    write_tile<false, 2, 2>(g, tc, lane, wm * 32, wn * 32, acc);
+    write_tile<false, 2, 2>(g, tc, lane, wm * 32, wn * 32, tot);
 }
 
 // sum ksplit partials in fixed order, apply alpha/beta
