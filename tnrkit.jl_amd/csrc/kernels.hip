@@ -145,76 +145,77 @@ __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local
     return c;
 }
 
-// global -> registers: the tile's BM x BK slab of op(A) (RA values per thread)
-template <int BM>
+// global -> registers: the tile's BM x BK slab of op(A), 16 BM / NT values
+// per thread of an NT-thread workgroup
+template <int BM, int NT>
 __device__ __forceinline__ void load_a(const GemmDesc& g, int m0, int k0, int t,
-                                       double (&ra)[BM / 16]) {
-    constexpr int R = BM / 16;
+                                       double (&ra)[16 * BM / NT]) {
+    constexpr int R = 16 * BM / NT;
     if (!g.ta) {  // A[i + k*lda], M x K: thread -> (i, k-group)
-        const int i = m0 + (t % BM), kk = t / BM;      // kk < 256/BM
+        const int i = m0 + (t % BM), kk = t / BM;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int k = k0 + kk + (256 / BM) * r;
+            const int k = k0 + kk + (NT / BM) * r;
             ra[r] = (i < g.M && k < g.K) ? g.A[(int64_t)i + (int64_t)k * g.lda] : 0.0;
         }
     } else {      // stored K x M: A[k + i*lda]
         const int kk = t & 15, i0 = t >> 4;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int i = m0 + i0 + 16 * r, k = k0 + kk;
+            const int i = m0 + i0 + (NT / 16) * r, k = k0 + kk;
             ra[r] = (i < g.M && k < g.K) ? g.A[(int64_t)k + (int64_t)i * g.lda] : 0.0;
         }
     }
 }
 
-template <int BM>
-__device__ __forceinline__ void store_a(const GemmDesc& g, int t, const double (&ra)[BM / 16],
+template <int BM, int NT>
+__device__ __forceinline__ void store_a(const GemmDesc& g, int t, const double (&ra)[16 * BM / NT],
                                         double (*As)[BM + 1]) {
-    constexpr int R = BM / 16;
+    constexpr int R = 16 * BM / NT;
     if (!g.ta) {
         const int i = t % BM, kk = t / BM;
 #pragma unroll
-        for (int r = 0; r < R; ++r) As[kk + (256 / BM) * r][i] = ra[r];
+        for (int r = 0; r < R; ++r) As[kk + (NT / BM) * r][i] = ra[r];
     } else {
         const int kk = t & 15, i0 = t >> 4;
 #pragma unroll
-        for (int r = 0; r < R; ++r) As[kk][i0 + 16 * r] = ra[r];
+        for (int r = 0; r < R; ++r) As[kk][i0 + (NT / 16) * r] = ra[r];
     }
 }
 
-template <int BN>
+template <int BN, int NT>
 __device__ __forceinline__ void load_b(const GemmDesc& g, int n0, int k0, int t,
-                                       double (&rb)[BN / 16]) {
-    constexpr int R = BN / 16;
+                                       double (&rb)[16 * BN / NT]) {
+    constexpr int R = 16 * BN / NT;
     if (!g.tb) {  // B[k + j*ldb], K x N
         const int kk = t & 15, j0 = t >> 4;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = n0 + j0 + 16 * r, k = k0 + kk;
+            const int j = n0 + j0 + (NT / 16) * r, k = k0 + kk;
             rb[r] = (j < g.N && k < g.K) ? g.B[(int64_t)k + (int64_t)j * g.ldb] : 0.0;
         }
     } else {      // stored N x K: B[j + k*ldb]
         const int j = n0 + (t % BN), kk = t / BN;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int k = k0 + kk + (256 / BN) * r;
+            const int k = k0 + kk + (NT / BN) * r;
             rb[r] = (j < g.N && k < g.K) ? g.B[(int64_t)j + (int64_t)k * g.ldb] : 0.0;
         }
     }
 }
 
-template <int BN>
-__device__ __forceinline__ void store_b(const GemmDesc& g, int t, const double (&rb)[BN / 16],
+template <int BN, int NT>
+__device__ __forceinline__ void store_b(const GemmDesc& g, int t, const double (&rb)[16 * BN / NT],
                                         double (*Bs)[BN + 1]) {
-    constexpr int R = BN / 16;
+    constexpr int R = 16 * BN / NT;
     if (!g.tb) {
         const int kk = t & 15, j0 = t >> 4;
 #pragma unroll
-        for (int r = 0; r < R; ++r) Bs[kk][j0 + 16 * r] = rb[r];
+        for (int r = 0; r < R; ++r) Bs[kk][j0 + (NT / 16) * r] = rb[r];
     } else {
         const int j = t % BN, kk = t / BN;
 #pragma unroll
-        for (int r = 0; r < R; ++r) Bs[kk + (256 / BN) * r][j] = rb[r];
+        for (int r = 0; r < R; ++r) Bs[kk + (NT / BN) * r][j] = rb[r];
     }
 }
 
@@ -243,13 +244,18 @@ __device__ __forceinline__ void write_tile(const GemmDesc& g, const TileCoord& t
         }
 }
 
-template <int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ descs, int nd) {
-    constexpr int MI = BM / 32, NI = BN / 32;
-    __shared__ double As[BK][BM + 1];
-    __shared__ double Bs[BK][BN + 1];
+// WM x WN waves, each owning a (BM/WM) x (BN/WN) block of the tile
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __restrict__ descs,
+                                                           int nd) {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
+    // two LDS stages: the next k-tile is stored while the current one is
+    // consumed, one barrier per k-tile
+    __shared__ double As[2][BK][BM + 1];
+    __shared__ double Bs[2][BK][BN + 1];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wm0 = (w & 1) * (BM / 2), wn0 = (w >> 1) * (BN / 2);
+    const int wm0 = (w % WM) * (BM / WM), wn0 = (w / WM) * (BN / WN);
     const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
     if (g.skip && *g.skip) return;
     const TileCoord tc = tile_coord<BM, BN>(g, (int64_t)blockIdx.x - g.tile_start);
@@ -261,27 +267,29 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
             acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
             tot[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
-    double ra[BM / 16], rb[BN / 16];
+    double ra[16 * BM / NT], rb[16 * BN / NT];
     if (tc.kbeg < tc.kend) {
-        load_a<BM>(g, tc.m0, tc.kbeg, t, ra);
-        load_b<BN>(g, tc.n0, tc.kbeg, t, rb);
+        load_a<BM, NT>(g, tc.m0, tc.kbeg, t, ra);
+        load_b<BN, NT>(g, tc.n0, tc.kbeg, t, rb);
+        store_a<BM, NT>(g, t, ra, As[0]);
+        store_b<BN, NT>(g, t, rb, Bs[0]);
     }
+    __syncthreads();
+    int buf = 0;
     for (int k0 = tc.kbeg; k0 < tc.kend; k0 += BK) {
-        store_a<BM>(g, t, ra, As);
-        store_b<BN>(g, t, rb, Bs);
-        __syncthreads();
-        if (k0 + BK < tc.kend) {
-            load_a<BM>(g, tc.m0, k0 + BK, t, ra);
-            load_b<BN>(g, tc.n0, k0 + BK, t, rb);
+        const bool more = k0 + BK < tc.kend;
+        if (more) {
+            load_a<BM, NT>(g, tc.m0, k0 + BK, t, ra);
+            load_b<BN, NT>(g, tc.n0, k0 + BK, t, rb);
         }
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int k = ks * 4 + (lane >> 4);
             double a[MI], b[NI];
 #pragma unroll
-            for (int mi = 0; mi < MI; ++mi) a[mi] = As[k][wm0 + mi * 16 + (lane & 15)];
+            for (int mi = 0; mi < MI; ++mi) a[mi] = As[buf][k][wm0 + mi * 16 + (lane & 15)];
 #pragma unroll
-            for (int ni = 0; ni < NI; ++ni) b[ni] = Bs[k][wn0 + ni * 16 + (lane & 15)];
+            for (int ni = 0; ni < NI; ++ni) b[ni] = Bs[buf][k][wn0 + ni * 16 + (lane & 15)];
 #pragma unroll
             for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -290,7 +298,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
                                                                        0, 0, 0);
         }
         // end of a K chunk: add its sum to the running total (chunk order)
-        if ((k0 + BK) % g.kchunk == 0 || k0 + BK >= tc.kend) {
+        if ((k0 + BK) % g.kchunk == 0 || !more) {
 #pragma unroll
             for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -300,7 +308,12 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
                     acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
                 }
         }
+        if (more) {
+            store_a<BM, NT>(g, t, ra, As[buf ^ 1]);
+            store_b<BN, NT>(g, t, rb, Bs[buf ^ 1]);
+        }
         __syncthreads();
+        buf ^= 1;
     }
     write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, tot);
 }
@@ -327,8 +340,8 @@ __global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict
         }
     double ra[4], rb[4];
     for (int k0 = tc.kbeg; k0 < tc.kend; k0 += BK) {
-        load_a<64>(g, tc.m0, k0, t, ra);
-        load_b<64>(g, tc.n0, k0, t, rb);
+        load_a<64, 256>(g, tc.m0, k0, t, ra);
+        load_b<64, 256>(g, tc.n0, k0, t, rb);
         // split and stage
         {
             float ah[4], al[4], bh[4], bl[4];
@@ -520,7 +533,22 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
         W[i + j * ld] = (i == j) ? 1.0 : 0.0;
     }
     const int half = np / 2, m = np - 1;
-    const int ntri = half * (half + 1) / 2;      // pair blocks (i <= j)
+    const int ntri = half * (half + 1) / 2;      // pair blocks (i <= j), <= 2 * JT
+    // this thread's fixed tasks: pair blocks e = t, t + JT of the upper
+    // triangle, and (pair, row) entries e = t + u JT of W
+    int gi[2] = {-1, -1}, gj[2] = {-1, -1};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int e = t + u * JT;
+        if (e < ntri) {
+            int j = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+            while (j * (j + 1) / 2 > e) --j;
+            while ((j + 1) * (j + 2) / 2 <= e) ++j;
+            gi[u] = e - j * (j + 1) / 2;
+            gj[u] = j;
+        }
+    }
+    const int wj0 = t / np, wr0 = t - (t / np) * np, wdj = JT / np, wdr = JT - (JT / np) * np;
     int sweeps = 0;
     for (int sw = 0; sw < max_sweeps; ++sw) {
         // convergence scan over all pairs p < q
@@ -558,11 +586,10 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
             if (!__syncthreads_or(on)) continue;
             // G <- J^T G J on the upper pair blocks; the lower ones are their
             // transposes, so G stays exactly symmetric
-            for (int e = t; e < ntri; e += JT) {
-                int j = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
-                while (j * (j + 1) / 2 > e) --j;
-                while ((j + 1) * (j + 2) / 2 <= e) ++j;
-                const int i = e - j * (j + 1) / 2;         // i <= j
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int i = gi[u], j = gj[u];
+                if (i < 0) continue;
                 const int4 Pi = pq[i], Pj = pq[j];
                 if (!Pi.z && !Pj.z) continue;
                 const int pi = Pi.x, qi = Pi.y;
@@ -588,15 +615,18 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                 G[qi + pj * ld] = y10; G[pj + qi * ld] = y10;
                 G[qi + qj * ld] = y11; G[qj + qi * ld] = y11;
             }
-            const float inv_np_w = inv_np;
-            for (int e = t; e < half * np; e += JT) {
-                const int j = (int)(((float)e + 0.5f) * inv_np_w), r = e - j * np;
+            // W <- W J: thread tasks (pair j, row r), j += JT / np per step
+            for (int j = wj0, r = wr0; j < half;) {
                 const int4 Pj = pq[j];
-                if (!Pj.z) continue;
-                const double2 rj = rcs[j];
-                const double wp = W[r + Pj.x * ld], wq = W[r + Pj.y * ld];
-                W[r + Pj.x * ld] = rj.x * wp - rj.y * wq;
-                W[r + Pj.y * ld] = rj.y * wp + rj.x * wq;
+                if (Pj.z) {
+                    const double2 rj = rcs[j];
+                    const double wp = W[r + Pj.x * ld], wq = W[r + Pj.y * ld];
+                    W[r + Pj.x * ld] = rj.x * wp - rj.y * wq;
+                    W[r + Pj.y * ld] = rj.y * wp + rj.x * wq;
+                }
+                r += wdr;
+                j += wdj;
+                if (r >= np) { r -= np; ++j; }
             }
             __syncthreads();
         }
@@ -881,11 +911,14 @@ void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles,
     if (split32)
         hipLaunchKernelGGL(k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
     else if (shape == GEMM_128x128)
-        hipLaunchKernelGGL((k_gemm_f64<128, 128>), dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
+        hipLaunchKernelGGL((k_gemm_f64<128, 128, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, st,
+                           d_descs, nd);
     else if (shape == GEMM_128x64)
-        hipLaunchKernelGGL((k_gemm_f64<128, 64>), dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
+        hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, st,
+                           d_descs, nd);
     else
-        hipLaunchKernelGGL((k_gemm_f64<64, 64>), dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
+        hipLaunchKernelGGL((k_gemm_f64<64, 64, 2, 2>), dim3((unsigned)tiles), dim3(256), 0, st,
+                           d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 

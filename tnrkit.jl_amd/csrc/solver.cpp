@@ -289,7 +289,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
         Buf sigV(&ctx.pool, (size_t)P * LMAX * 8);
         ConvState cs(ctx, P);
         std::vector<FillDesc> fills;
-        std::vector<RR> r0, rY, rZ, rV, rF;
+        std::vector<RR> rY, rZ, rV, rF;
         std::vector<GemmJob> jMV, jMtU, jF, jV;
         std::vector<ConvDesc> conv;
         for (int i = 0; i < P; ++i) {
@@ -312,8 +312,6 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             const int* skip = cs.done(i);
             double* sv = sigV.d() + (size_t)i * LMAX;
             fills.push_back(FillDesc{x.V.d(), s.n, l, s.n, s.seed, 0});
-            r0.push_back(RR{x.V.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
-                            x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
             GemmJob mv{s.M, x.V.d(), x.Y.d(), s.m, l, s.n, s.ld, s.n, s.m, 0, 0};   // Y = M V
             mv.skip = skip;
             jMV.push_back(mv);
@@ -334,7 +332,8 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             jV.push_back(GemmJob{x.V.d(), x.Wt.d(), x.Z.d(), s.n, l, l, s.n, l, s.n, 0, 0});
         }
         randfill(ctx, fills);
-        rr_grouped(ctx, r0, 1);
+        // the random start block is not orthonormalised: the first
+        // Rayleigh-Ritz step on Y = M V builds an orthonormal basis of its span
         GemmPlan pMV = gemm_plan(ctx, jMV), pMtU = gemm_plan(ctx, jMtU);
         RRPlan pY = rr_plan(ctx, rY, 1), pZ = rr_plan(ctx, rZ, 1);
         DescArena a;
@@ -431,7 +430,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         Buf sigV(&ctx.pool, (size_t)P * LMAX * 8);
         ConvState cs(ctx, P);
         std::vector<FillDesc> fills;
-        std::vector<RR> r0, rY, rV;
+        std::vector<RR> rY, rV;
         std::vector<ColDesc> sym;
         std::vector<GemmJob> jMV, jF, jH;
         std::vector<ConvDesc> conv;
@@ -452,8 +451,6 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             const int* skip = cs.done(i);
             double* sv = sigV.d() + (size_t)i * LMAX;
             fills.push_back(FillDesc{x.V.d(), s.n, l, s.n, s.seed, 0});
-            r0.push_back(RR{x.V.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
-                            x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
             sym.push_back(ColDesc{s.M, nullptr, nullptr, s.n, s.n, s.ld, 0});
             GemmJob mv{s.M, x.V.d(), x.Y.d(), s.n, l, s.n, s.ld, s.n, s.n, 0, 0};   // Y = M V
             mv.skip = skip;
@@ -470,7 +467,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         ColDesc* dsym = ctx.stage->put(ctx.stream, sym);
         launch_symmetrize(ctx.stream, dsym, (int)sym.size(), nmax);
         randfill(ctx, fills);
-        rr_grouped(ctx, r0, 1);
+        // (random start block used as is, see svd_grouped)
         GemmPlan pMV = gemm_plan(ctx, jMV);
         RRPlan pY = rr_plan(ctx, rY, 1);
         DescArena a;
