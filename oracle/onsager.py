@@ -3,7 +3,10 @@
 -beta f = ln(2)/2 + (1/2pi) int_0^pi ln[cosh^2(2b) + (1/k) sqrt(1 + k^2 - 2k cos 2t)] dt,
 k = 1/sinh^2(2b).  Pinned against the reference constant f_onsager at
 beta_c (src/models/ising.jl:2), reproduced to ~1e-16 in
-tests/test_oracle_schemes.py.
+tests/test_oracle_pins.py, and against a 30-digit mpmath evaluation of the
+same closed form on a beta grid (tests/golden/onsager_table.json).
+Test infrastructure: only tests, the smoke check and bench.py's CPU baseline
+use this package.
 """
 from __future__ import annotations
 

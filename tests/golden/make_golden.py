@@ -1,0 +1,97 @@
+"""Writes the golden fixtures of this directory (run from the repository root:
+python tests/golden/make_golden.py).
+
+* onsager_table.json - -beta f of the 2D Ising model from Onsager's closed
+  form, evaluated with mpmath at 30 digits on a beta grid (independent of the
+  scipy quadrature in oracle/onsager.py).
+* reference_values.json - the numbers the reference itself publishes for this
+  path (README.md and test/*.jl), with their file:line.
+* oracle_norms.json - per-step norm vectors of the FP64 oracle on fixed
+  models and configurations, so the GPU parity tests compare against stored
+  vectors and the CPU suite notices any drift of the oracle.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import mpmath
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import models as om        # noqa: E402
+from oracle import schemes as osch     # noqa: E402
+
+mpmath.mp.dps = 30
+
+
+def onsager_mp(beta):
+    b = mpmath.mpf(beta)
+    k = 1 / mpmath.sinh(2 * b) ** 2
+    c2 = mpmath.cosh(2 * b) ** 2
+    f = lambda t: mpmath.log(c2 + mpmath.sqrt(1 + k * k - 2 * k * mpmath.cos(2 * t)) / k)
+    val = mpmath.quad(f, [0, mpmath.pi / 4, mpmath.pi / 2, 3 * mpmath.pi / 4, mpmath.pi])
+    return -(mpmath.log(2) / 2 + val / (2 * mpmath.pi)) / b
+
+
+# (name, scheme, model, chi, maxiter)
+NORM_CASES = [
+    ("btrg_z2_ising_betac_chi16_25", "BTRG", ("ising_z2", None), 16, 25),
+    ("trg_z2_ising_betac_chi16_12", "TRG", ("ising_z2", None), 16, 12),
+    ("hotrg_z2_ising_betac_chi8_10", "HOTRG", ("ising_z2", None), 8, 10),
+    ("btrg_trivial_ising_b0.40_chi24_10", "BTRG", ("ising_trivial", 0.40), 24, 10),
+    ("btrg_z3_potts_betac_chi12_10", "BTRG", ("potts_z3", None), 12, 10),
+    ("trg_z6_clock_b1.1_chi12_8", "TRG", ("clock_z6", 1.1), 12, 8),
+]
+
+
+def model(spec):
+    name, beta = spec
+    if name == "ising_z2":
+        return om.classical_ising_z2(om.ising_betac_mp if beta is None else beta)
+    if name == "ising_trivial":
+        return om.classical_ising_trivial(beta)
+    if name == "potts_z3":
+        return om.classical_potts_zn(3, om.potts_betac(3) if beta is None else beta)
+    if name == "clock_z6":
+        return om.classical_clock_zn(6, beta)
+    raise ValueError(name)
+
+
+def main():
+    grid = [round(0.30 + 0.02 * i, 2) for i in range(16)] + [float(om.ising_betac)]
+    table = {"note": "-beta f from Onsager's closed form, mpmath 30 digits",
+             "beta": grid, "f": [float(onsager_mp(b)) for b in grid]}
+    with open(os.path.join(HERE, "onsager_table.json"), "w") as f:
+        json.dump(table, f, indent=1)
+
+    ref = {
+        "readme_btrg_z2_chi16_maxiter25_f": {
+            "value": "-2.1096504926141826902647832", "source": "README.md:87"},
+        "readme_btrg_rel_err_vs_onsager": {"value": 3.1e-7, "source": "README.md:91-94"},
+        "f_onsager_betac": {"value": "-2.1096511446082074596677792835110847808254932754354",
+                            "source": "src/models/ising.jl:2"},
+        "test_gates_vs_onsager": {
+            "TRG_chi24_maxiter25": {"rtol": 2e-6, "source": "test/schemes.jl:20-23"},
+            "BTRG_chi24_maxiter25": {"rtol": 6e-8, "source": "test/schemes.jl:59-62"},
+            "HOTRG_chi16_maxiter25": {"rtol": 6e-7, "scalefactor": 4,
+                                      "source": "test/schemes.jl:96-99"}},
+    }
+    with open(os.path.join(HERE, "reference_values.json"), "w") as f:
+        json.dump(ref, f, indent=1)
+
+    out = {}
+    for name, kind, spec, chi, n in NORM_CASES:
+        sc = {"TRG": osch.TRG, "BTRG": osch.BTRG, "HOTRG": osch.HOTRG}[kind](model(spec))
+        d = osch.run(sc, chi, n)
+        out[name] = {"scheme": kind, "model": spec[0], "beta": spec[1], "chi": chi,
+                     "maxiter": n, "norms": [float(x) for x in d]}
+        print(name, len(d), flush=True)
+    with open(os.path.join(HERE, "oracle_norms.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -22,6 +22,36 @@ def gpu_run(tk, cls, T, chi, n, **kw):
     return out
 
 
+def _golden_norms():
+    import json
+    import os
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_norms.json")
+    with open(p) as f:
+        return json.load(f)
+
+
+def _tk_model(tk, name, beta):
+    if name == "ising_z2":
+        return tk.classical_ising(beta, symmetry="Z2")
+    if name == "ising_trivial":
+        return tk.classical_ising(beta, symmetry="Trivial")
+    if name == "potts_z3":
+        return tk.classical_potts(3, beta)
+    if name == "clock_z6":
+        return tk.classical_clock(6, beta)
+    raise ValueError(name)
+
+
+@pytest.mark.parametrize("name", sorted(_golden_norms()))
+def test_against_golden_oracle_norms(tk, name):
+    """Stored oracle norm vectors (tests/golden/oracle_norms.json): every
+    scheme, Z2 / Z3 / Z6 / Trivial sectors, relative 1e-10."""
+    case = _golden_norms()[name]
+    d = gpu_run(tk, getattr(tk, case["scheme"]), _tk_model(tk, case["model"], case["beta"]),
+                case["chi"], case["maxiter"])
+    assert rel(d, np.array(case["norms"])) < TOL
+
+
 def test_readme_btrg_z2_chi16(tk):
     """README.md:78-87: f = -2.1096504926141826902647832 (26 norms)."""
     d = gpu_run(tk, tk.BTRG, tk.classical_ising(symmetry="Z2"), 16, 25)
