@@ -651,6 +651,144 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
     if (t == 0 && D.sweeps) *D.sweeps = sweeps;
 }
 
+// ------------------------------------------------ pivoted Cholesky QR
+// One 256-thread workgroup per n x n Gram matrix G = Y^T Y (n <= LMAX):
+// diagonal-pivoted Cholesky G[p, p] = R^T R, stopped where the largest
+// remaining diagonal falls to null_rel2 * max diag(G) (numerical rank r),
+// then X = R^{-1} (r x r).  Output W = P X (W[perm[i], j] = X[i, j], zero
+// columns from r on), so that Y W = Q has orthonormal columns spanning
+// span(Y), ordered by decreasing pivot; lam[j] = R[j][j] (the norm of
+// column j of Y after projecting out the earlier pivots).  Ties in the
+// pivot search go to the lowest index, so the result is a function of G.
+constexpr int CQT = 256;
+
+__device__ __forceinline__ void argmax_wave(double& v, int& i) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o, 64);
+        const int i2 = __shfl_xor(i, o, 64);
+        if (v2 > v || (v2 == v && i2 < i)) {
+            v = v2;
+            i = i2;
+        }
+    }
+}
+
+__global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ descs, int nd,
+                                                double null_rel2) {
+    extern __shared__ double lds[];
+    __shared__ double bv[CQT / 64];
+    __shared__ int bi[CQT / 64];
+    __shared__ int piv_s;
+    __shared__ double dmax_s;
+    const JacobiDesc D = descs[blockIdx.x];
+    if (D.skip && *D.skip) return;
+    const int n = D.n, ld = n + 1, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    double* A = lds;                   // symmetric working copy; R in its upper triangle
+    double* X = A + n * ld;            // R^{-1}
+    int* perm = (int*)(X + n * ld);
+    for (int e = t; e < n * n; e += CQT) {
+        const int j = e / n, i = e - j * n;
+        A[i + j * ld] = 0.5 * (D.G[i + (int64_t)j * D.ldg] + D.G[j + (int64_t)i * D.ldg]);
+        X[i + j * ld] = 0.0;
+    }
+    for (int i = t; i < n; i += CQT) perm[i] = i;
+    __syncthreads();
+    {
+        double v = (t < n) ? A[t + t * ld] : 0.0;
+        int ii = t;
+        argmax_wave(v, ii);
+        if (lane == 0) bv[w] = v;
+        __syncthreads();
+        if (t == 0) {
+            double m = bv[0];
+            for (int q = 1; q < CQT / 64; ++q) m = fmax(m, bv[q]);
+            dmax_s = m;
+        }
+        __syncthreads();
+    }
+    const double nfloor = null_rel2 * dmax_s;
+    int rank = n;
+    for (int j = 0; j < n; ++j) {
+        double v = -1.0;
+        int ii = n;
+        if (t < n - j) {
+            ii = j + t;
+            v = A[ii + ii * ld];
+        }
+        argmax_wave(v, ii);
+        if (lane == 0) {
+            bv[w] = v;
+            bi[w] = ii;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double b = bv[0];
+            int bj = bi[0];
+            for (int q = 1; q < CQT / 64; ++q)
+                if (bv[q] > b || (bv[q] == b && bi[q] < bj)) {
+                    b = bv[q];
+                    bj = bi[q];
+                }
+            piv_s = (b > nfloor && dmax_s > 0.0) ? bj : -1;
+        }
+        __syncthreads();
+        const int p = piv_s;
+        if (p < 0) {
+            rank = j;
+            break;
+        }
+        if (p != j) {   // symmetric exchange of indices j and p
+            for (int k = t; k < n; k += CQT) {
+                if (k == j || k == p) continue;
+                const double a = A[j + k * ld], b = A[p + k * ld];
+                A[j + k * ld] = b;
+                A[p + k * ld] = a;
+                const double c = A[k + j * ld], d = A[k + p * ld];
+                A[k + j * ld] = d;
+                A[k + p * ld] = c;
+            }
+            if (t == 0) {
+                const double a = A[j + j * ld];
+                A[j + j * ld] = A[p + p * ld];
+                A[p + p * ld] = a;
+                const int q = perm[j];
+                perm[j] = perm[p];
+                perm[p] = q;
+            }
+            __syncthreads();
+        }
+        const double r = sqrt(A[j + j * ld]);
+        __syncthreads();
+        for (int k = j + 1 + t; k < n; k += CQT) A[j + k * ld] /= r;
+        if (t == 0) A[j + j * ld] = r;
+        __syncthreads();
+        const int m = n - j - 1;
+        for (int e = t; e < m * m; e += CQT) {
+            const int kk = e / m, i = j + 1 + (e - kk * m), k = j + 1 + kk;
+            A[i + k * ld] -= A[j + i * ld] * A[j + k * ld];
+        }
+        __syncthreads();
+    }
+    // X = R^{-1}, one column per thread (back substitution)
+    for (int c = t; c < rank; c += CQT) {
+        X[c + c * ld] = 1.0 / A[c + c * ld];
+        for (int i = c - 1; i >= 0; --i) {
+            double s = 0.0;
+            for (int k = i + 1; k <= c; ++k) s += A[i + k * ld] * X[k + c * ld];
+            X[i + c * ld] = -s / A[i + i * ld];
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < n * n; e += CQT) {
+        const int j = e / n, i = e - j * n;
+        const double x = (j < rank && i <= j) ? X[i + j * ld] : 0.0;
+        D.W[perm[i] + (int64_t)j * D.ldw] = x;
+    }
+    for (int j = t; j < n; j += CQT) D.lam[j] = (j < rank) ? A[j + j * ld] : 0.0;
+    if (t == 0 && D.sweeps) *D.sweeps = 0;
+}
+
 // ------------------------------------------------- column norms / normalise
 // sigma_j = ||Y[:, j]||; U[:, j] = Y[:, j] / sigma_j, or 0 when
 // sigma_j <= rtol * max_j sigma_j (numerical null direction).  One
@@ -984,6 +1122,23 @@ void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, 
         ps.flops = fl;
         ps.bytes = by;
     }
+}
+
+size_t cholqr_lds_bytes(int n) {
+    return 2 * (size_t)n * (n + 1) * sizeof(double) + (size_t)n * sizeof(int);
+}
+
+void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax,
+                   double null_rel2) {
+    if (nd <= 0) return;
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void*)k_cholqr, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (int)cholqr_lds_bytes(LMAX));
+    TNR_HIP(attr);
+    ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
+    hipLaunchKernelGGL(k_cholqr, dim3(nd), dim3(CQT), cholqr_lds_bytes(nmax), st, d_descs, nd,
+                       null_rel2);
+    TNR_HIP(hipGetLastError());
 }
 
 void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, double rtol) {

@@ -36,6 +36,8 @@ static SolverCfg make_cfg() {
             else if (k == "block_extra") c.block_extra = (int)v;
             else if (k == "conv_target") c.conv_target = v;
             else if (k == "max_iters") c.max_iters = (int)v;
+            else if (k == "cholqr_y") c.cholqr_y = v != 0.0;
+            else if (k == "cholqr_z") c.cholqr_z = v != 0.0;
         }
         i = j + 1;
     }
@@ -72,6 +74,7 @@ struct RR {
 // what one pass leaves), 2 for the final Ritz extraction and direct solves.
 struct RRPlan {
     int passes = 1, lmax = 0;
+    bool chol = false;          // one pass, pivoted Cholesky QR instead of Jacobi
     GemmPlan gram[2], rot[2];
     std::vector<JacobiDesc> jac[2];
     std::vector<ColDesc> col;
@@ -80,10 +83,12 @@ struct RRPlan {
     const ColDesc* d_col = nullptr;
 };
 
-RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes) {
+RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool chol = false) {
     TNR_REQUIRE(passes == 1 || passes == 2, TNR_ERR_INTERNAL, "one or two passes");
+    TNR_REQUIRE(!chol || passes == 1, TNR_ERR_INTERNAL, "Cholesky QR is a one-pass step");
     RRPlan P;
     P.passes = passes;
+    P.chol = chol;
     for (auto& j : jobs) {
         TNR_REQUIRE(passes == 2 || !j.Wtot, TNR_ERR_INTERNAL, "total rotation needs two passes");
         P.lmax = std::max(P.lmax, j.l);
@@ -142,9 +147,13 @@ void rr_issue(Ctx& ctx, RRPlan& P) {
     for (int pass = 0; pass < P.passes; ++pass) {
         gemm_issue(ctx, P.gram[pass]);
         const bool in_loop = P.passes == 1;
-        launch_jacobi(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmax,
-                      in_loop ? cfg.jacobi_tol_iter : cfg.jacobi_tol,
-                      in_loop ? cfg.jacobi_sweeps_iter : cfg.jacobi_sweeps);
+        if (P.chol)
+            launch_cholqr(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmax,
+                          cfg.null_rtol * cfg.null_rtol);
+        else
+            launch_jacobi(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmax,
+                          in_loop ? cfg.jacobi_tol_iter : cfg.jacobi_tol,
+                          in_loop ? cfg.jacobi_sweeps_iter : cfg.jacobi_sweeps);
         gemm_issue(ctx, P.rot[pass]);
     }
     launch_colnorm(ctx.stream, P.d_col, (int)P.col.size(), cfg.null_rtol);
@@ -322,7 +331,9 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             jMtU.push_back(mtu);
             rZ.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip});
-            conv.push_back(ConvDesc{sv, l, s.kmax, 1, cs.it(i), cs.prod(i), cs.done(i)});
+            // Ritz values of the Z side, or with Cholesky QR the pivots R_jj
+            conv.push_back(ConvDesc{cfg.cholqr_z ? x.lam.d() : sv, l, s.kmax, 1, cs.it(i),
+                                    cs.prod(i), cs.done(i)});
             rV.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
             // Ritz extraction from Y = M V (two passes), V <- V Wtot
@@ -335,7 +346,10 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
         // the random start block is not orthonormalised: the first
         // Rayleigh-Ritz step on Y = M V builds an orthonormal basis of its span
         GemmPlan pMV = gemm_plan(ctx, jMV), pMtU = gemm_plan(ctx, jMtU);
-        RRPlan pY = rr_plan(ctx, rY, 1), pZ = rr_plan(ctx, rZ, 1);
+        // Y side: an orthonormal basis of span(M V) is all the next product
+        // needs (pivoted Cholesky QR); Z side: Rayleigh-Ritz, whose values
+        // drive the stopping rule
+        RRPlan pY = rr_plan(ctx, rY, 1, cfg.cholqr_y), pZ = rr_plan(ctx, rZ, 1, cfg.cholqr_z);
         DescArena a;
         gemm_stage(pMV, a);
         gemm_stage(pMtU, a);

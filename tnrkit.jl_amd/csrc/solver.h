@@ -65,6 +65,8 @@ struct SolverCfg {
     double null_rtol = 1e-14;    // columns below this x sigma_max are null
     double conv_target = 1e-14;  // accumulated contraction of the top-k subspace angle
     double block_mult = 2.0;     // block width max(block_mult k, k + block_extra) <= LMAX
+    bool cholqr_y = true;        // block iteration: Y side by pivoted Cholesky QR
+    bool cholqr_z = false;       // ... and Z side (stopping rule on the pivots R_jj)
     int block_extra = 2;
     int min_iters = 2;
     int max_iters = 80;
