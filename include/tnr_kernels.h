@@ -19,7 +19,8 @@ extern "C" {
 
 /* C = alpha op(A) op(B) + beta C, column-major FP64.  mode & 7: 0 = FP64
  * MFMA with the tile shape the engine would choose, 1 = FP32-MFMA hi/lo
- * split kernel, 2 / 3 / 4 = FP64 MFMA with 64x64 / 128x64 / 128x128 tiles;
+ * split kernel, 2 / 3 / 4 / 5 = FP64 MFMA with 64x64 / 128x64 / 128x128 /
+ * 64x64 (32-deep k-tiles) tiles;
  * mode | 8 never splits K over workgroups, mode | 16 always does (for K of
  * two or more 256-deep chunks); the default decides by launch size.
  * repeat >= 1 launches the product that many times on device-resident

@@ -45,13 +45,14 @@ def test_gemm_f64(tk, M, N, K, ta, tb):
     (1024, 64, 1024, 0, 0), (64, 64, 4096, 1, 0), (1024, 1024, 32, 0, 0)])
 def test_gemm_tile_shapes_agree_bitwise(tk, M, N, K, ta, tb):
     """Each output element is summed in the same k order under every tile
-    shape (64x64, 128x64, 128x128), so the three kernels agree bit for bit;
+    shape (64x64, 128x64, 128x128, 64x64 with 32-deep k-tiles), so the
+    kernels agree bit for bit;
     each also matches numpy to the FP64 GEMM bound."""
     rng = np.random.default_rng(M + 11 * N + 101 * K)
     A = rng.standard_normal((K, M) if ta else (M, K))
     B = rng.standard_normal((N, K) if tb else (K, N))
     Cm = rng.standard_normal((M, N))
-    outs = [gemm(tk, A, B, Cm, ta, tb, 0.9, -0.4, split=mode) for mode in (2, 3, 4, 0)]
+    outs = [gemm(tk, A, B, Cm, ta, tb, 0.9, -0.4, split=mode) for mode in (2, 3, 4, 5, 0)]
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
     ref = 0.9 * ((A.T if ta else A) @ (B.T if tb else B)) - 0.4 * Cm
@@ -70,7 +71,7 @@ def test_gemm_split_k_does_not_change_bits(tk, M, N, K, ta, tb):
     B = rng.standard_normal((N, K) if tb else (K, N))
     Cm = rng.standard_normal((M, N))
     outs = [gemm(tk, A, B, Cm, ta, tb, 1.1, 0.3, split=shape | flag)
-            for shape in (2, 3, 4) for flag in (8, 16)]
+            for shape in (2, 3, 4, 5) for flag in (8, 16)]
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
     ref = 1.1 * ((A.T if ta else A) @ (B.T if tb else B)) + 0.3 * Cm

@@ -330,7 +330,7 @@ void DescArena::commit(Ctx& ctx) {
 // K chunking: a function of K alone (every chunk is summed from zero and
 // the chunk sums are added in order), so whether a launch splits K over
 // workgroups changes no bits and can depend on the whole launch.
-static int kchunk_of(int K) { return K >= 512 ? 256 : std::max(GEMM_BK, (K + 15) / 16 * 16); }
+static int kchunk_of(int K) { return K >= 512 ? 256 : std::max(32, (K + 31) / 32 * 32); }
 static int nchunks_of(int K) { return K >= 512 ? (K + 255) / 256 : 1; }
 
 // Tile shape of one grouped launch: the largest tile that still gives the
@@ -373,9 +373,9 @@ struct TuneTrial {
     int shape = 0;
 };
 struct TuneEntry {
-    float ms[3] = {0, 0, 0};
-    int samples[3] = {0, 0, 0};
-    int requested[3] = {0, 0, 0};
+    float ms[GEMM_NSHAPES] = {0, 0, 0, 0};
+    int samples[GEMM_NSHAPES] = {0, 0, 0, 0};
+    int requested[GEMM_NSHAPES] = {0, 0, 0, 0};
     std::vector<TuneTrial> pending;
     int best = -1;
 };
@@ -410,10 +410,10 @@ void tune_poll(TuneEntry& t) {
         t.pending.erase(t.pending.begin() + (long)i);
     }
     bool all = true;
-    for (int s = 0; s < 3; ++s) all = all && t.samples[s] >= TUNE_SAMPLES;
+    for (int s = 0; s < GEMM_NSHAPES; ++s) all = all && t.samples[s] >= TUNE_SAMPLES;
     if (all) {
         int best = 0;
-        for (int s = 1; s < 3; ++s)
+        for (int s = 1; s < GEMM_NSHAPES; ++s)
             if (t.ms[s] < t.ms[best]) best = s;
         t.best = best;
     }
@@ -454,7 +454,7 @@ static int tuned_shape(const Ctx& ctx, const std::vector<GemmJob>& jobs, int heu
     if (t.best >= 0) return t.best;
     tune_poll(t);
     if (t.best >= 0) return t.best;
-    for (int s = 2; s >= 0; --s)
+    for (int s = GEMM_NSHAPES - 1; s >= 0; --s)
         if (t.requested[s] < TUNE_SAMPLES) {
             ++t.requested[s];
             *key_out = key;
@@ -491,7 +491,7 @@ GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int
     if (force_shape >= 0 && !split32) P.shape = force_shape;
     else if (split32) P.shape = GEMM_64x64;
     else P.shape = tuned_shape(ctx, jobs, pick_shape(jobs, false), &P.tune_key, &P.trial);
-    const int BMs = P.shape == GEMM_64x64 ? 64 : 128;
+    const int BMs = (P.shape == GEMM_64x64 || P.shape == GEMM_64x64_K32) ? 64 : 128;
     const int BNs = P.shape == GEMM_128x128 ? 128 : 64;
     // split K over workgroups only when the launch is too small to fill
     // the chip without it (results are the same either way)

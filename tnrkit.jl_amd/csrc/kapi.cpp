@@ -36,7 +36,7 @@ int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta, const
                     int repeat, double* ms_out) {
     return kguard([&] {
         const int kind = mode & 7, split = (mode & 8) ? 0 : (mode & 16) ? 1 : -1;
-        TNR_REQUIRE(M > 0 && N > 0 && K >= 0 && A && B && C && repeat >= 1 && kind <= 4 &&
+        TNR_REQUIRE(M > 0 && N > 0 && K >= 0 && A && B && C && repeat >= 1 && kind <= 5 &&
                         (mode & ~31) == 0 && (mode & 24) != 24,
                     TNR_ERR_ARG, "bad GEMM arguments");
         TNR_REQUIRE(lda >= (ta ? std::max(K, 1) : M) && ldb >= (tb ? N : std::max(K, 1)) && ldc >= M,

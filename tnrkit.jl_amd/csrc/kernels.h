@@ -131,7 +131,8 @@ void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t total)
 void launch_transpose(hipStream_t st, const TransDesc* d_descs, int nd, int64_t tiles,
                       int64_t elems);
 // tile shapes of the FP64 GEMM kernel (the split-FP32 kernel is 64 x 64)
-enum GemmShape { GEMM_64x64 = 0, GEMM_128x64 = 1, GEMM_128x128 = 2 };
+enum GemmShape { GEMM_64x64 = 0, GEMM_128x64 = 1, GEMM_128x128 = 2, GEMM_64x64_K32 = 3 };
+constexpr int GEMM_NSHAPES = 4;
 void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32,
                  double flops, int shape);
 void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const int64_t* d_starts,

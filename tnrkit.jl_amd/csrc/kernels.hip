@@ -145,12 +145,12 @@ __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local
     return c;
 }
 
-// global -> registers: the tile's BM x BK slab of op(A), 16 BM / NT values
+// global -> registers: the tile's BM x KB slab of op(A), KB BM / NT values
 // per thread of an NT-thread workgroup
-template <int BM, int NT>
+template <int BM, int NT, int KB>
 __device__ __forceinline__ void load_a(const GemmDesc& g, int m0, int k0, int t,
-                                       double (&ra)[16 * BM / NT]) {
-    constexpr int R = 16 * BM / NT;
+                                       double (&ra)[KB * BM / NT]) {
+    constexpr int R = KB * BM / NT;
     if (!g.ta) {  // A[i + k*lda], M x K: thread -> (i, k-group)
         const int i = m0 + (t % BM), kk = t / BM;
 #pragma unroll
@@ -159,39 +159,39 @@ __device__ __forceinline__ void load_a(const GemmDesc& g, int m0, int k0, int t,
             ra[r] = (i < g.M && k < g.K) ? g.A[(int64_t)i + (int64_t)k * g.lda] : 0.0;
         }
     } else {      // stored K x M: A[k + i*lda]
-        const int kk = t & 15, i0 = t >> 4;
+        const int kk = t % KB, i0 = t / KB;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int i = m0 + i0 + (NT / 16) * r, k = k0 + kk;
+            const int i = m0 + i0 + (NT / KB) * r, k = k0 + kk;
             ra[r] = (i < g.M && k < g.K) ? g.A[(int64_t)k + (int64_t)i * g.lda] : 0.0;
         }
     }
 }
 
-template <int BM, int NT>
-__device__ __forceinline__ void store_a(const GemmDesc& g, int t, const double (&ra)[16 * BM / NT],
+template <int BM, int NT, int KB>
+__device__ __forceinline__ void store_a(const GemmDesc& g, int t, const double (&ra)[KB * BM / NT],
                                         double (*As)[BM + 1]) {
-    constexpr int R = 16 * BM / NT;
+    constexpr int R = KB * BM / NT;
     if (!g.ta) {
         const int i = t % BM, kk = t / BM;
 #pragma unroll
         for (int r = 0; r < R; ++r) As[kk + (NT / BM) * r][i] = ra[r];
     } else {
-        const int kk = t & 15, i0 = t >> 4;
+        const int kk = t % KB, i0 = t / KB;
 #pragma unroll
-        for (int r = 0; r < R; ++r) As[kk][i0 + (NT / 16) * r] = ra[r];
+        for (int r = 0; r < R; ++r) As[kk][i0 + (NT / KB) * r] = ra[r];
     }
 }
 
-template <int BN, int NT>
+template <int BN, int NT, int KB>
 __device__ __forceinline__ void load_b(const GemmDesc& g, int n0, int k0, int t,
-                                       double (&rb)[16 * BN / NT]) {
-    constexpr int R = 16 * BN / NT;
+                                       double (&rb)[KB * BN / NT]) {
+    constexpr int R = KB * BN / NT;
     if (!g.tb) {  // B[k + j*ldb], K x N
-        const int kk = t & 15, j0 = t >> 4;
+        const int kk = t % KB, j0 = t / KB;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = n0 + j0 + (NT / 16) * r, k = k0 + kk;
+            const int j = n0 + j0 + (NT / KB) * r, k = k0 + kk;
             rb[r] = (j < g.N && k < g.K) ? g.B[(int64_t)k + (int64_t)j * g.ldb] : 0.0;
         }
     } else {      // stored N x K: B[j + k*ldb]
@@ -204,14 +204,14 @@ __device__ __forceinline__ void load_b(const GemmDesc& g, int n0, int k0, int t,
     }
 }
 
-template <int BN, int NT>
-__device__ __forceinline__ void store_b(const GemmDesc& g, int t, const double (&rb)[16 * BN / NT],
+template <int BN, int NT, int KB>
+__device__ __forceinline__ void store_b(const GemmDesc& g, int t, const double (&rb)[KB * BN / NT],
                                         double (*Bs)[BN + 1]) {
-    constexpr int R = 16 * BN / NT;
+    constexpr int R = KB * BN / NT;
     if (!g.tb) {
-        const int kk = t & 15, j0 = t >> 4;
+        const int kk = t % KB, j0 = t / KB;
 #pragma unroll
-        for (int r = 0; r < R; ++r) Bs[kk][j0 + (NT / 16) * r] = rb[r];
+        for (int r = 0; r < R; ++r) Bs[kk][j0 + (NT / KB) * r] = rb[r];
     } else {
         const int j = t % BN, kk = t / BN;
 #pragma unroll
@@ -244,16 +244,17 @@ __device__ __forceinline__ void write_tile(const GemmDesc& g, const TileCoord& t
         }
 }
 
-// WM x WN waves, each owning a (BM/WM) x (BN/WN) block of the tile
-template <int BM, int BN, int WM, int WN>
+// WM x WN waves, each owning a (BM/WM) x (BN/WN) block of the tile; k-tiles
+// of KB.  Every shape runs the MFMAs over k in the same 4-wide steps.
+template <int BM, int BN, int WM, int WN, int KB>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __restrict__ descs,
                                                            int nd) {
     constexpr int NT = 64 * WM * WN;
     constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
     // two LDS stages: the next k-tile is stored while the current one is
     // consumed, one barrier per k-tile
-    __shared__ double As[2][BK][BM + 1];
-    __shared__ double Bs[2][BK][BN + 1];
+    __shared__ double As[2][KB][BM + 1];
+    __shared__ double Bs[2][KB][BN + 1];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm0 = (w % WM) * (BM / WM), wn0 = (w / WM) * (BN / WN);
     const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
@@ -267,23 +268,23 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
             acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
             tot[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
-    double ra[16 * BM / NT], rb[16 * BN / NT];
+    double ra[KB * BM / NT], rb[KB * BN / NT];
     if (tc.kbeg < tc.kend) {
-        load_a<BM, NT>(g, tc.m0, tc.kbeg, t, ra);
-        load_b<BN, NT>(g, tc.n0, tc.kbeg, t, rb);
-        store_a<BM, NT>(g, t, ra, As[0]);
-        store_b<BN, NT>(g, t, rb, Bs[0]);
+        load_a<BM, NT, KB>(g, tc.m0, tc.kbeg, t, ra);
+        load_b<BN, NT, KB>(g, tc.n0, tc.kbeg, t, rb);
+        store_a<BM, NT, KB>(g, t, ra, As[0]);
+        store_b<BN, NT, KB>(g, t, rb, Bs[0]);
     }
     __syncthreads();
     int buf = 0;
-    for (int k0 = tc.kbeg; k0 < tc.kend; k0 += BK) {
-        const bool more = k0 + BK < tc.kend;
+    for (int k0 = tc.kbeg; k0 < tc.kend; k0 += KB) {
+        const bool more = k0 + KB < tc.kend;
         if (more) {
-            load_a<BM, NT>(g, tc.m0, k0 + BK, t, ra);
-            load_b<BN, NT>(g, tc.n0, k0 + BK, t, rb);
+            load_a<BM, NT, KB>(g, tc.m0, k0 + KB, t, ra);
+            load_b<BN, NT, KB>(g, tc.n0, k0 + KB, t, rb);
         }
 #pragma unroll
-        for (int ks = 0; ks < BK / 4; ++ks) {
+        for (int ks = 0; ks < KB / 4; ++ks) {
             const int k = ks * 4 + (lane >> 4);
             double a[MI], b[NI];
 #pragma unroll
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
                                                                        0, 0, 0);
         }
         // end of a K chunk: add its sum to the running total (chunk order)
-        if ((k0 + BK) % g.kchunk == 0 || !more) {
+        if ((k0 + KB) % g.kchunk == 0 || !more) {
 #pragma unroll
             for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -309,8 +310,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
                 }
         }
         if (more) {
-            store_a<BM, NT>(g, t, ra, As[buf ^ 1]);
-            store_b<BN, NT>(g, t, rb, Bs[buf ^ 1]);
+            store_a<BM, NT, KB>(g, t, ra, As[buf ^ 1]);
+            store_b<BN, NT, KB>(g, t, rb, Bs[buf ^ 1]);
         }
         __syncthreads();
         buf ^= 1;
@@ -340,8 +341,8 @@ __global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict
         }
     double ra[4], rb[4];
     for (int k0 = tc.kbeg; k0 < tc.kend; k0 += BK) {
-        load_a<64, 256>(g, tc.m0, k0, t, ra);
-        load_b<64, 256>(g, tc.n0, k0, t, rb);
+        load_a<64, 256, BK>(g, tc.m0, k0, t, ra);
+        load_b<64, 256, BK>(g, tc.n0, k0, t, rb);
         // split and stage
         {
             float ah[4], al[4], bh[4], bl[4];
@@ -1049,14 +1050,17 @@ void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles,
     if (split32)
         hipLaunchKernelGGL(k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
     else if (shape == GEMM_128x128)
-        hipLaunchKernelGGL((k_gemm_f64<128, 128, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, st,
-                           d_descs, nd);
+        hipLaunchKernelGGL((k_gemm_f64<128, 128, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
+                           st, d_descs, nd);
     else if (shape == GEMM_128x64)
-        hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, st,
-                           d_descs, nd);
+        hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
+                           st, d_descs, nd);
+    else if (shape == GEMM_64x64_K32)
+        hipLaunchKernelGGL((k_gemm_f64<64, 64, 2, 2, 32>), dim3((unsigned)tiles), dim3(256), 0,
+                           st, d_descs, nd);
     else
-        hipLaunchKernelGGL((k_gemm_f64<64, 64, 2, 2>), dim3((unsigned)tiles), dim3(256), 0, st,
-                           d_descs, nd);
+        hipLaunchKernelGGL((k_gemm_f64<64, 64, 2, 2, 16>), dim3((unsigned)tiles), dim3(256), 0,
+                           st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 
