@@ -236,7 +236,7 @@ static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t t
     if (total <= 0 || descs.empty()) return;
     std::vector<CopyDesc> elem;
     std::vector<TransDesc> tr;
-    int64_t estart = 0, tiles = 0, telems = 0;
+    int64_t estart = 0, eelems = 0, tiles = 0, telems = 0;
     for (const CopyDesc& d : descs) {
         int s0 = -1, d0 = -1;
         for (int j = 0; j < d.nd; ++j) {
@@ -248,7 +248,7 @@ static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t t
                             (d.nd > 2 ? d.dim[2] : 1) * (d.nd > 3 ? d.dim[3] : 1) *
                             (d.nd > 4 ? d.dim[4] : 1) * (d.nd > 5 ? d.dim[5] : 1);
         const bool weighted = d.wleg[0] >= 0 || d.wleg[1] >= 0;
-        if (!weighted && s0 >= 0 && d0 >= 0 && s0 != d0 && d.dim[s0] >= 16 && d.dim[d0] >= 16 &&
+        if (!weighted && s0 >= 0 && d0 >= 0 && s0 != d0 && d.dim[s0] >= 8 && d.dim[d0] >= 8 &&
             d.nd - 2 <= 4) {
             TransDesc t;
             std::memset(&t, 0, sizeof(t));
@@ -270,16 +270,22 @@ static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t t
             }
             for (; k < 4; ++k) t.bdim[k] = 1;
             t.nb = (int32_t)nb;
-            t.t0 = (t.n0 + 63) / 64;
-            t.t1 = (t.n1 + 63) / 64;
+            auto pow2 = [](int v) { int p = 1; while (p < v && p < 64) p <<= 1; return p; };
+            t.tx = pow2(t.n0);
+            t.ty = pow2(t.n1);
+            t.sub = 4096 / (t.tx * t.ty);
+            t.t0 = (t.n0 + t.tx - 1) / t.tx;
+            t.t1 = (t.n1 + t.ty - 1) / t.ty;
             t.tile_start = tiles;
-            tiles += (int64_t)t.t0 * t.t1 * nb;
+            tiles += (int64_t)t.t0 * t.t1 * ((nb + t.sub - 1) / t.sub);
             telems += cnt;
             tr.push_back(t);
         } else {
+            TNR_REQUIRE(cnt < ((int64_t)1 << 31), TNR_ERR_INTERNAL, "copy job too large");
             CopyDesc e = d;
-            e.start = estart;
-            estart += cnt;
+            e.start = estart;              // in 4096-element chunks
+            estart += (cnt + 4095) / 4096;
+            eelems += cnt;
             elem.push_back(e);
         }
     }
@@ -290,7 +296,7 @@ static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t t
     }
     if (!elem.empty()) {
         CopyDesc* dd = ctx.stage->put(ctx.stream, elem);
-        launch_copy(ctx.stream, dd, (int)elem.size(), estart);
+        launch_copy(ctx.stream, dd, (int)elem.size(), estart, eelems);
     }
 }
 

@@ -16,7 +16,7 @@ struct CopyDesc {
     int32_t dim[MAXLEG];          // extents (dst order)
     int32_t nd;                   // number of dims
     int32_t wleg[2];              // dims (dst order) carrying weights, -1 = none
-    int64_t start;                // prefix sum of element counts
+    int64_t start;                // prefix sum of 4096-element chunks
 };
 
 // Tiled transpose job: a sub-block whose source-contiguous dim (extent n0,
@@ -30,6 +30,7 @@ struct TransDesc {
     int64_t bs[4], bd[4];         // batch strides (src, dst)
     int32_t bdim[4];
     int32_t n0, n1, nb;           // nb = product of batch extents
+    int32_t tx, ty, sub;          // sub-tile extents (powers of two <= 64), sub-tiles per tile
     int32_t t0, t1;               // tiles along n0, n1
     int64_t tile_start;
 };
@@ -127,7 +128,8 @@ struct GatherDesc {
     int64_t start;
 };
 
-void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t total);  // bytes = 16*total
+// start = prefix of 4096-element chunks; every job holds < 2^31 elements
+void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t chunks, int64_t elems);
 void launch_transpose(hipStream_t st, const TransDesc* d_descs, int nd, int64_t tiles,
                       int64_t elems);
 // tile shapes of the FP64 GEMM kernel (the split-FP32 kernel is 64 x 64)

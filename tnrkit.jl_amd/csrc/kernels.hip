@@ -31,30 +31,54 @@ __device__ __forceinline__ int find_desc_copy(const CopyDesc* d, int n, int64_t 
 
 // ------------------------------------------------------------------ permute
 // dst[sum i_k dstr_k] = src[sum i_k sstr_k] * w0[i_wleg0] * w1[i_wleg1]
+// One chunk of COPY_CHUNK consecutive elements (destination order) of one
+// job per workgroup iteration: the job is looked up once per chunk, and the
+// index decomposition runs in 32-bit arithmetic (a job holds < 2^31
+// elements; checked on the host).
+constexpr int COPY_CHUNK = 4096;
+
 __global__ __launch_bounds__(256) void k_copy(const CopyDesc* __restrict__ descs, int nd,
-                                              int64_t total) {
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const CopyDesc& D = descs[find_desc_copy(descs, nd, e)];
-        int64_t r = e - D.start;
-        int64_t so = 0, dof = 0;
-        int idx[MAXLEG];
-#pragma unroll
-        for (int k = 0; k < MAXLEG; ++k) {
-            if (k < D.nd) {
-                int64_t ik = r % D.dim[k];
-                r /= D.dim[k];
-                idx[k] = (int)ik;
-                so += ik * D.sstr[k];
-                dof += ik * D.dstr[k];
-            } else {
-                idx[k] = 0;
-            }
+                                              int64_t chunks) {
+    const int t = threadIdx.x;
+    for (int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        int lo = 0, hi = nd - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (descs[mid].start <= c) lo = mid; else hi = mid - 1;
         }
-        double v = D.src[so];
-        if (D.wleg[0] >= 0) v *= D.w[0][idx[D.wleg[0]]];
-        if (D.wleg[1] >= 0) v *= D.w[1][idx[D.wleg[1]]];
-        D.dst[dof] = v;
+        const CopyDesc& D = descs[lo];
+        const int ndim = D.nd;
+        uint32_t dim[MAXLEG];
+        uint32_t cnt = 1;
+#pragma unroll
+        for (int q = 0; q < MAXLEG; ++q) {
+            dim[q] = q < ndim ? (uint32_t)D.dim[q] : 1u;
+            cnt *= dim[q];
+        }
+        const uint32_t base = (uint32_t)(c - D.start) * COPY_CHUNK;
+        const int wl0 = D.wleg[0], wl1 = D.wleg[1];
+        for (int u = 0; u < COPY_CHUNK / 256; ++u) {
+            uint32_t r = base + (uint32_t)(t + 256 * u);
+            if (r >= cnt) break;
+            int64_t so = 0, dof = 0;
+            uint32_t i_w0 = 0, i_w1 = 0;
+#pragma unroll
+            for (int q = 0; q < MAXLEG; ++q) {
+                if (q < ndim) {
+                    const uint32_t nq = r / dim[q];
+                    const uint32_t iq = r - nq * dim[q];
+                    r = nq;
+                    so += (int64_t)iq * D.sstr[q];
+                    dof += (int64_t)iq * D.dstr[q];
+                    if (q == wl0) i_w0 = iq;
+                    if (q == wl1) i_w1 = iq;
+                }
+            }
+            double v = D.src[so];
+            if (wl0 >= 0) v *= D.w[0][i_w0];
+            if (wl1 >= 0) v *= D.w[1][i_w1];
+            D.dst[dof] = v;
+        }
     }
 }
 
@@ -64,8 +88,11 @@ __global__ __launch_bounds__(256) void k_copy(const CopyDesc* __restrict__ descs
 // consecutive destination elements per pass (coalesced) through LDS.
 __global__ __launch_bounds__(256) void k_transpose(const TransDesc* __restrict__ descs, int nd,
                                                    int64_t tiles) {
-    __shared__ double tile[64][65];
-    const int t = threadIdx.x, x = t & 63, y = t >> 6;
+    // 4096 doubles of LDS hold `sub` sub-tiles of tx (along n0) by ty (along
+    // n1), one per consecutive batch index; rows padded by one element
+    __shared__ double buf[4096 + 64 * 16];
+    __shared__ int64_t soff[64], doff[64];           // batch offsets per sub-tile (-1: none)
+    const int t = threadIdx.x;
     for (int64_t tid = blockIdx.x; tid < tiles; tid += gridDim.x) {
         int lo = 0, hi = nd - 1;
         while (lo < hi) {
@@ -73,30 +100,47 @@ __global__ __launch_bounds__(256) void k_transpose(const TransDesc* __restrict__
             if (descs[mid].tile_start <= tid) lo = mid; else hi = mid - 1;
         }
         const TransDesc& D = descs[lo];
+        const int tx = D.tx, ty = D.ty, sub = D.sub, pitch = tx + 1;
+        const int lx = __ffs(tx) - 1, ly = __ffs(ty) - 1, la = lx + ly;   // log2 extents
         int64_t r = tid - D.tile_start;
-        const int i0 = (int)(r % D.t0) * 64;
+        const int i0 = (int)(r % D.t0) * tx;
         r /= D.t0;
-        const int i1 = (int)(r % D.t1) * 64;
+        const int i1 = (int)(r % D.t1) * ty;
         r /= D.t1;
-        int64_t so = 0, dof = 0;
-        for (int k = 0; k < 4; ++k) {
-            if (D.bdim[k] <= 1) continue;
-            const int64_t ik = r % D.bdim[k];
-            r /= D.bdim[k];
-            so += ik * D.bs[k];
-            dof += ik * D.bd[k];
+        const int64_t b0 = r * sub;                 // first batch index of this tile
+        __syncthreads();                            // previous tile done with buf / offsets
+        if (t < sub) {
+            const int64_t bi = b0 + t;
+            int64_t so = -1, dof = -1;
+            if (bi < D.nb) {
+                int64_t q = bi;
+                so = 0;
+                dof = 0;
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int64_t ik = q % D.bdim[kk];
+                    q /= D.bdim[kk];
+                    so += ik * D.bs[kk];
+                    dof += ik * D.bd[kk];
+                }
+            }
+            soff[t] = so;
+            doff[t] = dof;
         }
-        const double* src = D.src + so;
-        double* dst = D.dst + dof;
         __syncthreads();
-        for (int j = y; j < 64; j += 4) {          // j along n1, x along n0
-            const int a = i0 + x, b = i1 + j;
-            if (a < D.n0 && b < D.n1) tile[j][x] = src[a + (int64_t)b * D.ss1];
+        for (int e = t; e < 4096; e += 256) {       // x along n0 (source-contiguous)
+            const int s = e >> la, rem = e & ((1 << la) - 1), j = rem >> lx, x = rem & (tx - 1);
+            const int a = i0 + x, c = i1 + j;
+            const int64_t so = soff[s];
+            if (so >= 0 && a < D.n0 && c < D.n1)
+                buf[s * ty * pitch + j * pitch + x] = D.src[so + a + (int64_t)c * D.ss1];
         }
         __syncthreads();
-        for (int j = y; j < 64; j += 4) {          // j along n0, x along n1
-            const int a = i0 + j, b = i1 + x;
-            if (a < D.n0 && b < D.n1) dst[b + (int64_t)a * D.ds0] = tile[x][j];
+        for (int e = t; e < 4096; e += 256) {       // x along n1 (destination-contiguous)
+            const int s = e >> la, rem = e & ((1 << la) - 1), j = rem >> ly, x = rem & (ty - 1);
+            const int a = i0 + j, c = i1 + x;
+            const int64_t dof = doff[s];
+            if (dof >= 0 && a < D.n0 && c < D.n1)
+                D.dst[dof + c + (int64_t)a * D.ds0] = buf[s * ty * pitch + x * pitch + j];
         }
     }
 }
@@ -1027,10 +1071,12 @@ static int grid_for(int64_t total, int cap = 4096) {
     return (int)g;
 }
 
-void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t total) {
-    if (total <= 0 || nd <= 0) return;
-    ProfScope ps(st, PK_COPY, 0.0, 16.0 * (double)total);
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd, total);
+void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t chunks,
+                 int64_t elems) {
+    if (chunks <= 0 || nd <= 0) return;
+    ProfScope ps(st, PK_COPY, 0.0, 16.0 * (double)elems);
+    const int grid = (int)std::min<int64_t>(chunks, 8192);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, d_descs, nd, chunks);
     TNR_HIP(hipGetLastError());
 }
 
