@@ -372,7 +372,7 @@ static int pick_shape(const std::vector<GemmJob>& jobs, bool split32) {
 // later without waiting); once every shape has TUNE_SAMPLES timings the
 // fastest is used for that key.  Until then the heuristic above decides.
 namespace {
-constexpr int TUNE_SAMPLES = 2;
+constexpr int TUNE_SAMPLES = 3;
 using TuneKey = std::array<int, 9>;
 struct TuneTrial {
     hipEvent_t e0 = nullptr, e1 = nullptr;
