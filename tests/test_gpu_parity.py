@@ -158,3 +158,30 @@ def test_lanes_do_not_change_results(tk, monkeypatch):
         out[lanes] = tk.run_batch(reqs)
     for a, b, c in zip(out["1"], out["2"], out["3"]):
         assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_device_run_equals_host_loop(tk):
+    """run!(.., maxiter(n)) as one device call (tnr_run) and the generic host
+    loop of step!/finalize! (taken for any non-default Finalizer) give the
+    same norms, bit for bit, for every scheme."""
+    for cls, chi in ((tk.TRG, 12), (tk.BTRG, 16), (tk.HOTRG, 8)):
+        T = tk.classical_ising(0.42, symmetry="Z2")
+        a = gpu_run(tk, cls, T, chi, 7)
+        sc = cls(T)
+        b = np.array(tk.run(sc, tk.truncrank(chi), tk.maxiter(7), tk.Finalizer(tk.finalize),
+                            verbosity=0))
+        sc.close()
+        assert np.array_equal(a, b)
+
+
+def test_convcrit_run_stops_like_the_reference(tk):
+    """trivial_convcrit on the device path: the loop stops once the last norm
+    falls to delta (stopping.jl:19, 42); norms match a maxiter run of the
+    same length."""
+    T = tk.classical_ising(0.30, symmetry="Trivial")          # high temperature: norms -> 1
+    sc = tk.BTRG(T)
+    d = np.array(tk.run(sc, tk.truncrank(8), tk.maxiter(40) & tk.trivial_convcrit(1.0 + 1e-9),
+                        verbosity=0))
+    sc.close()
+    ref = gpu_run(tk, tk.BTRG, T, 8, len(d) - 1)
+    assert np.array_equal(d, ref)
