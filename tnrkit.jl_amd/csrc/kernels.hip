@@ -660,18 +660,40 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                 G[qi + pj * ld] = y10; G[pj + qi * ld] = y10;
                 G[qi + qj * ld] = y11; G[qj + qi * ld] = y11;
             }
-            // W <- W J: thread tasks (pair j, row r), j += JT / np per step
-            for (int j = wj0, r = wr0; j < half;) {
-                const int4 Pj = pq[j];
-                if (Pj.z) {
-                    const double2 rj = rcs[j];
-                    const double wp = W[r + Pj.x * ld], wq = W[r + Pj.y * ld];
-                    W[r + Pj.x * ld] = rj.x * wp - rj.y * wq;
-                    W[r + Pj.y * ld] = rj.y * wp + rj.x * wq;
+            // W <- W J: this thread's (pair, row) tasks touch distinct
+            // columns, so all their loads are issued before any store
+            {
+                constexpr int WT = (LMAX / 2) * LMAX / JT + 1;   // tasks per thread, max
+                int wa[WT], wb[WT];
+                double wp[WT], wq[WT], wc[WT], ws[WT];
+                bool wl[WT];
+                int j = wj0, r = wr0;
+#pragma unroll
+                for (int u = 0; u < WT; ++u) {
+                    wl[u] = false;
+                    if (j < half) {
+                        const int4 Pj = pq[j];
+                        if (Pj.z) {
+                            const double2 rj = rcs[j];
+                            wa[u] = r + Pj.x * ld;
+                            wb[u] = r + Pj.y * ld;
+                            wc[u] = rj.x;
+                            ws[u] = rj.y;
+                            wp[u] = W[wa[u]];
+                            wq[u] = W[wb[u]];
+                            wl[u] = true;
+                        }
+                    }
+                    r += wdr;
+                    j += wdj;
+                    if (r >= np) { r -= np; ++j; }
                 }
-                r += wdr;
-                j += wdj;
-                if (r >= np) { r -= np; ++j; }
+#pragma unroll
+                for (int u = 0; u < WT; ++u)
+                    if (wl[u]) {
+                        W[wa[u]] = wc[u] * wp[u] - ws[u] * wq[u];
+                        W[wb[u]] = ws[u] * wp[u] + wc[u] * wq[u];
+                    }
             }
             __syncthreads();
         }
@@ -740,8 +762,9 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
     for (int i = t; i < n; i += CQT) perm[i] = i;
     __syncthreads();
     {
-        double v = (t < n) ? A[t + t * ld] : 0.0;
+        double v = 0.0;
         int ii = t;
+        for (int c = t; c < n; c += CQT) v = fmax(v, A[c + c * ld]);
         argmax_wave(v, ii);
         if (lane == 0) bv[w] = v;
         __syncthreads();
@@ -757,9 +780,12 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
     for (int j = 0; j < n; ++j) {
         double v = -1.0;
         int ii = n;
-        if (t < n - j) {
-            ii = j + t;
-            v = A[ii + ii * ld];
+        for (int c = j + t; c < n; c += CQT) {
+            const double dc = A[c + c * ld];
+            if (dc > v) {
+                v = dc;
+                ii = c;
+            }
         }
         argmax_wave(v, ii);
         if (lane == 0) {
@@ -808,23 +834,40 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
         for (int k = j + 1 + t; k < n; k += CQT) A[j + k * ld] /= r;
         if (t == 0) A[j + j * ld] = r;
         __syncthreads();
-        const int m = n - j - 1;
-        for (int e = t; e < m * m; e += CQT) {
-            const int kk = e / m, i = j + 1 + (e - kk * m), k = j + 1 + kk;
-            A[i + k * ld] -= A[j + i * ld] * A[j + k * ld];
+        // trailing update: rows across the lanes, columns k = j+1+w (mod 4)
+        // per wave, four columns per pass so their loads overlap
+        constexpr int WS = CQT / 64;
+        for (int i = j + 1 + lane; i < n; i += 64) {
+            const double aji = A[j + i * ld];
+            int k = j + 1 + w;
+            for (; k + 3 * WS < n; k += 4 * WS) {
+                const double b0 = A[j + k * ld], b1 = A[j + (k + WS) * ld];
+                const double b2 = A[j + (k + 2 * WS) * ld], b3 = A[j + (k + 3 * WS) * ld];
+                const double x0 = A[i + k * ld], x1 = A[i + (k + WS) * ld];
+                const double x2 = A[i + (k + 2 * WS) * ld], x3 = A[i + (k + 3 * WS) * ld];
+                A[i + k * ld] = x0 - aji * b0;
+                A[i + (k + WS) * ld] = x1 - aji * b1;
+                A[i + (k + 2 * WS) * ld] = x2 - aji * b2;
+                A[i + (k + 3 * WS) * ld] = x3 - aji * b3;
+            }
+            for (; k < n; k += WS) A[i + k * ld] -= aji * A[j + k * ld];
         }
         __syncthreads();
     }
-    // X = R^{-1}, one column per thread (back substitution)
-    for (int c = t; c < rank; c += CQT) {
-        X[c + c * ld] = 1.0 / A[c + c * ld];
-        for (int i = c - 1; i >= 0; --i) {
-            double s = 0.0;
-            for (int k = i + 1; k <= c; ++k) s += A[i + k * ld] * X[k + c * ld];
-            X[i + c * ld] = -s / A[i + i * ld];
-        }
-    }
+    // X = R^{-1}: solve R X = I by right-looking elimination from the last
+    // row up; each step is parallel over the columns (waves) and rows (lanes)
+    for (int c = t; c < rank; c += CQT) X[c + c * ld] = 1.0;
     __syncthreads();
+    for (int kk = rank - 1; kk >= 0; --kk) {
+        const double rkk = A[kk + kk * ld];
+        for (int c = kk + t; c < rank; c += CQT) X[kk + c * ld] /= rkk;
+        __syncthreads();
+        for (int c = kk + w; c < rank; c += CQT / 64) {
+            const double xk = X[kk + c * ld];
+            for (int i = lane; i < kk; i += 64) X[i + c * ld] -= A[i + kk * ld] * xk;
+        }
+        __syncthreads();
+    }
     for (int e = t; e < n * n; e += CQT) {
         const int j = e / n, i = e - j * n;
         const double x = (j < rank && i <= j) ? X[i + j * ld] : 0.0;
