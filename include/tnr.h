@@ -47,6 +47,8 @@ extern "C" {
 #define TNR_TRG    0   /* src/schemes/trg.jl:26-41   */
 #define TNR_BTRG   1   /* src/schemes/btrg.jl:26-94  */
 #define TNR_HOTRG  2   /* src/schemes/hotrg.jl:27-160 */
+#define TNR_IMPURITY_TRG    3   /* src/schemes/impuritytrg.jl:40-103 */
+#define TNR_IMPURITY_HOTRG  4   /* src/schemes/impurityhotrg.jl:27-95 */
 
 #define TNR_PREC_FP64     0  /* all contractions FP64 MFMA */
 #define TNR_PREC_SPLIT32  1  /* contractions on FP32 MFMA, hi/lo split
@@ -73,8 +75,9 @@ int tnr_finalize(tnr_scheme* s, double* norm);
 
 /* run!(scheme, truncrank(chi), maxiter(maxiter); finalize_beginning):
  * src/schemes/tnrscheme.jl:33-59 with src/utility/stopping.jl:3-5,18.
- * `norms` must hold maxiter + 1 doubles; *n_out receives the count written
- * (maxiter + finalize_beginning). */
+ * `norms` must hold max(maxiter, 1) + finalize_beginning rows of
+ * tnr_ncomponents(kind) doubles (one per finalize!); *n_out receives the
+ * number of rows written. */
 int tnr_run(tnr_scheme* s, int chi, int maxiter, int finalize_beginning,
             double* norms, int* n_out);
 
@@ -87,6 +90,26 @@ int tnr_free_energy(const double* norms, int n, double beta,
 int tnr_get_dims(const tnr_scheme* s, int32_t* leg_dims, size_t* ndata);
 /* Copy T out (format above) / replace T (checkpoint, resume, debugging). */
 int tnr_get_tensor(tnr_scheme* s, double* data, size_t ndata);
+
+/* ---- impurity schemes ------------------------------------------------- */
+/* ImpurityTRG(T, T_imp1, T_imp2, T_imp3, T_imp4)
+ * (src/schemes/impuritytrg.jl:40-58, ntensors = 5) or
+ * ImpurityHOTRG(T, T_imp_order1_1, T_imp_order1_2, T_imp_order2)
+ * (src/schemes/impurityhotrg.jl:27-46, ntensors = 4): data[i] / ndata[i] is
+ * tensor i in the format of tnr_create; all share leg_dims (the reference
+ * asserts equal spaces).  tnr_step and tnr_run work as for the other kinds. */
+int tnr_create_multi(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims,
+                     const double* const* data, const size_t* ndata, int ntensors,
+                     int precision);
+/* Values per finalize!: 1 (TRG, BTRG, HOTRG), 2 (ImpurityTRG: npure, nimp,
+ * src/utility/finalize.jl:74-88), 4 (ImpurityHOTRG: n, n_11, n_12, n_2,
+ * finalize.jl:90-101); negative status for an unknown kind. */
+int tnr_ncomponents(int kind);
+/* finalize! of any scheme: writes tnr_ncomponents(kind) values. */
+int tnr_finalize_multi(tnr_scheme* s, double* out);
+/* Tensor `index` of the scheme (0 = T, 1.. = the impurity tensors in
+ * constructor order), in the format of tnr_create. */
+int tnr_get_tensor_at(tnr_scheme* s, int index, double* data, size_t ndata);
 int tnr_set_tensor(tnr_scheme* s, const int32_t* leg_dims,
                    const double* data, size_t ndata);
 /* BTRG bond weights: which = 1 -> S1 (vertical), 2 -> S2 (horizontal);

@@ -37,6 +37,17 @@ int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta,
 int tnr_kernel_svd(int m, int n, const double* M, int k, double* U,
                    double* s, double* V, int* iters);
 
+/* Pivoted-row gauge of k kept left singular vectors U (m x k, column-major,
+ * ld = m) with singular values s (descending): Q (k x k, column-major) such
+ * that U Q (and V Q on the right side) is the basis the impurity TRG's SVDs
+ * use.  Singular values within 1e-9 * s[0] of their neighbour form a
+ * cluster, values at or below 1e-14 * s[0] are null (Q = I there); inside a
+ * cluster column t of U Q is the normalised projection of the coordinate
+ * vector e_r onto the part of the span orthogonal to columns 0..t-1, r the
+ * first row whose residual norm is within 1e-10 of the largest.  No
+ * counterpart in the reference, which keeps LAPACK's basis; k <= 96. */
+int tnr_kernel_pivoted_gauge(int m, int k, const double* U, const double* s, double* Q);
+
 /* Largest-|lambda| k eigenpairs of the symmetrised n x n matrix M:
  * U (n x k), lam (k). */
 int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam,

@@ -39,24 +39,38 @@ def _cosh_sinh(beta):
     return np.cosh(b), np.sinh(b)
 
 
-def classical_ising_trivial(beta) -> ZT:
-    """src/models/ising.jl:10-15,44-55 (h = 0): 2 * delta_even * bond^4 with
-    bond = diag(sqrt(cosh b), sqrt(sinh b)); BigFloat beta is honoured by
-    evaluating the bond entries in high precision before rounding."""
+def _ising_trivial(beta, h, impurity):
     x, y = _cosh_sinh(beta)
     if isinstance(beta, mpmath.mpf):
         w = np.array([float(mpmath.sqrt(x)), float(mpmath.sqrt(y))])
+        ch, sh = float(mpmath.cosh(h * beta)), float(mpmath.sinh(h * beta))
     else:
         w = np.array([np.sqrt(x), np.sqrt(y)])
+        ch, sh = np.cosh(h * float(beta)), np.sinh(h * float(beta))
+    even, odd = (sh, ch) if impurity else (ch, sh)
     init = np.zeros((2, 2, 2, 2))
     for i in range(2):
         for j in range(2):
             for k in range(2):
                 for l in range(2):
-                    init[i, j, k, l] = 1.0 if (i + j + k + l) % 2 == 0 else 0.0
+                    init[i, j, k, l] = even if (i + j + k + l) % 2 == 0 else odd
     t = 2.0 * init * w[:, None, None, None] * w[None, :, None, None] \
         * w[None, None, :, None] * w[None, None, None, :]
     return ZT(t)
+
+
+def classical_ising_trivial(beta, h=0.0) -> ZT:
+    """src/models/ising.jl:10-15,44-55: 2 * init * bond^4, init = cosh(h b)
+    on even-parity index sums and sinh(h b) on odd ones, bond =
+    diag(sqrt(cosh b), sqrt(sinh b)); BigFloat beta is honoured by
+    evaluating the entries in high precision before rounding."""
+    return _ising_trivial(beta, h, impurity=False)
+
+
+def classical_ising_impurity(beta, h=0.0) -> ZT:
+    """src/models/ising.jl:70-102: the magnetisation impurity, init with the
+    parities swapped (sinh(h b) on even sums, cosh(h b) on odd)."""
+    return _ising_trivial(beta, h, impurity=True)
 
 
 def classical_ising_z2(beta) -> ZT:

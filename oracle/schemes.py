@@ -10,6 +10,8 @@ Restates, function by function:
 * ``run``        - src/schemes/tnrscheme.jl:33-59 with ``maxiter``
   (src/utility/stopping.jl:3-5,18)
 * ``free_energy``- src/utility/free_energy.jl:13-20
+* ``ImpurityTRG``  - src/schemes/impuritytrg.jl:63-91, finalize.jl:74-88
+* ``ImpurityHOTRG``- src/schemes/impurityhotrg.jl:48-84, finalize.jl:90-101
 
 Contractions are pairwise in a cost-optimal order (TensorOperations'
 order only changes roundoff).  Each factorization appends its truncation
@@ -36,9 +38,10 @@ def _diag(s, q_cod, N) -> ZT:
     return ZT(np.diag(s), [q_cod, (-q_cod) % N], N)
 
 
-def svd12(T: ZT, chi: int, log=None):
-    """src/utility/projectors.jl:211-217: (U*sqrt(s), sqrt(s)*V)."""
-    U, s, Vt, info = svd_trunc(T, 2, chi)
+def svd12(T: ZT, chi: int, log=None, gauge="lapack"):
+    """src/utility/projectors.jl:211-217: (U*sqrt(s), sqrt(s)*V); ``gauge``
+    as in svd_trunc."""
+    U, s, Vt, info = svd_trunc(T, 2, chi, gauge)
     if log is not None:
         log.append(info)
     sq = np.sqrt(s)
@@ -195,6 +198,96 @@ def step_hotrg_x(A1: ZT, A2: ZT, Uy: ZT) -> ZT:
     R = contract([-1, -2, -3, -4, -5], (A1, [-1, 1, -2, -3], False), (Uy, [1, -4, -5], True))
     S = contract([-1, -2, -3, -4, -5], (R, [-1, -2, 5, 2, -3], False), (A2, [5, 2, -4, -5], False))
     return contract([-1, -2, -3, -4], (S, [-1, 3, -2, 4, -4], False), (Uy, [3, 4, -3], False))
+
+
+# ---------------------------------------------------------------- impurities
+
+def _trg_net(D, B, C, A) -> ZT:
+    """T[-1 -2; -3 -4] := D[-2; 1 2] * B[-1; 4 1] * C[4 3; -3] * A[3 2; -4]"""
+    X = contract([-1, -2, -3, -4], (D, [-2, 1, -3], False), (B, [-1, -4, 1], False))
+    Y = contract([-1, -2, -3, -4], (C, [-1, 3, -3], False), (A, [3, -2, -4], False))
+    return contract([-1, -2, -3, -4], (X, [-1, -2, 2, 4], False), (Y, [4, 2, -3, -4], False))
+
+
+class ImpurityTRG:
+    """src/schemes/impuritytrg.jl:40-91: the pure tensor T and four
+    impurity tensors on a 2x2 plaquette (1 2 / 4 3).
+
+    The impurity norm of this network contracts B1 with Ap and A1 with Bp
+    (factors of different SVDs), so it depends on the singular-vector basis
+    of every SVD, not only on the tensors: ``gauge="lapack"`` is the
+    reference (LAPACK's basis, whatever it is), ``gauge="pivoted"`` fixes
+    the basis with ztensor.pivoted_gauge."""
+    scalefactor = 2.0
+
+    def __init__(self, T: ZT, T1: ZT, T2: ZT, T3: ZT, T4: ZT, gauge="lapack"):
+        self.T, self.T1, self.T2, self.T3, self.T4 = (x.copy() for x in (T, T1, T2, T3, T4))
+        self.log = []
+        self.gauge = gauge
+
+    def step(self, chi):
+        tp = (1, 3, 0, 2)                          # transpose(., ((2, 4), (1, 3)))
+        g = self.gauge
+        A1, B1 = svd12(self.T1, chi, self.log, g)
+        C2, D2 = svd12(permute(self.T2, tp), chi, self.log, g)
+        A3, B3 = svd12(self.T3, chi, self.log, g)
+        C4, D4 = svd12(permute(self.T4, tp), chi, self.log, g)
+        Ap, Bp = svd12(self.T, chi, self.log, g)
+        Cp, Dp = svd12(permute(self.T, tp), chi, self.log, g)
+        self.T = _trg_net(Dp, Bp, Cp, Ap)
+        self.T1 = _trg_net(Dp, B1, C2, Ap)
+        self.T2 = _trg_net(D2, B3, Cp, Ap)
+        self.T3 = _trg_net(D4, Bp, Cp, A3)
+        self.T4 = _trg_net(Dp, Bp, C4, A1)
+        return self
+
+    def finalize(self):
+        """finalize.jl:74-88: npure = |tr T[1 2; 2 1]|, every tensor /= npure;
+        nimp = |T1[5 4; 6 1] T2[1 2; 7 5] T3[3 7; 2 8] T4[8 6; 4 3]|."""
+        npure = abs(float(np.einsum("abba->", self.T.data)))
+        for t in (self.T, self.T1, self.T2, self.T3, self.T4):
+            t.data /= npure
+        nimp = abs(float(np.einsum("edfa,abge,cgbh,hfdc->", self.T1.data, self.T2.data,
+                                   self.T3.data, self.T4.data, optimize="optimal")))
+        return (npure, nimp)
+
+
+class ImpurityHOTRG:
+    """src/schemes/impurityhotrg.jl:27-84: T, the two first-order impurity
+    tensors and the second-order one; y projector then x step, x projector
+    then y step, impurity tensors averaged over their placements."""
+    scalefactor = 4.0
+
+    def __init__(self, T: ZT, T11: ZT, T12: ZT, T2: ZT):
+        self.T, self.T11, self.T12, self.T2 = (x.copy() for x in (T, T11, T12, T2))
+        self.log = []
+
+    def _half(self, proj, apply, chi):
+        U, _ = proj(self.T, self.T, chi, self.log)
+        f = lambda a, b: apply(a, b, U).data
+        T, T11, T12, T2 = self.T, self.T11, self.T12, self.T2
+        n_T = f(T, T)
+        n_11 = 0.5 * (f(T11, T) + f(T, T11))
+        n_12 = 0.5 * (f(T12, T) + f(T, T12))
+        n_2 = 0.25 * (f(T2, T) + f(T, T2) + f(T11, T12) + f(T12, T11))
+        q = apply(T, T, U).q
+        self.T = ZT(n_T, q, T.N)
+        self.T11 = ZT(n_11, q, T.N)
+        self.T12 = ZT(n_12, q, T.N)
+        self.T2 = ZT(n_2, q, T.N)
+
+    def step(self, chi):
+        self._half(get_hotrg_yproj, step_hotrg_x, chi)
+        self._half(get_hotrg_xproj, step_hotrg_y, chi)
+        return self
+
+    def finalize(self):
+        """finalize.jl:90-101: traces of all four, each divided by |tr T|."""
+        ns = [abs(float(np.einsum("abba->", t.data)))
+              for t in (self.T, self.T11, self.T12, self.T2)]
+        for t in (self.T, self.T11, self.T12, self.T2):
+            t.data /= ns[0]
+        return tuple(ns)
 
 
 # ---------------------------------------------------------------- driver

@@ -135,6 +135,35 @@ def matrix_blocks(t: ZT, ncod: int):
         yield c, r, k
 
 
+def block_row_rank(t: ZT, ncod: int, rows):
+    """Position of each dense row (C-order index over legs[:ncod]) inside its
+    coupled-charge block in TensorKit's block layout: sector tuples
+    (labels = outgoing charges) in colexicographic order, first leg fastest,
+    and inside a tuple the elements column-major over the legs.  The
+    pivoted-row gauge breaks ties between rows in this order."""
+    rows = np.asarray(rows)
+    shape = t.data.shape[:ncod]
+    idx = np.unravel_index(rows, shape)
+    tup = np.zeros(len(rows), dtype=np.int64)
+    inner = np.zeros(len(rows), dtype=np.int64)
+    stride = np.ones(len(rows), dtype=np.int64)
+    for l in range(ncod):
+        q = np.asarray(t.q[l]) % t.N
+        pos = np.zeros(len(q), dtype=np.int64)
+        dim = np.zeros(t.N, dtype=np.int64)
+        for i, a in enumerate(q):
+            pos[i] = dim[a]
+            dim[a] += 1
+        a = q[idx[l]]
+        tup += a * t.N ** l
+        inner += pos[idx[l]] * stride
+        stride = stride * dim[a]
+    order = np.lexsort((inner, tup))
+    rank = np.empty(len(rows), dtype=np.int64)
+    rank[order] = np.arange(len(rows))
+    return rank
+
+
 class Trunc:
     """Diagnostics of one truncated factorization (the sigma gap log)."""
 
@@ -176,12 +205,72 @@ def _global_cut(vals_per_block, chi, by_abs):
     return kept, flat, keep, drop
 
 
-def svd_trunc(t: ZT, ncod: int, chi: int):
+# Pivoted-row gauge of a truncated SVD (used by the impurity TRG, whose
+# impurity norm contracts factors of different SVDs with each other and so
+# depends on the basis each SVD returns; not part of the reference, which
+# takes whatever LAPACK returns).  Per block: singular values within
+# GAUGE_CLUSTER * s_max of their neighbour form a cluster, values at or
+# below GAUGE_NULL * s_max are null (their vectors are set to zero); inside
+# each cluster the basis is rebuilt as u_t = P_t e_r / |P_t e_r|, with P_t
+# the projector on the part of the cluster's span orthogonal to u_0..u_{t-1}
+# and r the first row whose residual norm e_r^T P_t e_r is within GAUGE_TIE
+# of the largest.  Every u_t is then a function of the span alone (any
+# rotation or sign of the input basis gives the same result; a lone vector
+# only has its sign set), is positive at its pivot row, and a symmetry that acts diagonally on the row
+# coordinates (the hidden Z2 of a Trivial Ising tensor) maps each u_t to
+# +-u_t, so the kept vectors never mix even and odd sectors.
+GAUGE_CLUSTER = 1e-9
+GAUGE_NULL = 1e-14
+GAUGE_TIE = 1e-10
+
+
+def pivoted_gauge(u, s, vt, row_rank=None):
+    """u (rows x k), s (k, descending), vt (k x cols) of one block: the
+    same factorization in the pivoted-row gauge; returns (u, vt).  Ties
+    between rows go to the smallest ``row_rank`` (default: row index)."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    vt = np.array(vt, dtype=np.float64, copy=True)
+    k = len(s)
+    if k == 0:
+        return u, vt
+    null = s <= GAUGE_NULL * s[0]
+    u[:, null] = 0.0
+    vt[null, :] = 0.0
+    i = 0
+    while i < k:
+        if null[i]:
+            break
+        j = i + 1
+        while j < k and not null[j] and s[j - 1] - s[j] <= GAUGE_CLUSTER * s[0]:
+            j += 1
+        d = j - i
+        Uc = u[:, i:j]
+        Cq = np.zeros((d, d))
+        rn0 = np.sum(Uc * Uc, axis=1)
+        for t in range(d):
+            proj = Uc @ Cq[:, :t]
+            rn = rn0 - np.sum(proj * proj, axis=1)
+            cand = np.flatnonzero(rn >= (1.0 - GAUGE_TIE) * rn.max())
+            r = int(cand[0] if row_rank is None else cand[np.argmin(row_rank[cand])])
+            if d == 1:                     # a lone vector: only its sign
+                Cq[0, 0] = 1.0 if Uc[r, 0] >= 0.0 else -1.0
+                break
+            c = Uc[r, :] - Cq[:, :t] @ proj[r, :]
+            Cq[:, t] = c / np.sqrt(rn[r])
+        u[:, i:j] = Uc @ Cq
+        vt[i:j, :] = Cq.T @ vt[i:j, :]
+        i = j
+    return u, vt
+
+
+def svd_trunc(t: ZT, ncod: int, chi: int, gauge: str = "lapack"):
     """Truncated SVD of the (legs[:ncod] | legs[ncod:]) matrix view.
 
     Returns (U, s, Vt, info): U has legs (cod..., new), Vt (new, dom...),
     s the kept singular values in new-leg order (by sector, descending
-    within a sector)."""
+    within a sector).  ``gauge`` "lapack" keeps LAPACK's singular vectors
+    (the reference), "pivoted" applies pivoted_gauge to each block's kept
+    triplets."""
     nl = t.data.ndim
     rshape = t.data.shape[:ncod]
     cshape = t.data.shape[ncod:]
@@ -204,8 +293,13 @@ def svd_trunc(t: ZT, ncod: int, chi: int):
         if not idx:
             continue
         m = len(idx)
-        U[r, pos:pos + m] = u[:, idx]
-        Vt[pos:pos + m, k] = vt[idx, :]
+        ub, vb = u[:, idx], vt[idx, :]
+        if gauge == "pivoted":
+            ub, vb = pivoted_gauge(ub, s[idx], vb, block_row_rank(t, ncod, r))
+        elif gauge != "lapack":
+            raise ValueError("gauge must be 'lapack' or 'pivoted'")
+        U[r, pos:pos + m] = ub
+        Vt[pos:pos + m, k] = vb
         s_out[pos:pos + m] = s[idx]
         qnew[pos:pos + m] = c
         sectors[c] = m

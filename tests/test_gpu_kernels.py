@@ -173,3 +173,56 @@ def test_eigh_truncated(tk, n, k):
     Pk = U @ U.T
     Pr = v[:, :k] @ v[:, :k].T
     assert np.max(np.abs(Pk - Pr)) < 1e-9
+
+
+def _z2_factor(rng, m, k, s):
+    """Left singular vectors of a matrix whose rows split into even (first
+    half) and odd coordinates, every value of s shared by one even and one
+    odd vector, handed over scrambled inside each degenerate pair."""
+    h = m // 2
+    E, _ = np.linalg.qr(rng.standard_normal((h, k)))
+    O, _ = np.linalg.qr(rng.standard_normal((m - h, k)))
+    U = np.zeros((m, 2 * k))
+    U[:h, 0::2] = E
+    U[h:, 1::2] = O
+    for j in range(k):
+        q, _ = np.linalg.qr(rng.standard_normal((2, 2)))
+        U[:, 2 * j:2 * j + 2] = U[:, 2 * j:2 * j + 2] @ q
+    return U, np.repeat(s, 2)
+
+
+@pytest.mark.parametrize("m,k,null", [(40, 6, 0), (300, 24, 4), (97, 40, 8)])
+def test_pivoted_gauge_kernel_vs_oracle(tk, m, k, null):
+    """k_canon against ztensor.pivoted_gauge: scrambled degenerate pairs
+    come back as the same basis (to 1e-12), each vector purely even or
+    purely odd, positive at its pivot; distinct values only change sign;
+    null columns are left alone."""
+    from oracle.ztensor import pivoted_gauge
+    rng = np.random.default_rng(m + k)
+    s = np.sort(rng.uniform(0.1, 1.0, k // 2))[::-1]
+    U, sv = _z2_factor(rng, m, k // 2, s)
+    if null:                      # whole degenerate pairs
+        U[:, -null:] = 0.0
+        sv[-null:] = 0.0
+    Q = np.zeros((k, k), order="F")
+    Uf = np.asfortranarray(U)
+    st = tk.lib().tnr_kernel_pivoted_gauge(m, k, p(Uf), p(np.ascontiguousarray(sv)), p(Q))
+    assert st == 0, tk.lib().tnr_last_error()
+    got = U @ Q
+    ref, _ = pivoted_gauge(U, sv, np.eye(k))
+    assert np.max(np.abs(got - ref)) < 1e-12
+    h = m // 2
+    live = sv > 1e-14 * sv[0]
+    for j in np.flatnonzero(live):
+        even, odd = np.linalg.norm(got[:h, j]), np.linalg.norm(got[h:, j])
+        assert min(even, odd) < 1e-13
+    assert np.max(np.abs(got.T[live][:, :] @ got[:, live] - np.eye(int(live.sum())))) < 1e-12
+    # distinct values: a sign per column, taken from the pivot row
+    s2 = np.linspace(1.0, 0.2, k)
+    V, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    Q2 = np.zeros((k, k), order="F")
+    Vf = np.asfortranarray(V)
+    assert tk.lib().tnr_kernel_pivoted_gauge(m, k, p(Vf), p(s2), p(Q2)) == 0
+    assert np.array_equal(np.abs(Q2), np.eye(k))
+    ref2, _ = pivoted_gauge(V, s2, np.eye(k))
+    assert np.array_equal(V @ Q2, ref2)

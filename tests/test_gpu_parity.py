@@ -185,3 +185,88 @@ def test_convcrit_run_stops_like_the_reference(tk):
     sc.close()
     ref = gpu_run(tk, tk.BTRG, T, 8, len(d) - 1)
     assert np.array_equal(d, ref)
+
+
+def _close_rows(got, ref, cols=None, rtol=1e-10):
+    """Row-wise: every compared component within rtol of the oracle, or
+    within rtol * 1e-4 of the row's first value when it is smaller."""
+    got, ref = np.asarray(got, dtype=float), np.asarray(ref, dtype=float)
+    assert got.shape == ref.shape
+    if cols is not None:
+        got, ref = got[:, cols], ref[:, cols]
+    tol = rtol * (np.abs(ref) + 1e-4 * np.abs(ref[:, :1]))
+    return bool(np.all(np.abs(got - ref) <= tol)), float(np.max(np.abs(got - ref) / tol))
+
+
+# The impurity TRG norm contracts factors of different SVDs (B1 with Ap,
+# A1 with Bp), so it depends on each SVD's singular-vector basis: the
+# device fixes it with the pivoted-row gauge and is compared with the
+# oracle in the same gauge.  At h = 0, <m> = 0 by the Z2 symmetry; the gauge
+# keeps it at roundoff (~1e-16 npure) for the first steps, compared with the
+# floor of 1e-14 npure; a later step can lift the odd-sector roundoff (to
+# 3e-7 npure at beta 0.3, step 6), which the cases stop short of.
+@pytest.mark.parametrize("beta,h,chi,n,cols", [
+    (0.44, 0.1, 8, 8, None), (0.40, 0.1, 12, 6, None), (0.6, 0.05, 8, 6, None),
+    (0.3, 0.0, 8, 5, None), (0.44, 0.0, 8, 8, None)])
+def test_impurity_trg_vs_oracle(tk, beta, h, chi, n, cols):
+    """src/schemes/impuritytrg.jl:63-91 with finalize.jl:74-88: (npure, nimp)
+    per step against the oracle, magnetisation impurity at site 1."""
+    T, Ti = tk.classical_ising(beta, "Trivial", h), tk.classical_ising_impurity(beta, h)
+    sc = tk.ImpurityTRG(T, Ti, T, T, T)
+    got = tk.run(sc, tk.truncrank(chi), tk.maxiter(n), verbosity=0)
+    sc.close()
+    To, Tio = om.classical_ising_trivial(beta, h), om.classical_ising_impurity(beta, h)
+    ref = osch.run(osch.ImpurityTRG(To, Tio, To, To, To, gauge="pivoted"), chi, n)
+    ok, worst = _close_rows(got, ref, cols)
+    assert ok, worst
+
+
+@pytest.mark.parametrize("beta,h,chi,n,cols", [
+    (0.44, 0.0, 8, 5, [0, 3]), (0.35, 0.05, 6, 5, None), (0.5, 0.1, 8, 4, None)])
+def test_impurity_hotrg_vs_oracle(tk, beta, h, chi, n, cols):
+    """src/schemes/impurityhotrg.jl:48-84 with finalize.jl:90-101:
+    (n, n_11, n_12, n_2) per step against the oracle (n_2 = <sigma sigma>
+    is Z2-even and compared at h = 0 too)."""
+    T, Ti = tk.classical_ising(beta, "Trivial", h), tk.classical_ising_impurity(beta, h)
+    sc = tk.ImpurityHOTRG(T, Ti, Ti, T)
+    got = tk.run(sc, tk.truncrank(chi), tk.maxiter(n), verbosity=0)
+    sc.close()
+    To, Tio = om.classical_ising_trivial(beta, h), om.classical_ising_impurity(beta, h)
+    ref = osch.run(osch.ImpurityHOTRG(To, Tio, Tio, To), chi, n)
+    ok, worst = _close_rows(got, ref, cols)
+    assert ok, worst
+
+
+def test_impurity_reference_gates(tk):
+    """The reference's own impurity tests on the device path:
+    test/schemes.jl:443-450 (ImpurityTRG chi 24, Onsager 2e-6),
+    :400-410 (ImpurityHOTRG chi 16, Onsager 6e-7), :454-475 (<m> -> 0 at
+    beta 0.1, -> 1 at beta 2 with h 1e-6) and :412-438 (<m^2> -> 0 at
+    beta 0.2, -> 1 at beta 1)."""
+    T, Ti = tk.classical_ising(None, "Trivial"), tk.classical_ising_impurity(None)
+    d = tk.run(tk.ImpurityTRG(T, Ti, T, T, T), tk.truncrank(24), tk.maxiter(25), verbosity=0)
+    f = tk.free_energy([x[0] for x in d], tk.ising_betac)
+    assert abs(f / tk.f_onsager - 1) < 2e-6
+    d = tk.run(tk.ImpurityHOTRG(T, Ti, Ti, T), tk.truncrank(16), tk.maxiter(25), verbosity=0)
+    f = tk.free_energy([x[0] for x in d], tk.ising_betac, scalefactor=4.0)
+    assert abs(f / tk.f_onsager - 1) < 6e-7
+    T, Ti = tk.classical_ising(0.1, "Trivial"), tk.classical_ising_impurity(0.1)
+    d = tk.run(tk.ImpurityTRG(T, Ti, T, T, T), tk.truncrank(16), tk.maxiter(25), verbosity=0)
+    assert abs(d[-1][1] / d[-1][0]) < 1e-4
+    T, Ti = tk.classical_ising(2.0, "Trivial", 1e-6), tk.classical_ising_impurity(2.0, 1e-6)
+    d = tk.run(tk.ImpurityTRG(T, Ti, T, T, T), tk.truncrank(16), tk.maxiter(25), verbosity=0)
+    assert abs(d[-1][1] / d[-1][0] - 1) < 1e-4
+    T, Ti = tk.classical_ising(0.2, "Trivial"), tk.classical_ising_impurity(0.2)
+    d = tk.run(tk.ImpurityHOTRG(T, Ti, Ti, T), tk.truncrank(8), tk.maxiter(25), verbosity=0)
+    assert abs(d[-1][3] / d[-1][0]) < 1e-14
+    T, Ti = tk.classical_ising(1.0, "Trivial"), tk.classical_ising_impurity(1.0)
+    d = tk.run(tk.ImpurityHOTRG(T, Ti, Ti, T), tk.truncrank(8), tk.maxiter(25), verbosity=0)
+    assert abs(d[-1][3] / d[-1][0] - 1) < 1e-2
+
+
+def test_impurity_host_loop_equals_device_run(tk):
+    T, Ti = tk.classical_ising(0.42, "Trivial"), tk.classical_ising_impurity(0.42)
+    a = tk.run(tk.ImpurityTRG(T, Ti, T, T, T), tk.truncrank(8), tk.maxiter(4), verbosity=0)
+    b = tk.run(tk.ImpurityTRG(T, Ti, T, T, T), tk.truncrank(8), tk.maxiter(4),
+               tk.Finalizer(tk.finalize), verbosity=0)
+    assert a == b

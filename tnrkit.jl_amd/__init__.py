@@ -12,9 +12,11 @@ The directory name is not a valid Python identifier; load it with
 ``__graft_entry__.package()`` (it registers the module as ``tnrkit_jl_amd``).
 """
 from ._lib import TNRError, lib  # noqa: F401
-from .models import (classical_clock, classical_ising, classical_potts, f_onsager,  # noqa: F401
+from .models import (classical_clock, classical_ising, classical_ising_impurity,  # noqa: F401
+                     classical_potts, f_onsager,
                      ising_betac, potts_betac)
-from .schemes import (BTRG, FP64, HOTRG, SPLIT32, TRG, Finalizer, MultipleCrit,  # noqa: F401
+from .schemes import (BTRG, FP64, HOTRG, SPLIT32, TRG, Finalizer, ImpurityHOTRG,  # noqa: F401
+                      ImpurityTRG, MultipleCrit,
                       convcrit, default_Finalizer, finalize, free_energy, maxiter, run, step,
                       trivial_convcrit, truncrank)
 from .serving import Request, Server, run_batch  # noqa: F401

@@ -20,6 +20,7 @@ TNR_ERR_BUDGET = -9
 TNR_ERR_BUSY = -10
 
 TNR_TRG, TNR_BTRG, TNR_HOTRG = 0, 1, 2
+TNR_IMPURITY_TRG, TNR_IMPURITY_HOTRG = 3, 4
 TNR_PREC_FP64, TNR_PREC_SPLIT32 = 0, 1
 
 
@@ -54,6 +55,11 @@ SIGNATURES = {
     "tnr_free_energy": (C.c_int, [_DP, C.c_int, C.c_double, C.c_double, C.c_double, _DP]),
     "tnr_get_dims": (C.c_int, [_P, _I32P, _SZP]),
     "tnr_get_tensor": (C.c_int, [_P, _DP, C.c_size_t]),
+    "tnr_create_multi": (C.c_int, [_PP, C.c_int, C.c_int, _I32P, C.POINTER(_DP), _SZP, C.c_int,
+                                   C.c_int]),
+    "tnr_ncomponents": (C.c_int, [C.c_int]),
+    "tnr_finalize_multi": (C.c_int, [_P, _DP]),
+    "tnr_get_tensor_at": (C.c_int, [_P, C.c_int, _DP, C.c_size_t]),
     "tnr_set_tensor": (C.c_int, [_P, _I32P, _DP, C.c_size_t]),
     "tnr_get_bond_weights": (C.c_int, [_P, C.c_int, _DP, C.c_size_t]),
     "tnr_get_trunc_log": (C.c_int, [_P, _DP, C.c_int, _IP]),
@@ -72,6 +78,7 @@ SIGNATURES = {
                                   C.c_int, _DP, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                   _DP]),
     "tnr_kernel_svd": (C.c_int, [C.c_int, C.c_int, _DP, C.c_int, _DP, _DP, _DP, _IP]),
+    "tnr_kernel_pivoted_gauge": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP]),
     "tnr_kernel_eigh": (C.c_int, [C.c_int, _DP, C.c_int, _DP, _DP, _IP]),
     "tnr_kernel_jacobi": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP, _IP, _DP]),
     "tnr_profile_enable": (C.c_int, [C.c_int]),

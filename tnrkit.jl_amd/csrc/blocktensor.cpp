@@ -768,6 +768,7 @@ void trace_normalize_many(Ctx& ctx, const std::vector<TraceJob>& jobs) {
         P.size = L.size;
         P.partial = partial.d() + x * TRACE_PARTS;
         P.norm_slot = jobs[x].norm_slot;
+        P.divide_by = jobs[x].divide_by;
         probs.push_back(P);
         max_size = std::max(max_size, L.size);
     }
@@ -775,6 +776,32 @@ void trace_normalize_many(Ctx& ctx, const std::vector<TraceJob>& jobs) {
     TraceDesc* dd = ctx.stage->put(ctx.stream, descs);
     TraceProb* dp = ctx.stage->put(ctx.stream, probs);
     launch_trace_normalize(ctx.stream, dd, dp, (int)probs.size(), max_size);
+}
+
+// ------------------------------------------------------ linear combinations
+void lincomb_many(Ctx& ctx, const std::vector<LinJob>& jobs) {
+    std::vector<LinDesc> d;
+    int64_t start = 0;
+    for (const LinJob& j : jobs) {
+        TNR_REQUIRE(j.out && !j.terms.empty() && j.terms.size() <= 4, TNR_ERR_INTERNAL,
+                    "bad linear combination");
+        LinDesc x{};
+        x.out = j.out->d();
+        x.n = j.out->L.size;
+        x.nin = (int32_t)j.terms.size();
+        for (size_t i = 0; i < j.terms.size(); ++i) {
+            TNR_REQUIRE(j.terms[i].second->L.same(j.out->L), TNR_ERR_ARG,
+                        "combined tensors must share their spaces");
+            x.in[i] = j.terms[i].second->d();
+            x.coef[i] = j.terms[i].first;
+        }
+        x.start = start;
+        start += x.n;
+        if (x.n > 0) d.push_back(x);
+    }
+    if (d.empty()) return;
+    LinDesc* dd = ctx.stage->put(ctx.stream, d);
+    launch_lincomb(ctx.stream, dd, (int)d.size(), start);
 }
 
 // ------------------------------------------------------------------ gather

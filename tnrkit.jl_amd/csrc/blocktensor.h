@@ -202,7 +202,15 @@ struct TraceJob {
     const LegWeights* w_x1 = nullptr;
     const LegWeights* w_x2 = nullptr;
     double* norm_slot = nullptr;     // device
+    const double* divide_by = nullptr;   // device: divide by this instead of the own norm
 };
+
+// out = sum_i c_i * t_i over tensors of one layout (out may be one of them)
+struct LinJob {
+    BT* out = nullptr;
+    std::vector<std::pair<double, const BT*>> terms;   // at most four
+};
+void lincomb_many(Ctx& ctx, const std::vector<LinJob>& jobs);
 void trace_normalize_many(Ctx& ctx, const std::vector<TraceJob>& jobs);
 
 // Grouped strided matrix copies with optional row/column weights.

@@ -105,6 +105,61 @@ int tnr_create(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims, cons
     });
 }
 
+int tnr_create_multi(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims,
+                     const double* const* data, const size_t* ndata, int ntensors,
+                     int precision) {
+    return guarded([&] {
+        if (!out) return fail(TNR_ERR_ARG, "out is null");
+        *out = nullptr;
+        if (kind != TNR_IMPURITY_TRG && kind != TNR_IMPURITY_HOTRG)
+            return fail(TNR_ERR_ARG, "kind must be TNR_IMPURITY_TRG or TNR_IMPURITY_HOTRG");
+        if (zn < 1 || zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
+        if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
+            return fail(TNR_ERR_ARG, "unknown precision");
+        if (ntensors != scheme_tensors(kind))
+            return fail(TNR_ERR_ARG, "ImpurityTRG takes 5 tensors, ImpurityHOTRG 4");
+        if (!data || !ndata) return fail(TNR_ERR_ARG, "null tensor list");
+        std::unique_ptr<tnr_scheme> h(new tnr_scheme());
+        h->ctx.reset(new Ctx());
+        h->ctx->precision = precision;
+        std::vector<BT> Ts;
+        for (int i = 0; i < ntensors; ++i)
+            Ts.push_back(upload(*h->ctx, zn, leg_dims, data[i], ndata[i]));
+        scheme_init_multi(*h->ctx, h->s, kind, std::move(Ts));
+        *out = h.release();
+        return TNR_OK;
+    });
+}
+
+int tnr_ncomponents(int kind) {
+    const int c = scheme_components(kind);
+    if (!c) {
+        g_last_error = "unknown scheme kind";
+        return TNR_ERR_ARG;
+    }
+    return c;
+}
+
+int tnr_finalize_multi(tnr_scheme* s, double* out) {
+    return guarded([&] {
+        if (!s || !out) return fail(TNR_ERR_ARG, "null pointer");
+        finalize_host(*s->ctx, s->s, out);
+        check_norms(out, 1);
+        return TNR_OK;
+    });
+}
+
+int tnr_get_tensor_at(tnr_scheme* s, int index, double* data, size_t ndata) {
+    return guarded([&] {
+        if (!s || !data) return fail(TNR_ERR_ARG, "null pointer");
+        if (index < 0 || index > (int)s->s.imp.size()) return fail(TNR_ERR_ARG, "no such tensor");
+        const BT& t = index == 0 ? s->s.T : s->s.imp[index - 1];
+        if ((int64_t)ndata != t.L.size) return fail(TNR_ERR_ARG, "ndata does not match");
+        download(*s->ctx, t, data);
+        return TNR_OK;
+    });
+}
+
 int tnr_step(tnr_scheme* s, int chi) {
     return guarded([&] {
         if (!s) return fail(TNR_ERR_ARG, "null handle");
@@ -118,6 +173,8 @@ int tnr_step(tnr_scheme* s, int chi) {
 int tnr_finalize(tnr_scheme* s, double* norm) {
     return guarded([&] {
         if (!s || !norm) return fail(TNR_ERR_ARG, "null pointer");
+        if (scheme_components(s->s.kind) != 1)
+            return fail(TNR_ERR_STATE, "impurity schemes return several values: tnr_finalize_multi");
         double* slot = s->slots(1);
         finalize_many(*s->ctx, {&s->s}, {slot});
         s->ctx->fetch(norm, slot, sizeof(double));
@@ -133,8 +190,20 @@ int tnr_run(tnr_scheme* s, int chi, int maxiter, int finalize_beginning, double*
         if (chi < 1) return fail(TNR_ERR_ARG, "chi must be >= 1");
         const int steps = steps_of(maxiter);
         const int n = steps + (finalize_beginning ? 1 : 0);
-        double* slot = s->slots(n);
         Ctx& ctx = *s->ctx;
+        const int nc = scheme_components(s->s.kind);
+        if (nc > 1) {   // impurity schemes: each finalize! fetches its values
+            int w = 0;
+            if (finalize_beginning) finalize_host(ctx, s->s, norms + nc * w++);
+            for (int i = 0; i < steps; ++i) {
+                step_many(ctx, {&s->s}, {chi});
+                finalize_host(ctx, s->s, norms + nc * w++);
+            }
+            if (n_out) *n_out = n;
+            for (int i = 0; i < n; ++i) check_norms(norms + nc * i, 1);
+            return TNR_OK;
+        }
+        double* slot = s->slots(n);
         int w = 0;
         if (finalize_beginning) finalize_many(ctx, {&s->s}, {slot + w++});
         for (int i = 0; i < steps; ++i) {

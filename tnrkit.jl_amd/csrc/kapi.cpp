@@ -3,6 +3,7 @@
 #include <cstring>
 #include <string>
 
+#include "schemes.h"
 #include "solver.h"
 #include "../../include/tnr_kernels.h"
 
@@ -100,6 +101,27 @@ int tnr_kernel_svd(int m, int n, const double* M, int k, double* U, double* s, d
         }
         std::memcpy(s, out[0].s_host.data(), k * 8);
         if (iters) *iters = out[0].iters;
+        return TNR_OK;
+    });
+}
+
+int tnr_kernel_pivoted_gauge(int m, int k, const double* U, const double* s, double* Q) {
+    return kguard([&] {
+        TNR_REQUIRE(m > 0 && k > 0 && k <= std::min(m, LMAX) && U && s && Q, TNR_ERR_ARG,
+                    "bad gauge arguments");
+        Ctx ctx;
+        Buf dU = up(ctx, U, (size_t)m * k);
+        Buf dQ(&ctx.pool, (size_t)k * k * 8);
+        std::vector<int> cl;
+        int knn = 0;
+        const int nc = gauge_clusters(s, k, cl, knn);
+        if (cl.empty()) cl.push_back(0);
+        const int* dcl = static_cast<const int*>(
+            ctx.stage->put(ctx.stream, cl.data(), cl.size() * sizeof(int)));
+        std::vector<CanonDesc> cd{CanonDesc{dU.d(), dQ.d(), dcl, m, m, k, knn, nc}};
+        CanonDesc* dcd = ctx.stage->put(ctx.stream, cd);
+        launch_canon(ctx.stream, dcd, 1, k, kGaugeTie);
+        ctx.fetch(Q, dQ.d(), (size_t)k * k * 8);
         return TNR_OK;
     });
 }

@@ -74,7 +74,19 @@ struct TraceProb {
     int64_t size;
     double* partial;              // TRACE_PARTS partial sums
     double* norm_slot;
+    const double* divide_by;      // null: T /= |trace|; else T /= *divide_by (written earlier)
 };
+
+// out[e] = sum_i coef[i] * in[i][e], added left to right (in-place allowed)
+struct LinDesc {
+    double* out;
+    const double* in[4];
+    double coef[4];
+    int32_t nin;
+    int64_t n;
+    int64_t start;
+};
+void launch_lincomb(hipStream_t st, const LinDesc* d_descs, int nd, int64_t total);
 constexpr int TRACE_PARTS = 64;
 
 // Symmetric n x n eigenproblem (n <= LMAX).
@@ -94,6 +106,18 @@ struct ColDesc {
     double* sigma;
     int32_t nrow, ncol, ldy, ldu;
     const int* skip;              // optional
+};
+
+// Pivoted-row gauge of one truncated SVD problem (k_canon): Q (k x k,
+// column-major, ld k) rotates the kept left vectors U (m x k) inside each
+// cluster of equal singular values, cluster c spanning columns
+// [cl[c], cl[c+1]) with cl[0] = 0 and cl[nc] = knn; columns from knn on are
+// null and get Q = I.
+struct CanonDesc {
+    const double* U;
+    double* Q;
+    const int* cl;
+    int32_t m, ldu, k, knn, nc;
 };
 
 // Device-side stopping rule of the block iteration (one thread per problem):
@@ -154,6 +178,8 @@ void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax)
 void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rtol, double target,
                  int min_iters, int max_iters);
 void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd);
+// kmax: the largest k of the problems (sizes the LDS); tie: relative pivot tie
+void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, double tie);
 void launch_randfill(hipStream_t st, const FillDesc* d_descs, int nd, int64_t total);
 void launch_gather(hipStream_t st, const GatherDesc* d_descs, int nd, int64_t total);
 

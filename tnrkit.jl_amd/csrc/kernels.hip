@@ -531,10 +531,28 @@ __global__ __launch_bounds__(256) void k_normalize(const TraceProb* __restrict__
     double v = (threadIdx.x < TRACE_PARTS) ? P.partial[threadIdx.x] : 0.0;
     double s = fabs(block_sum_fixed(v, red));
     if (blockIdx.x == 0 && threadIdx.x == 0) *P.norm_slot = s;
-    const double inv = 1.0 / s;
+    const double inv = 1.0 / (P.divide_by ? *P.divide_by : s);
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < P.size;
          e += (int64_t)gridDim.x * blockDim.x)
         P.T[e] *= inv;
+}
+
+// ---------------------------------------------------- linear combinations
+__global__ __launch_bounds__(256) void k_lincomb(const LinDesc* __restrict__ descs, int nd,
+                                                 int64_t total) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nd - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (descs[mid].start <= e) lo = mid; else hi = mid - 1;
+        }
+        const LinDesc& D = descs[lo];
+        const int64_t r = e - D.start;
+        double v = D.coef[0] * D.in[0][r];
+        for (int i = 1; i < D.nin; ++i) v += D.coef[i] * D.in[i][r];
+        D.out[r] = v;
+    }
 }
 
 // ------------------------------------------------------------- Jacobi eigen
@@ -915,6 +933,119 @@ __global__ __launch_bounds__(1024) void k_colnorm(const ColDesc* __restrict__ de
     }
 }
 
+// ------------------------------------------------ pivoted-row SVD gauge
+// One 256-thread workgroup per SVD problem.  For each cluster of columns
+// [i0, i0 + d) of U (m x k) the new basis is u_t = U_c c_t with
+//   rn(r) = |U_c[r, :]|^2 - sum_{s<t} (U_c[r, :] . c_s)^2   (residual row norm)
+//   r_t   = first row with rn(r) >= (1 - tie) * max_r rn(r)
+//   c_t   = (U_c[r_t, :]^T - sum_{s<t} c_s (U_c[r_t, :] . c_s)) / sqrt(rn(r_t)),
+// i.e. the normalised projection of the coordinate vector e_{r_t} onto the
+// part of span(U_c) orthogonal to u_0..u_{t-1}; the c_t are the columns of
+// the cluster's diagonal block of Q (a lone vector only gets its sign,
+// exactly +-1).  Every column of Q is written once.
+constexpr int CNT = 256;
+
+__device__ __forceinline__ double canon_resid(const double* __restrict__ U, int ldu, int r, int i0,
+                                              int d, const double* Cq, int tt) {
+    double rn = 0.0;
+    for (int j = 0; j < d; ++j) {
+        const double x = U[r + (int64_t)(i0 + j) * ldu];
+        rn += x * x;
+    }
+    for (int s = 0; s < tt; ++s) {
+        double pj = 0.0;
+        for (int j = 0; j < d; ++j) pj += U[r + (int64_t)(i0 + j) * ldu] * Cq[j + s * d];
+        rn -= pj * pj;
+    }
+    return rn;
+}
+
+__global__ __launch_bounds__(CNT) void k_canon(const CanonDesc* __restrict__ descs, int nd,
+                                               double tie) {
+    extern __shared__ double lds[];
+    __shared__ double rv[CNT / 64];
+    __shared__ int ri[CNT / 64];
+    __shared__ double s_max, s_rn;
+    __shared__ int s_row;
+    const CanonDesc D = descs[blockIdx.x];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, k = D.k;
+    double* Cq = lds;                   // d x d, column-major, ld d
+    double* xr = Cq + (size_t)k * k;    // U_c[r_t, :]
+    double* pr = xr + k;                // U_c[r_t, :] . c_s
+    for (int c = 0; c < D.nc; ++c) {
+        const int i0 = D.cl[c], d = D.cl[c + 1] - i0;
+        for (int tt = 0; tt < d; ++tt) {
+            double best = -1.0;
+            for (int r = t; r < D.m; r += CNT)
+                best = fmax(best, canon_resid(D.U, D.ldu, r, i0, d, Cq, tt));
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) best = fmax(best, __shfl_xor(best, o, 64));
+            if (lane == 0) rv[w] = best;
+            __syncthreads();
+            if (t == 0) {
+                double m = rv[0];
+                for (int q = 1; q < CNT / 64; ++q) m = fmax(m, rv[q]);
+                s_max = m;
+            }
+            __syncthreads();
+            const double thr = (1.0 - tie) * s_max;
+            int first = 0x7fffffff;
+            for (int r = t; r < D.m; r += CNT)
+                if (canon_resid(D.U, D.ldu, r, i0, d, Cq, tt) >= thr) {
+                    first = r;
+                    break;
+                }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) first = min(first, __shfl_xor(first, o, 64));
+            if (lane == 0) ri[w] = first;
+            __syncthreads();
+            if (t == 0) {
+                int m = ri[0];
+                for (int q = 1; q < CNT / 64; ++q) m = min(m, ri[q]);
+                s_row = m;
+            }
+            __syncthreads();
+            const int r = s_row < D.m ? s_row : 0;     // NaN input: no row qualifies
+            for (int j = t; j < d; j += CNT) xr[j] = D.U[r + (int64_t)(i0 + j) * D.ldu];
+            __syncthreads();
+            for (int s = t; s < tt; s += CNT) {
+                double pj = 0.0;
+                for (int j = 0; j < d; ++j) pj += xr[j] * Cq[j + s * d];
+                pr[s] = pj;
+            }
+            __syncthreads();
+            if (t == 0) {
+                double rn = 0.0;
+                for (int j = 0; j < d; ++j) rn += xr[j] * xr[j];
+                for (int s = 0; s < tt; ++s) rn -= pr[s] * pr[s];
+                s_rn = rn;
+            }
+            __syncthreads();
+            const double inv = s_rn > 0.0 ? 1.0 / sqrt(s_rn) : 0.0;
+            if (d == 1) {                 // a lone vector: only its sign
+                if (t == 0) Cq[0] = xr[0] >= 0.0 ? 1.0 : -1.0;
+                __syncthreads();
+                break;
+            }
+            for (int j = t; j < d; j += CNT) {
+                double cj = xr[j];
+                for (int s = 0; s < tt; ++s) cj -= Cq[j + s * d] * pr[s];
+                Cq[j + tt * d] = cj * inv;
+            }
+            __syncthreads();
+        }
+        for (int e = t; e < k * d; e += CNT) {
+            const int jj = e / k, i = e - jj * k;
+            D.Q[i + (int64_t)(i0 + jj) * k] = (i >= i0 && i < i0 + d) ? Cq[(i - i0) + jj * d] : 0.0;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < k * (k - D.knn); e += CNT) {
+        const int jj = D.knn + e / k, i = e % k;
+        D.Q[i + (int64_t)jj * k] = (i == jj) ? 1.0 : 0.0;
+    }
+}
+
 // stopping rule of the block iteration, one thread per problem
 __global__ void k_conv(const ConvDesc* __restrict__ descs, int nd, double null_rtol,
                        double target, int min_iters, int max_iters) {
@@ -1234,10 +1365,31 @@ void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax,
     TNR_HIP(hipGetLastError());
 }
 
+void launch_lincomb(hipStream_t st, const LinDesc* d_descs, int nd, int64_t total) {
+    if (total <= 0 || nd <= 0) return;
+    ProfScope ps(st, PK_MISC, 0.0, 0.0);
+    hipLaunchKernelGGL(k_lincomb, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd,
+                       total);
+    TNR_HIP(hipGetLastError());
+}
+
 void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, double rtol) {
     if (nd <= 0) return;
     ProfScope ps(st, PK_COLNORM, 0.0, 0.0);
     hipLaunchKernelGGL(k_colnorm, dim3(nd), dim3(1024), 0, st, d_descs, nd, rtol);
+    TNR_HIP(hipGetLastError());
+}
+
+void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, double tie) {
+    if (nd <= 0) return;
+    TNR_REQUIRE(kmax >= 1 && kmax <= LMAX, TNR_ERR_INTERNAL, "gauge block above LMAX");
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void*)k_canon, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (int)(((size_t)LMAX * LMAX + 2 * (size_t)LMAX) * sizeof(double)));
+    TNR_HIP(attr);
+    ProfScope ps(st, PK_MISC, 0.0, 0.0);
+    const size_t shm = ((size_t)kmax * kmax + 2 * (size_t)kmax) * sizeof(double);
+    hipLaunchKernelGGL(k_canon, dim3(nd), dim3(CNT), shm, st, d_descs, nd, tie);
     TNR_HIP(hipGetLastError());
 }
 

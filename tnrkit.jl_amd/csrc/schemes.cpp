@@ -79,13 +79,28 @@ struct Factor {
 // singular values
 using Weigh = std::function<void(size_t t, const std::vector<double>&, std::vector<double>&,
                                   std::vector<double>&)>;
+using SvdWeigh = Weigh;
 
 // svd_trunc(T; trunc = truncrank(chi)) per coupled-charge block with the
 // chi largest values kept across blocks (stable, descending), then
 // U * diag(wu) and diag(wv) * V^T as block tensors.
+// Pivoted-row gauge (canonical = true; the impurity TRG needs it because
+// its impurity norm contracts factors of different SVDs): per block, kept
+// values within GAUGE_CLUSTER * s_max of their neighbour form a cluster,
+// values at or below GAUGE_NULL * s_max are null; inside each cluster the
+// basis is rebuilt by k_canon (the normalised projection of the pivot
+// coordinate vector onto what is left of the span, pivot = first row whose
+// residual norm is within GAUGE_TIE of the largest), then U Q and V Q are
+// formed with one grouped GEMM.  Result: a function of each cluster's span,
+// positive at every pivot row, never mixing coordinates a diagonal symmetry
+// separates (the hidden Z2 of a Trivial Ising tensor).
+constexpr double GAUGE_CLUSTER = 1e-9;
+constexpr double GAUGE_NULL = 1e-14;
+
+
 std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
                                    const std::vector<int>& chis, const std::vector<int>& tags,
-                                   const Weigh& weigh) {
+                                   const Weigh& weigh, bool canonical = false) {
     std::vector<SvdIn> in;
     std::vector<std::pair<int, int>> owner;
     for (size_t t = 0; t < Ts.size(); ++t) {
@@ -109,6 +124,14 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
     std::vector<GatherDesc> gd;
     std::vector<int64_t> gw;     // per descriptor: offset of its weights in wall
     std::vector<double> wall;    // all weight vectors, uploaded once
+    // pivoted-row gauge: descriptors, cluster bounds (offsets into allcl),
+    // rotation GEMMs and the buffers they write
+    std::vector<CanonDesc> cd;
+    std::vector<size_t> cl_off;
+    std::vector<int> allcl;
+    std::vector<GemmJob> rot;
+    std::vector<Buf> keep_alive;
+    int canon_kmax = 0;
     size_t p0 = 0;
     for (size_t t = 0; t < Ts.size(); ++t) {
         const Layout& L = Ts[t]->L;
@@ -167,21 +190,50 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
             TNR_REQUIRE(f.U.L.R[c] == in[p].m && f.U.L.C[c] == kc[p] && f.V.L.R[c] == kc[p] &&
                             f.V.L.C[c] == in[p].n,
                         TNR_ERR_INTERNAL, "factor layout mismatch");
+            const double* usrc = out[p].U.d();
+            const double* vsrc = out[p].V.d();
+            int ldus = out[p].ldu, ldvs = out[p].ldv;
+            if (canonical) {
+                const int k = kc[p], m = in[p].m, n = in[p].n;
+                int knn = 0;
+                cl_off.push_back(allcl.size());
+                const int nc = gauge_clusters(out[p].s_host.data(), k, allcl, knn);
+                Buf Q(&ctx.pool, (size_t)k * k * 8), Ur(&ctx.pool, (size_t)m * k * 8),
+                    Vr(&ctx.pool, (size_t)n * k * 8);
+                cd.push_back(CanonDesc{usrc, Q.d(), nullptr, m, ldus, k, knn, nc});
+                rot.push_back(GemmJob{usrc, Q.d(), Ur.d(), m, k, k, ldus, k, m, 0, 0});
+                rot.push_back(GemmJob{vsrc, Q.d(), Vr.d(), n, k, k, ldvs, k, n, 0, 0});
+                canon_kmax = std::max(canon_kmax, k);
+                usrc = Ur.d(); ldus = m;
+                vsrc = Vr.d(); ldvs = n;
+                keep_alive.push_back(std::move(Q));
+                keep_alive.push_back(std::move(Ur));
+                keep_alive.push_back(std::move(Vr));
+            }
             // U * diag(wu): column weights; diag(wv) * V^T: row weights
             GatherDesc gu{};
-            gu.in = out[p].U.d(); gu.out = f.U.d() + f.U.L.off[c];
-            gu.rows = in[p].m; gu.cols = kc[p]; gu.ldi = out[p].ldu; gu.ldo = in[p].m;
+            gu.in = usrc; gu.out = f.U.d() + f.U.L.off[c];
+            gu.rows = in[p].m; gu.cols = kc[p]; gu.ldi = ldus; gu.ldo = in[p].m;
             gd.push_back(gu);
             gw.push_back((int64_t)(owu + off[c]));
             GatherDesc gv{};
-            gv.in = out[p].V.d(); gv.out = f.V.d() + f.V.L.off[c];
-            gv.rows = kc[p]; gv.cols = in[p].n; gv.ldi = out[p].ldv; gv.ldo = kc[p]; gv.trans = 1;
+            gv.in = vsrc; gv.out = f.V.d() + f.V.L.off[c];
+            gv.rows = kc[p]; gv.cols = in[p].n; gv.ldi = ldvs; gv.ldo = kc[p]; gv.trans = 1;
             gd.push_back(gv);
             gw.push_back((int64_t)(owv + off[c]));
         }
         p0 = p1;
     }
     if (gd.empty()) return F;
+    if (!cd.empty()) {
+        if (allcl.empty()) allcl.push_back(0);
+        const int* dcl = static_cast<const int*>(
+            ctx.stage->put(ctx.stream, allcl.data(), std::max<size_t>(1, allcl.size()) * sizeof(int)));
+        for (size_t i = 0; i < cd.size(); ++i) cd[i].cl = dcl + cl_off[i];
+        CanonDesc* dcd = ctx.stage->put(ctx.stream, cd);
+        launch_canon(ctx.stream, dcd, (int)cd.size(), canon_kmax, kGaugeTie);
+        gemm_grouped(ctx, rot);
+    }
     const double* dw = static_cast<const double*>(
         ctx.stage->put(ctx.stream, wall.data(), std::max<size_t>(1, wall.size()) * sizeof(double)));
     for (size_t i = 0; i < gd.size(); ++i) {
@@ -317,6 +369,33 @@ void log_factor(Scheme& s, double kept, double dropped, int iters, double conv) 
 }
 
 // ------------------------------------------------------------------- TRG
+// T[-1 -2; -3 -4] := D[-2; 1 2] * B[-1; 4 1] * C[4 3; -3] * A[3 2; -4]
+// (src/schemes/trg.jl:39, impuritytrg.jl:84-88) for every (A, B, C, D)
+struct TrgNet { const BT *A, *B, *C, *D; };
+
+std::vector<BT> trg_networks(Ctx& ctx, const std::vector<TrgNet>& nets) {
+    std::vector<ContractJob> cj;
+    for (const TrgNet& t : nets) {
+        cj.push_back(ContractJob{{t.B, {-1, -4, 1}}, {t.D, {-2, 1, -3}}, {-1, -2, -4, -3}, 2});
+        cj.push_back(ContractJob{{t.C, {-1, 1, -3}}, {t.A, {1, -2, -4}}, {-1, -2, -3, -4}, 2});
+    }
+    auto XY = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < nets.size(); ++i)
+        cj.push_back(ContractJob{{&XY[2 * i], {-1, -2, 1, 2}}, {&XY[2 * i + 1], {1, 2, -3, -4}},
+                                 {-1, -2, -3, -4}, 2});
+    return contract_many(ctx, cj);
+}
+
+SvdWeigh sqrt_weights() {
+    return [](size_t, const std::vector<double>& s, std::vector<double>& wu,
+              std::vector<double>& wv) {
+        wu.resize(s.size());
+        for (size_t i = 0; i < s.size(); ++i) wu[i] = std::sqrt(s[i]);
+        wv = wu;
+    };
+}
+
 void step_trg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
     const size_t n = G.size();
     auto sqrtw = [](size_t, const std::vector<double>& s, std::vector<double>& wu,
@@ -338,20 +417,9 @@ void step_trg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& c
     auto F = svd_trunc_many(ctx, Ts, ch, tags, sqrtw);
     std::vector<Factor> F1(std::make_move_iterator(F.begin()), std::make_move_iterator(F.begin() + n));
     std::vector<Factor> F2(std::make_move_iterator(F.begin() + n), std::make_move_iterator(F.end()));
-    // T[-1 -2; -3 -4] := D[-2; 1 2] * B[-1; 4 1] * C[4 3; -3] * A[3 2; -4]
-    std::vector<ContractJob> cj;
-    for (size_t i = 0; i < n; ++i) {
-        const BT& A = F1[i].U; const BT& B = F1[i].V;
-        const BT& C = F2[i].U; const BT& D = F2[i].V;
-        cj.push_back(ContractJob{{&B, {-1, -4, 1}}, {&D, {-2, 1, -3}}, {-1, -2, -4, -3}, 2});
-        cj.push_back(ContractJob{{&C, {-1, 1, -3}}, {&A, {1, -2, -4}}, {-1, -2, -3, -4}, 2});
-    }
-    auto XY = contract_many(ctx, cj);
-    cj.clear();
-    for (size_t i = 0; i < n; ++i)
-        cj.push_back(ContractJob{{&XY[2 * i], {-1, -2, 1, 2}}, {&XY[2 * i + 1], {1, 2, -3, -4}},
-                                 {-1, -2, -3, -4}, 2});
-    auto Tn = contract_many(ctx, cj);
+    std::vector<TrgNet> nets;
+    for (size_t i = 0; i < n; ++i) nets.push_back(TrgNet{&F1[i].U, &F1[i].V, &F2[i].U, &F2[i].V});
+    auto Tn = trg_networks(ctx, nets);
     for (size_t i = 0; i < n; ++i) {
         G[i]->tlog.clear();
         log_factor(*G[i], F1[i].kept_min, F1[i].dropped_max, F1[i].iters, F1[i].conv);
@@ -496,51 +564,165 @@ std::vector<BT> hotrg_proj(Ctx& ctx, const std::vector<Scheme*>& G, const std::v
     return U;
 }
 
+// _step_hotrg_y(A1, A2, U) (hotrg.jl:42-60) or _step_hotrg_x(A1, A2, U)
+// (hotrg.jl:62-80) for every (A1, A2, U), as three grouped contractions
+struct HotrgApply { const BT *A1, *A2, *U; };
+
+std::vector<BT> hotrg_apply(Ctx& ctx, const std::vector<HotrgApply>& jobs, bool ydir) {
+    std::vector<ContractJob> cj;
+    for (const HotrgApply& j : jobs) {
+        if (ydir)   // P[m1, m3, x3 | x2, x5] = conj(U[1 2; -1]) A2[1 5; -3 3]
+            cj.push_back(ContractJob{{j.U, {1, -4, -1}, true}, {j.A2, {1, -5, -2, -3}},
+                                     {-1, -2, -3, -4, -5}, 3});
+        else        // R[m1, x3, m2 | x2, x5] = A1[-1 1; 3 5] conj(U[1 2; -2])
+            cj.push_back(ContractJob{{j.A1, {-1, 1, -2, -5}}, {j.U, {1, -4, -3}, true},
+                                     {-1, -2, -3, -4, -5}, 3});
+    }
+    auto P = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        if (ydir)   // Q[m1, m2, m3 | x3, x4] = P A1[2 -2; 5 4]
+            cj.push_back(ContractJob{{&P[i], {-1, -3, -4, 2, 5}}, {jobs[i].A1, {2, -2, 5, -5}},
+                                     {-1, -2, -3, -4, -5}, 3});
+        else        // S[m1, m2, m4 | x3, x4] = R A2[5 2; 4 -4]
+            cj.push_back(ContractJob{{&P[i], {-1, -4, -2, 2, 5}}, {jobs[i].A2, {5, 2, -5, -3}},
+                                     {-1, -2, -3, -4, -5}, 3});
+    }
+    auto Q = contract_many(ctx, cj);
+    P.clear();
+    cj.clear();
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        if (ydir)
+            cj.push_back(ContractJob{{&Q[i], {-1, -2, -3, 3, 4}}, {jobs[i].U, {3, 4, -4}},
+                                     {-1, -2, -3, -4}, 2});
+        else
+            cj.push_back(ContractJob{{&Q[i], {-1, -2, -4, 3, 4}}, {jobs[i].U, {3, 4, -3}},
+                                     {-1, -2, -3, -4}, 2});
+    }
+    return contract_many(ctx, cj);
+}
+
+// ------------------------------------------------------------ ImpurityTRG
+// src/schemes/impuritytrg.jl:63-91: six SVD12 factorizations per scheme
+// (pure T and its transpose, T_imp1, transpose(T_imp2), T_imp3,
+// transpose(T_imp4)) in one grouped solve, then five TRG networks
+void step_impurity_trg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
+    const size_t n = G.size();
+    std::vector<PermJob> pj;
+    for (auto* s : G) {
+        pj.push_back(PermJob{&s->T, {1, 3, 0, 2}, 2, false, {}});
+        pj.push_back(PermJob{&s->imp[1], {1, 3, 0, 2}, 2, false, {}});
+        pj.push_back(PermJob{&s->imp[3], {1, 3, 0, 2}, 2, false, {}});
+    }
+    auto P = permute_many(ctx, pj);
+    std::vector<const BT*> Ts;
+    std::vector<int> ch, tags;
+    for (size_t i = 0; i < n; ++i) {
+        const BT* src[6] = {&G[i]->T, &P[3 * i], &G[i]->imp[0], &P[3 * i + 1], &G[i]->imp[2],
+                            &P[3 * i + 2]};
+        for (int f = 0; f < 6; ++f) {
+            Ts.push_back(src[f]);
+            ch.push_back(chis[i]);
+            tags.push_back(40 + f);
+        }
+    }
+    auto F = svd_trunc_many(ctx, Ts, ch, tags, sqrt_weights(), /*canonical=*/true);
+    std::vector<TrgNet> nets;
+    for (size_t i = 0; i < n; ++i) {
+        const Factor* f = &F[6 * i];
+        const BT *Ap = &f[0].U, *Bp = &f[0].V, *Cp = &f[1].U, *Dp = &f[1].V;
+        const BT *A1 = &f[2].U, *B1 = &f[2].V, *C2 = &f[3].U, *D2 = &f[3].V;
+        const BT *A3 = &f[4].U, *B3 = &f[4].V, *C4 = &f[5].U, *D4 = &f[5].V;
+        nets.push_back(TrgNet{Ap, Bp, Cp, Dp});   // pure
+        nets.push_back(TrgNet{Ap, B1, C2, Dp});   // T_imp1
+        nets.push_back(TrgNet{Ap, B3, Cp, D2});   // T_imp2
+        nets.push_back(TrgNet{A3, Bp, Cp, D4});   // T_imp3
+        nets.push_back(TrgNet{A1, Bp, C4, Dp});   // T_imp4
+    }
+    auto out = trg_networks(ctx, nets);
+    for (size_t i = 0; i < n; ++i) {
+        G[i]->tlog.clear();
+        for (int f = 0; f < 6; ++f) {
+            const Factor& x = F[6 * i + f];
+            log_factor(*G[i], x.kept_min, x.dropped_max, x.iters, x.conv);
+        }
+        G[i]->T = std::move(out[5 * i]);
+        for (int k = 0; k < 4; ++k) G[i]->imp[k] = std::move(out[5 * i + 1 + k]);
+    }
+}
+
+// ---------------------------------------------------------- ImpurityHOTRG
+// src/schemes/impurityhotrg.jl:48-84: projector from (T, T), then nine
+// placements of the tensors through it, averaged:
+//   T' = X(T, T), T11' = (X(T11, T) + X(T, T11)) / 2, likewise T12',
+//   T2' = (X(T2, T) + X(T, T2) + X(T11, T12) + X(T12, T11)) / 4
+// (scaling by 1/2, 1/4 is exact, so summing the scaled terms left to right
+// gives the reference's bits); first along x (y projector), then along y.
+void impurity_hotrg_half(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis,
+                         bool ydir) {
+    const size_t n = G.size();
+    auto U = hotrg_proj(ctx, G, chis, /*xdir=*/ydir, ydir ? 51 : 52);
+    std::vector<HotrgApply> jobs;
+    for (size_t i = 0; i < n; ++i) {
+        const BT *T = &G[i]->T, *T11 = &G[i]->imp[0], *T12 = &G[i]->imp[1], *T2 = &G[i]->imp[2];
+        const BT* u = &U[i];
+        const BT* pairs[9][2] = {{T, T},    {T11, T}, {T, T11}, {T12, T},  {T, T12},
+                                 {T2, T},   {T, T2},  {T11, T12}, {T12, T11}};
+        for (auto& p : pairs) jobs.push_back(HotrgApply{p[0], p[1], u});
+    }
+    auto R = hotrg_apply(ctx, jobs, ydir);
+    std::vector<LinJob> lj;
+    for (size_t i = 0; i < n; ++i) {
+        BT* r = &R[9 * i];
+        lj.push_back(LinJob{&r[1], {{0.5, &r[1]}, {0.5, &r[2]}}});
+        lj.push_back(LinJob{&r[3], {{0.5, &r[3]}, {0.5, &r[4]}}});
+        lj.push_back(LinJob{&r[5], {{0.25, &r[5]}, {0.25, &r[6]}, {0.25, &r[7]}, {0.25, &r[8]}}});
+    }
+    lincomb_many(ctx, lj);
+    for (size_t i = 0; i < n; ++i) {
+        G[i]->T = std::move(R[9 * i]);
+        G[i]->imp[0] = std::move(R[9 * i + 1]);
+        G[i]->imp[1] = std::move(R[9 * i + 3]);
+        G[i]->imp[2] = std::move(R[9 * i + 5]);
+    }
+}
+
+// src/schemes/hotrg.jl:154-160: x projector, y step, y projector, x step
 void step_hotrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
     const size_t n = G.size();
     for (auto* s : G) s->tlog.clear();
     auto Ux = hotrg_proj(ctx, G, chis, true, 31);
-    // _step_hotrg_y: T[-1 -2; -3 -4] := conj(Ux[1 2; -1]) Ux[3 4; -4] A2[1 5; -3 3] A1[2 -2; 5 4]
-    std::vector<ContractJob> cj;
-    for (size_t i = 0; i < n; ++i)   // P[m1, m3, x3 | x2, x5]
-        cj.push_back(ContractJob{{&Ux[i], {1, -4, -1}, true}, {&G[i]->T, {1, -5, -2, -3}},
-                                 {-1, -2, -3, -4, -5}, 3});
-    auto P = contract_many(ctx, cj);
-    cj.clear();
-    for (size_t i = 0; i < n; ++i)   // Q[m1, m2, m3 | x3, x4]
-        cj.push_back(ContractJob{{&P[i], {-1, -3, -4, 2, 5}}, {&G[i]->T, {2, -2, 5, -5}},
-                                 {-1, -2, -3, -4, -5}, 3});
-    auto Q = contract_many(ctx, cj);
-    P.clear();
-    cj.clear();
-    for (size_t i = 0; i < n; ++i)   // T[m1, m2 | m3, m4]
-        cj.push_back(ContractJob{{&Q[i], {-1, -2, -3, 3, 4}}, {&Ux[i], {3, 4, -4}},
-                                 {-1, -2, -3, -4}, 2});
-    auto Ty = contract_many(ctx, cj);
-    Q.clear();
+    std::vector<HotrgApply> jobs;
+    for (size_t i = 0; i < n; ++i) jobs.push_back(HotrgApply{&G[i]->T, &G[i]->T, &Ux[i]});
+    auto Ty = hotrg_apply(ctx, jobs, /*ydir=*/true);
     for (size_t i = 0; i < n; ++i) G[i]->T = std::move(Ty[i]);
     auto Uy = hotrg_proj(ctx, G, chis, false, 32);
-    // _step_hotrg_x: T[-1 -2; -3 -4] := A1[-1 1; 3 5] A2[5 2; 4 -4] conj(Uy[1 2; -2]) Uy[3 4; -3]
-    cj.clear();
-    for (size_t i = 0; i < n; ++i)   // R[m1, x3, m2 | x2, x5]
-        cj.push_back(ContractJob{{&G[i]->T, {-1, 1, -2, -5}}, {&Uy[i], {1, -4, -3}, true},
-                                 {-1, -2, -3, -4, -5}, 3});
-    auto R = contract_many(ctx, cj);
-    cj.clear();
-    for (size_t i = 0; i < n; ++i)   // S[m1, m2, m4 | x3, x4]
-        cj.push_back(ContractJob{{&R[i], {-1, -4, -2, 2, 5}}, {&G[i]->T, {5, 2, -5, -3}},
-                                 {-1, -2, -3, -4, -5}, 3});
-    auto S = contract_many(ctx, cj);
-    R.clear();
-    cj.clear();
-    for (size_t i = 0; i < n; ++i)   // T[m1, m2 | m3, m4]
-        cj.push_back(ContractJob{{&S[i], {-1, -2, -4, 3, 4}}, {&Uy[i], {3, 4, -3}},
-                                 {-1, -2, -3, -4}, 2});
-    auto Tx = contract_many(ctx, cj);
+    jobs.clear();
+    for (size_t i = 0; i < n; ++i) jobs.push_back(HotrgApply{&G[i]->T, &G[i]->T, &Uy[i]});
+    auto Tx = hotrg_apply(ctx, jobs, /*ydir=*/false);
     for (size_t i = 0; i < n; ++i) G[i]->T = std::move(Tx[i]);
 }
 
+void step_impurity_hotrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
+    for (auto* s : G) s->tlog.clear();
+    impurity_hotrg_half(ctx, G, chis, /*ydir=*/false);   // _get_hotrg_yproj + _step_hotrg_x
+    impurity_hotrg_half(ctx, G, chis, /*ydir=*/true);    // _get_hotrg_xproj + _step_hotrg_y
+}
+
 }  // namespace
+
+int gauge_clusters(const double* sv, int k, std::vector<int>& cl, int& knn) {
+    knn = 0;
+    while (knn < k && sv[knn] > GAUGE_NULL * sv[0]) ++knn;
+    if (knn == 0) return 0;
+    const size_t b = cl.size();
+    cl.push_back(0);
+    for (int i = 1; i < knn; ++i)
+        if (sv[i - 1] - sv[i] > GAUGE_CLUSTER * sv[0]) cl.push_back(i);
+    const int nc = (int)(cl.size() - b);
+    cl.push_back(knn);
+    return nc;
+}
 
 // ------------------------------------------------------------------ public
 void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T) {
@@ -557,6 +739,41 @@ void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T) {
     }
 }
 
+int scheme_tensors(int kind) {
+    switch (kind) {
+        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: return 1;
+        case TNR_IMPURITY_TRG: return 5;
+        case TNR_IMPURITY_HOTRG: return 4;
+        default: return 0;
+    }
+}
+
+int scheme_components(int kind) {
+    switch (kind) {
+        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: return 1;
+        case TNR_IMPURITY_TRG: return 2;
+        case TNR_IMPURITY_HOTRG: return 4;
+        default: return 0;
+    }
+}
+
+void scheme_init_multi(Ctx& ctx, Scheme& s, int kind, std::vector<BT>&& Ts) {
+    TNR_REQUIRE(kind == TNR_IMPURITY_TRG || kind == TNR_IMPURITY_HOTRG, TNR_ERR_ARG,
+                "unknown impurity scheme kind");
+    TNR_REQUIRE((int)Ts.size() == scheme_tensors(kind), TNR_ERR_ARG,
+                "wrong number of tensors for this scheme");
+    // impuritytrg.jl:52-56, impurityhotrg.jl:40-44: every tensor on T's spaces
+    for (size_t i = 1; i < Ts.size(); ++i)
+        TNR_REQUIRE(Ts[i].L.same(Ts[0].L), TNR_ERR_ARG,
+                    "the impurity tensors must carry the spaces of T");
+    s.kind = kind;
+    s.k = 0.0;
+    s.steps = 0;
+    s.imp.clear();
+    for (size_t i = 1; i < Ts.size(); ++i) s.imp.push_back(std::move(Ts[i]));
+    scheme_set_tensor(ctx, s, std::move(Ts[0]));
+}
+
 void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k) {
     TNR_REQUIRE(kind == TNR_TRG || kind == TNR_BTRG || kind == TNR_HOTRG, TNR_ERR_ARG,
                 "unknown scheme kind");
@@ -567,7 +784,7 @@ void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k) {
 }
 
 void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<int>& chis) {
-    for (int kind : {TNR_TRG, TNR_BTRG, TNR_HOTRG}) {
+    for (int kind : {TNR_TRG, TNR_BTRG, TNR_HOTRG, TNR_IMPURITY_TRG, TNR_IMPURITY_HOTRG}) {
         std::vector<Scheme*> g;
         std::vector<int> c;
         for (size_t i = 0; i < group.size(); ++i)
@@ -578,7 +795,9 @@ void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<in
         if (g.empty()) continue;
         if (kind == TNR_TRG) step_trg(ctx, g, c);
         else if (kind == TNR_BTRG) step_btrg(ctx, g, c);
-        else step_hotrg(ctx, g, c);
+        else if (kind == TNR_HOTRG) step_hotrg(ctx, g, c);
+        else if (kind == TNR_IMPURITY_TRG) step_impurity_trg(ctx, g, c);
+        else step_impurity_hotrg(ctx, g, c);
         for (auto* s : g) s->steps++;
     }
 }
@@ -599,6 +818,46 @@ void finalize_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vecto
         }
     }
     trace_normalize_many(ctx, tj);
+}
+
+void finalize_host(Ctx& ctx, Scheme& s, double* out) {
+    Buf slots(&ctx.pool, 8 * sizeof(double));
+    double* sl = slots.d();
+    if (s.kind == TNR_TRG || s.kind == TNR_BTRG || s.kind == TNR_HOTRG) {
+        finalize_many(ctx, {&s}, {sl});
+        ctx.fetch(out, sl, sizeof(double));
+        return;
+    }
+    // npure = |tr T[1 2; 2 1]|, T /= npure; then every impurity tensor's own
+    // trace (ImpurityHOTRG reports them) and division by npure
+    trace_normalize_many(ctx, {TraceJob{&s.T, nullptr, nullptr, sl, nullptr}});
+    std::vector<TraceJob> tj;
+    for (size_t i = 0; i < s.imp.size(); ++i)
+        tj.push_back(TraceJob{&s.imp[i], nullptr, nullptr, sl + 1 + i, sl});
+    trace_normalize_many(ctx, tj);
+    if (s.kind == TNR_IMPURITY_HOTRG) {   // finalize.jl:90-101
+        ctx.fetch(out, sl, 4 * sizeof(double));
+        return;
+    }
+    // ImpurityTRG (finalize.jl:74-88): nimp = |T1[5 4; 6 1] T2[1 2; 7 5]
+    // T3[3 7; 2 8] T4[8 6; 4 3]|, as P12[4,6,2,7] and P34[7,2,6,4], then
+    // M[4, 4'] = P12 . P34 over (6, 2, 7) and the trace of M
+    std::vector<ContractJob> cj{
+        ContractJob{{&s.imp[0], {5, -1, -2, 1}}, {&s.imp[1], {1, -3, -4, 5}}, {-1, -2, -3, -4}, 2},
+        ContractJob{{&s.imp[2], {3, -1, -2, 8}}, {&s.imp[3], {8, -3, -4, 3}}, {-1, -2, -3, -4}, 2}};
+    auto PP = contract_many(ctx, cj);
+    cj = {ContractJob{{&PP[0], {-1, 1, 2, 3}}, {&PP[1], {3, 2, 1, -2}}, {-1, -2}, 1}};
+    auto M = contract_many(ctx, cj);
+    const Layout& L = M[0].L;
+    std::vector<double> h((size_t)std::max<int64_t>(L.size, 1));
+    ctx.fetch(h.data(), M[0].d(), (size_t)L.size * sizeof(double));
+    double tr = 0.0;
+    for (int c = 0; c < L.N; ++c) {
+        TNR_REQUIRE(L.R[c] == L.C[c], TNR_ERR_INTERNAL, "plaquette trace of a non-square block");
+        for (int64_t i = 0; i < L.R[c]; ++i) tr += h[(size_t)(L.off[c] + i + i * L.R[c])];
+    }
+    ctx.fetch(out, sl, sizeof(double));
+    out[1] = std::fabs(tr);
 }
 
 double free_energy(const double* norms, int n, double beta, double scalefactor,

@@ -19,6 +19,7 @@ struct Scheme {
     double k = -0.5;           // BTRG bond-weight exponent
     BT T;                      // current tensor, (12|34) layout
     BondVec S1, S2;            // BTRG bond weights (vertical, horizontal)
+    std::vector<BT> imp;       // ImpurityTRG: T_imp1..4; ImpurityHOTRG: T_imp_order1_1, _1_2, _2
     int steps = 0;
     // per factorization of the last step: smallest kept value, largest
     // dropped value, solver iterations, convergence bound
@@ -26,6 +27,12 @@ struct Scheme {
 };
 
 void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k);
+// impurity schemes: Ts = {T, impurity tensors...}, all on T's spaces
+void scheme_init_multi(Ctx& ctx, Scheme& s, int kind, std::vector<BT>&& Ts);
+int scheme_tensors(int kind);      // tensors a scheme holds (0: unknown kind)
+int scheme_components(int kind);   // values one finalize! returns
+// finalize! of one scheme with its values on the host (scheme_components)
+void finalize_host(Ctx& ctx, Scheme& s, double* out);
 void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T);
 
 // step!(scheme, truncrank(chi)) for every scheme of the group
@@ -37,6 +44,13 @@ void finalize_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vecto
 // free_energy(data, beta; scalefactor, initial_size)
 double free_energy(const double* norms, int n, double beta, double scalefactor,
                    double initial_size);
+
+// Pivoted-row gauge of a truncated SVD (used by the impurity TRG): for k
+// kept singular values sv (descending) appends the cluster bounds
+// 0 = b_0 < ... < b_nc = knn to cl and returns nc; knn = values above the
+// null threshold.  kGaugeTie is k_canon's relative pivot tie.
+int gauge_clusters(const double* sv, int k, std::vector<int>& cl, int& knn);
+constexpr double kGaugeTie = 1e-10;
 
 // HBM estimate (bytes) of a run at bond dimension chi, for admission
 size_t scheme_hbm_estimate(int kind, int N, int chi);

@@ -28,28 +28,53 @@ def _bond(beta, exact_betac):
         e, ei = b.exp(), (-b).exp()
         x, y = (e + ei) / 2, (e - ei) / 2
         return float(x.sqrt()), float(y.sqrt()), float(x), float(y)
-    x, y = math.cosh(beta), math.sinh(beta)
-    return math.sqrt(x), math.sqrt(y), x, y
+    x, y = float(np.cosh(beta)), float(np.sinh(beta))
+    return float(np.sqrt(x)), float(np.sqrt(y)), x, y
 
 
-def classical_ising(beta=None, symmetry="Z2"):
-    """2D Ising partition-function tensor at h = 0.
+def _field_factors(b, exact, h):
+    """cosh(h b), sinh(h b); at the exact beta_c evaluated at high precision
+    first, as the reference does in BigFloat."""
+    if h == 0.0:
+        return 1.0, 0.0
+    if exact:
+        bd = (Decimal(1) + Decimal(2).sqrt()).ln() / 2 * Decimal(float(h))
+        e, ei = bd.exp(), (-bd).exp()
+        return float((e + ei) / 2), float((e - ei) / 2)
+    return float(np.cosh(h * b)), float(np.sinh(h * b))
 
-    ``symmetry`` "Trivial" follows src/models/ising.jl:44-55, "Z2" follows
-    :56-67.  ``beta=None`` means the reference's beta_c."""
+
+def _ising_dense(beta, h, impurity):
     exact = beta is None
     b = ising_betac if exact else float(beta)
     sx, sy, x, y = _bond(b, exact)
+    ch, sh = _field_factors(b, exact, float(h))
+    even, odd = (sh, ch) if impurity else (ch, sh)
+    w = np.array([sx, sy])
+    T = np.zeros((2, 2, 2, 2))
+    for i in range(2):
+        for j in range(2):
+            for k in range(2):
+                for l in range(2):
+                    init = even if (i + j + k + l) % 2 == 0 else odd
+                    T[i, j, k, l] = 2.0 * init * w[i] * w[j] * w[k] * w[l]
+    return trivial(T)
+
+
+def classical_ising(beta=None, symmetry="Z2", h=0.0):
+    """2D Ising partition-function tensor.
+
+    ``symmetry`` "Trivial" follows src/models/ising.jl:44-55 (with field
+    ``h``: cosh(h b) on even index sums, sinh(h b) on odd ones), "Z2" follows
+    :56-67 (h = 0 only, ising.jl:57).  ``beta=None`` means the reference's
+    beta_c."""
     if symmetry == "Trivial":
-        w = np.array([sx, sy])
-        T = np.zeros((2, 2, 2, 2))
-        for i in range(2):
-            for j in range(2):
-                for k in range(2):
-                    for l in range(2):
-                        if (i + j + k + l) % 2 == 0:
-                            T[i, j, k, l] = 2.0 * w[i] * w[j] * w[k] * w[l]
-        return trivial(T)
+        return _ising_dense(beta, h, impurity=False)
+    if h != 0.0:
+        raise ValueError("an external field is not compatible with Z2 symmetry (ising.jl:57)")
+    exact = beta is None
+    b = ising_betac if exact else float(beta)
+    sx, sy, x, y = _bond(b, exact)
     if symmetry == "Z2":
         if exact:
             bd = (Decimal(1) + Decimal(2).sqrt()).ln() / 2
@@ -62,6 +87,12 @@ def classical_ising(beta=None, symmetry="Z2"):
         data = np.array([xx, xy, xy, yy, xy, xy, xy, xy])
         return SymTensor(2, np.ones((4, 2), dtype=np.int32), data)
     raise ValueError("symmetry must be 'Trivial' or 'Z2'")
+
+
+def classical_ising_impurity(beta=None, h=0.0):
+    """src/models/ising.jl:70-102: the magnetisation impurity (Trivial only),
+    the Ising tensor with the parities of init swapped."""
+    return _ising_dense(beta, h, impurity=True)
 
 
 def potts_betac(q):

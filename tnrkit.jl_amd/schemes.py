@@ -181,6 +181,59 @@ class HOTRG(TNRScheme):
         super().__init__(T, 0.0, precision)
 
 
+class _MultiScheme(TNRScheme):
+    """A scheme holding the pure tensor and impurity tensors on its spaces."""
+    ntensors = 0
+
+    def __init__(self, *tensors, precision=FP64):
+        if len(tensors) != self.ntensors or not all(isinstance(t, SymTensor) for t in tensors):
+            raise TypeError(f"{type(self).__name__} takes {self.ntensors} SymTensors")
+        T = tensors[0]
+        self._T0 = T
+        self.k = 0.0
+        self.precision = int(precision)
+        self._h = C.c_void_p()
+        dims = np.ascontiguousarray(T.leg_dims, dtype=np.int32).ravel()
+        datas = [np.ascontiguousarray(t.data, dtype=np.float64) for t in tensors]
+        ptrs = (C.POINTER(C.c_double) * len(datas))(
+            *[d.ctypes.data_as(C.POINTER(C.c_double)) for d in datas])
+        sizes = (C.c_size_t * len(datas))(*[d.size for d in datas])
+        check(lib().tnr_create_multi(C.byref(self._h), self.kind, T.N,
+                                     dims.ctypes.data_as(C.POINTER(C.c_int32)), ptrs, sizes,
+                                     len(datas), self.precision))
+
+    def tensor(self, index):
+        """Tensor ``index`` (0 = T, then the impurity tensors in constructor order)."""
+        N = self._T0.N
+        dims = np.zeros(4 * N, dtype=np.int32)
+        n = C.c_size_t()
+        check(lib().tnr_get_dims(self._h, dims.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n)))
+        data = np.zeros(n.value)
+        check(lib().tnr_get_tensor_at(self._h, int(index), data.ctypes.data_as(C.POINTER(C.c_double)),
+                                      n.value))
+        return SymTensor(N, dims.reshape(4, N), data)
+
+
+class ImpurityTRG(_MultiScheme):
+    """src/schemes/impuritytrg.jl:40-103: ImpurityTRG(T, T_imp1, T_imp2,
+    T_imp3, T_imp4); finalize! returns (npure, nimp) (finalize.jl:74-88)."""
+    kind = _lib.TNR_IMPURITY_TRG
+    ntensors = 5
+
+
+class ImpurityHOTRG(_MultiScheme):
+    """src/schemes/impurityhotrg.jl:27-95: ImpurityHOTRG(T, T_imp_order1_1,
+    T_imp_order1_2, T_imp_order2); finalize! returns (n, n_11, n_12, n_2)
+    (finalize.jl:90-101); each step rescales the area by 4."""
+    kind = _lib.TNR_IMPURITY_HOTRG
+    ntensors = 4
+    scalefactor = 4.0
+
+
+def _ncomp(scheme):
+    return lib().tnr_ncomponents(scheme.kind)
+
+
 # ---------------------------------------------------------------- driver
 def step(scheme: TNRScheme, trunc: truncrank):
     """step!(scheme, trunc)"""
@@ -188,11 +241,17 @@ def step(scheme: TNRScheme, trunc: truncrank):
     return scheme
 
 
-def finalize(scheme: TNRScheme) -> float:
-    """finalize!(scheme) (src/utility/finalize.jl:4-14)"""
-    n = C.c_double()
-    check(lib().tnr_finalize(scheme._h, C.byref(n)))
-    return n.value
+def finalize(scheme: TNRScheme):
+    """finalize!(scheme) (src/utility/finalize.jl:4-14); a tuple for the
+    impurity schemes (finalize.jl:74-101)."""
+    nc = _ncomp(scheme)
+    if nc == 1:
+        n = C.c_double()
+        check(lib().tnr_finalize(scheme._h, C.byref(n)))
+        return n.value
+    out = np.zeros(nc)
+    check(lib().tnr_finalize_multi(scheme._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+    return tuple(float(x) for x in out)
 
 
 class Finalizer:
@@ -217,11 +276,15 @@ def run(scheme: TNRScheme, trscheme: truncrank, criterion: stopcrit,
         log.info("Starting simulation\n %r\n", scheme)
     if isinstance(criterion, maxiter) and finalizer is default_Finalizer:
         n_steps = max(criterion.n, 1)
-        out = np.zeros(n_steps + 1)
+        nc = _ncomp(scheme)
+        out = np.zeros((n_steps + 1) * nc)
         n = C.c_int()
         check(lib().tnr_run(scheme._h, trscheme.howmany, criterion.n, int(bool(finalize_beginning)),
                             out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
-        data = list(out[:n.value])
+        if nc == 1:
+            data = list(out[:n.value])
+        else:
+            data = [tuple(float(x) for x in out[i * nc:(i + 1) * nc]) for i in range(n.value)]
         steps = n_steps
     else:
         data = []
