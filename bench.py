@@ -230,6 +230,83 @@ def workload_c3(tk, maxiter):
             "cases": cases, "both_batched_s": dt}
 
 
+def _oracle_impurity_seconds(kind, chi, maxiter, budget_s):
+    """Single-threaded oracle time of one impurity run (run in a spawned
+    process with one BLAS thread), steps past budget_s extrapolated."""
+    import multiprocessing as mp
+    saved = {k: os.environ.get(k) for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS",
+                                             "MKL_NUM_THREADS")}
+    for k in saved:
+        os.environ[k] = "1"
+    try:
+        with mp.get_context("spawn").Pool(1) as pool:
+            return pool.apply(_cpu_impurity, (kind, chi, maxiter, budget_s))
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _cpu_impurity(kind, chi, maxiter, budget_s):
+    from oracle import models as om
+    from oracle import schemes as osch
+    T, Ti = om.classical_ising_trivial(om.ising_betac_mp), om.classical_ising_impurity(om.ising_betac_mp)
+    sc = (osch.ImpurityTRG(T, Ti, T, T, T, gauge="pivoted") if kind == "ImpurityTRG"
+          else osch.ImpurityHOTRG(T, Ti, Ti, T))
+    t0 = time.perf_counter()
+    sc.finalize()
+    last, steps = 0.0, 0
+    while steps < maxiter:
+        t1 = time.perf_counter()
+        sc.step(chi)
+        sc.finalize()
+        last = time.perf_counter() - t1
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    return (time.perf_counter() - t0) + (maxiter - steps) * last, steps
+
+
+def workload_imp(tk, maxiter, cpu_seconds):
+    """ImpurityTRG (sites 1-4, magnetisation impurity at site 1) and
+    ImpurityHOTRG (first- and second-order impurities) on Trivial Ising at
+    beta_c: time per run and per step on the device, free energy against
+    Onsager, <m> (ImpurityTRG) and <m^2> (ImpurityHOTRG) of the last step,
+    and the single-threaded oracle on the same run beside it (at the
+    reference's test bond dimensions)."""
+    cases = []
+    T, Ti = tk.classical_ising(None, "Trivial"), tk.classical_ising_impurity(None)
+    for kind, sf, chis in (("ImpurityTRG", 2.0, (24, 32, 48)), ("ImpurityHOTRG", 4.0, (16, 24, 32))):
+        for chi in chis:
+            mk = ((lambda: tk.ImpurityTRG(T, Ti, T, T, T)) if kind == "ImpurityTRG"
+                  else (lambda: tk.ImpurityHOTRG(T, Ti, Ti, T)))
+            w = mk()
+            tk.run(w, tk.truncrank(chi), tk.maxiter(2), verbosity=0)   # warm-up
+            w.close()
+            sc = mk()
+            d, dt = _timed(lambda: tk.run(sc, tk.truncrank(chi), tk.maxiter(maxiter),
+                                          verbosity=0))
+            sc.close()
+            f = tk.free_energy([x[0] for x in d], tk.ising_betac, scalefactor=sf)
+            c = {"scheme": kind, "chi": chi, "maxiter": maxiter, "s_per_run": dt,
+                 "steps_per_s": maxiter / dt, "rel_df_vs_onsager": abs(f - tk.f_onsager) /
+                 abs(tk.f_onsager)}
+            if kind == "ImpurityTRG":
+                c["m_last"] = d[-1][1] / d[-1][0]
+            else:
+                c["m2_last"] = d[-1][3] / d[-1][0]
+            if chi == chis[0] and cpu_seconds > 0:
+                cs, steps = _oracle_impurity_seconds(kind, chi, maxiter, cpu_seconds)
+                c["oracle_cpu_s_per_run"] = cs
+                c["oracle_cpu_steps_timed"] = steps
+                c["gpu_speedup_vs_oracle_1core"] = cs / dt
+            cases.append(c)
+    return {"workload": "Impurity schemes: Trivial Ising beta_c, magnetisation impurities",
+            "cases": cases}
+
+
 def workload_c4(tk, n, maxiter, check):
     """512 BTRG runs, beta ~ U[0.35, 0.55], chi ~ U{16, 24, 32, 48}."""
     rng = np.random.default_rng(SEED + 4)
@@ -325,8 +402,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrivals per second")
-    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4"], default="scan",
-                    help="scan = the headline beta scan; c2-c4 print a report line of their own")
+    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp"], default="scan",
+                    help="scan = the headline beta scan; c2-c4 and imp print a report line "
+                         "of their own")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -339,6 +417,8 @@ def main():
             r = workload_c2(tk, a.maxiter)
         elif a.workload == "c3":
             r = workload_c3(tk, a.maxiter)
+        elif a.workload == "imp":
+            r = workload_imp(tk, a.maxiter, 0.0 if a.no_cpu else a.cpu_seconds)
         else:
             r = workload_c4(tk, 512 if a.batch == 128 else a.batch, a.maxiter, a.check)
         print(json.dumps(r))

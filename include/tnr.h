@@ -97,7 +97,11 @@ int tnr_get_tensor(tnr_scheme* s, double* data, size_t ndata);
  * ImpurityHOTRG(T, T_imp_order1_1, T_imp_order1_2, T_imp_order2)
  * (src/schemes/impurityhotrg.jl:27-46, ntensors = 4): data[i] / ndata[i] is
  * tensor i in the format of tnr_create; all share leg_dims (the reference
- * asserts equal spaces).  tnr_step and tnr_run work as for the other kinds. */
+ * asserts equal spaces).  tnr_step and tnr_run work as for the other kinds.
+ * ImpurityTRG's impurity norm contracts factors of different SVDs, so it
+ * depends on the singular-vector basis, which the reference leaves to
+ * LAPACK; here every ImpurityTRG SVD is put in the pivoted-row gauge of
+ * tnr_kernel_pivoted_gauge (include/tnr_kernels.h). */
 int tnr_create_multi(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims,
                      const double* const* data, const size_t* ndata, int ntensors,
                      int precision);
