@@ -91,6 +91,32 @@ int tnr_get_dims(const tnr_scheme* s, int32_t* leg_dims, size_t* ndata);
 /* Copy T out (format above) / replace T (checkpoint, resume, debugging). */
 int tnr_get_tensor(tnr_scheme* s, double* data, size_t ndata);
 
+/* ---- observables on the scheme's tensor --------------------------------
+ * Each works on the scheme's current T (BTRG: T with S1, S2 absorbed as
+ * the reference does); the transfer matrix of `unitcell` (1 or 2) tensors
+ * in a row needs equal left/right and top/bottom spaces, and every
+ * coupled-charge block of it at most 128 wide. */
+/* All eigenvalues of the transfer matrix (eig_full in cft.jl:16, 50 and
+ * 296, 326): re/im hold `cap` values; *count receives the number written
+ * (or needed).  Order: decreasing modulus, ties by decreasing real part,
+ * then decreasing imaginary part. */
+int tnr_transfer_spectrum(tnr_scheme* s, int unitcell, double* re, double* im,
+                          size_t cap, size_t* count);
+/* cft_data(scheme; v, unitcell, is_real): src/utility/cft.jl:5-71 (the
+ * spectrum above, real part > 0 and modulus > 1e-12 kept, then
+ * unitcell / (2 pi v) log(d_1 / d_i)); im is 0 when is_real. */
+int tnr_cft_data(tnr_scheme* s, int unitcell, double v, int is_real,
+                 double* re, double* im, size_t cap, size_t* count);
+/* ground_state_degeneracy(scheme; unitcell): cft.jl:278-339. */
+int tnr_ground_state_degeneracy(tnr_scheme* s, int unitcell, double* gsd);
+/* gu_wen_ratio(scheme) -> X1, X2: cft.jl:374-396. */
+int tnr_gu_wen_ratio(tnr_scheme* s, double* x1, double* x2);
+/* central_charge(scheme, n): cft.jl:256-268. */
+int tnr_central_charge(tnr_scheme* s, double n, double* c);
+/* finalize_two_by_two!(scheme): src/utility/finalize.jl:17-41 (TRG, HOTRG
+ * and BTRG): *out = n^(1/4) of the 2x2 torus, T /= n^(1/4) on the device. */
+int tnr_finalize_two_by_two(tnr_scheme* s, double* out);
+
 /* ---- impurity schemes ------------------------------------------------- */
 /* ImpurityTRG(T, T_imp1, T_imp2, T_imp3, T_imp4)
  * (src/schemes/impuritytrg.jl:40-58, ntensors = 5) or

@@ -48,6 +48,14 @@ int tnr_kernel_svd(int m, int n, const double* M, int k, double* U,
  * counterpart in the reference, which keeps LAPACK's basis; k <= 96. */
 int tnr_kernel_pivoted_gauge(int m, int k, const double* U, const double* s, double* Q);
 
+/* All eigenvalues of `count` real n x n matrices A (column-major, ld = n,
+ * stored one after another; n <= 128): wr/wi (count x n) receive real and
+ * imaginary parts, conjugate pairs adjacent, in the solver's deflation order.
+ * Balancing, Householder reduction to Hessenberg form and Francis
+ * double-shift QR, one wavefront per matrix; the engine of cft_data and
+ * ground_state_degeneracy (eig_full in src/utility/cft.jl:16). */
+int tnr_kernel_eig(int count, int n, const double* A, double* wr, double* wi);
+
 /* Largest-|lambda| k eigenpairs of the symmetrised n x n matrix M:
  * U (n x k), lam (k). */
 int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam,

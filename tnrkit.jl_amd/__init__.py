@@ -14,10 +14,15 @@ The directory name is not a valid Python identifier; load it with
 from ._lib import TNRError, lib  # noqa: F401
 from .models import (classical_clock, classical_ising, classical_ising_impurity,  # noqa: F401
                      classical_potts, f_onsager,
-                     ising_betac, potts_betac)
+                     ising_betac, ising_cft_exact, potts_betac)
 from .schemes import (BTRG, FP64, HOTRG, SPLIT32, TRG, Finalizer, ImpurityHOTRG,  # noqa: F401
                       ImpurityTRG, MultipleCrit,
                       convcrit, default_Finalizer, finalize, free_energy, maxiter, run, step,
                       trivial_convcrit, truncrank)
+from .observables import (GSDegeneracy_Finalizer, central_charge, cft_data,  # noqa: F401
+                          finalize_central_charge, finalize_groundstatedegeneracy,
+                          finalize_gu_wen_ratio, finalize_two_by_two, ground_state_degeneracy,
+                          gu_wen_ratio, guwenratio_Finalizer, transfer_spectrum,
+                          two_by_two_Finalizer)
 from .serving import Request, Server, run_batch  # noqa: F401
 from .tensors import SymTensor, trivial  # noqa: F401

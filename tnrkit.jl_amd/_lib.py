@@ -79,6 +79,15 @@ SIGNATURES = {
                                   _DP]),
     "tnr_kernel_svd": (C.c_int, [C.c_int, C.c_int, _DP, C.c_int, _DP, _DP, _DP, _IP]),
     "tnr_kernel_pivoted_gauge": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP]),
+    "tnr_kernel_eig": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP]),
+    "tnr_transfer_spectrum": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP, C.c_size_t,
+                                        C.POINTER(C.c_size_t)]),
+    "tnr_cft_data": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, _DP, _DP, C.c_size_t,
+                               C.POINTER(C.c_size_t)]),
+    "tnr_ground_state_degeneracy": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "tnr_gu_wen_ratio": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "tnr_central_charge": (C.c_int, [C.c_void_p, C.c_double, _DP]),
+    "tnr_finalize_two_by_two": (C.c_int, [C.c_void_p, _DP]),
     "tnr_kernel_eigh": (C.c_int, [C.c_int, _DP, C.c_int, _DP, _DP, _IP]),
     "tnr_kernel_jacobi": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP, _IP, _DP]),
     "tnr_profile_enable": (C.c_int, [C.c_int]),

@@ -16,7 +16,7 @@ CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 # never let the compiler form packed FP32 ops on the device side
 HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["kernels.hip", "runtime.cpp", "blocktensor.cpp", "solver.cpp", "schemes.cpp", "capi.cpp",
+SOURCES = ["kernels.hip", "runtime.cpp", "blocktensor.cpp", "solver.cpp", "schemes.cpp", "observables.cpp", "capi.cpp",
            "kapi.cpp"]
 
 

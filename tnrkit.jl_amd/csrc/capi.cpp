@@ -13,6 +13,7 @@
 #include <new>
 #include <thread>
 
+#include "observables.h"
 #include "schemes.h"
 
 using namespace tnr;
@@ -156,6 +157,76 @@ int tnr_get_tensor_at(tnr_scheme* s, int index, double* data, size_t ndata) {
         const BT& t = index == 0 ? s->s.T : s->s.imp[index - 1];
         if ((int64_t)ndata != t.L.size) return fail(TNR_ERR_ARG, "ndata does not match");
         download(*s->ctx, t, data);
+        return TNR_OK;
+    });
+}
+
+// ---- observables on the scheme's tensor
+namespace {
+int write_complex(const std::vector<std::complex<double>>& v, double* re, double* im, size_t cap,
+                  size_t* count) {
+    if (count) *count = v.size();
+    if (v.size() > cap)
+        return fail(TNR_ERR_ARG, ("output holds " + std::to_string(cap) + " values, " +
+                                  std::to_string(v.size()) + " needed").c_str());
+    for (size_t i = 0; i < v.size(); ++i) {
+        re[i] = v[i].real();
+        if (im) im[i] = v[i].imag();
+    }
+    return TNR_OK;
+}
+}  // namespace
+
+int tnr_transfer_spectrum(tnr_scheme* s, int unitcell, double* re, double* im, size_t cap,
+                          size_t* count) {
+    return guarded([&] {
+        if (!s || !re || !im || !count) return fail(TNR_ERR_ARG, "null pointer");
+        return write_complex(transfer_spectrum(*s->ctx, s->s, unitcell), re, im, cap, count);
+    });
+}
+
+int tnr_cft_data(tnr_scheme* s, int unitcell, double v, int is_real, double* re, double* im,
+                 size_t cap, size_t* count) {
+    return guarded([&] {
+        if (!s || !re || !im || !count) return fail(TNR_ERR_ARG, "null pointer");
+        if (!(v > 0.0)) return fail(TNR_ERR_ARG, "v must be positive");
+        return write_complex(cft_data(*s->ctx, s->s, unitcell, v, is_real != 0), re, im, cap,
+                             count);
+    });
+}
+
+int tnr_ground_state_degeneracy(tnr_scheme* s, int unitcell, double* gsd) {
+    return guarded([&] {
+        if (!s || !gsd) return fail(TNR_ERR_ARG, "null pointer");
+        *gsd = ground_state_degeneracy(*s->ctx, s->s, unitcell);
+        return TNR_OK;
+    });
+}
+
+int tnr_gu_wen_ratio(tnr_scheme* s, double* x1, double* x2) {
+    return guarded([&] {
+        if (!s || !x1 || !x2) return fail(TNR_ERR_ARG, "null pointer");
+        gu_wen_ratio(*s->ctx, s->s, x1, x2);
+        return TNR_OK;
+    });
+}
+
+int tnr_central_charge(tnr_scheme* s, double n, double* c) {
+    return guarded([&] {
+        if (!s || !c) return fail(TNR_ERR_ARG, "null pointer");
+        if (!(n > 0.0)) return fail(TNR_ERR_ARG, "n must be positive");
+        *c = central_charge(*s->ctx, s->s, n);
+        return TNR_OK;
+    });
+}
+
+int tnr_finalize_two_by_two(tnr_scheme* s, double* out) {
+    return guarded([&] {
+        if (!s || !out) return fail(TNR_ERR_ARG, "null pointer");
+        if (s->s.kind != TNR_TRG && s->s.kind != TNR_BTRG && s->s.kind != TNR_HOTRG)
+            return fail(TNR_ERR_STATE, "finalize_two_by_two! is defined for TRG, BTRG and HOTRG");
+        *out = finalize_two_by_two(*s->ctx, s->s);
+        check_norms(out, 1);
         return TNR_OK;
     });
 }

@@ -126,6 +126,30 @@ int tnr_kernel_pivoted_gauge(int m, int k, const double* U, const double* s, dou
     });
 }
 
+int tnr_kernel_eig(int count, int n, const double* A, double* wr, double* wi) {
+    return kguard([&] {
+        TNR_REQUIRE(count > 0 && n > 0 && n <= EIG_NMAX && A && wr && wi, TNR_ERR_ARG,
+                    "bad eigenvalue arguments");
+        Ctx ctx;
+        Buf dA = up(ctx, A, (size_t)count * n * n);
+        Buf dr(&ctx.pool, (size_t)count * n * 8), di(&ctx.pool, (size_t)count * n * 8);
+        Buf dinfo(&ctx.pool, (size_t)count * sizeof(int));
+        std::vector<EigDesc> d;
+        for (int i = 0; i < count; ++i)
+            d.push_back(EigDesc{dA.d() + (size_t)i * n * n, dr.d() + (size_t)i * n,
+                                di.d() + (size_t)i * n, dinfo.as<int>() + i, n, n});
+        EigDesc* dd = ctx.stage->put(ctx.stream, d);
+        launch_eig(ctx.stream, dd, count, n);
+        std::vector<int> info(count);
+        ctx.fetch(wr, dr.d(), (size_t)count * n * 8);
+        ctx.fetch(wi, di.d(), (size_t)count * n * 8);
+        ctx.fetch(info.data(), dinfo.d(), (size_t)count * sizeof(int));
+        for (int x : info)
+            TNR_REQUIRE(x == 0, TNR_ERR_NOCONV, "QR iteration did not converge");
+        return TNR_OK;
+    });
+}
+
 int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam, int* iters) {
     return kguard([&] {
         TNR_REQUIRE(n > 0 && k > 0 && k <= n && M && U && lam, TNR_ERR_ARG, "bad eigh arguments");

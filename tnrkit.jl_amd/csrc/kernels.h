@@ -120,6 +120,27 @@ struct CanonDesc {
     int32_t m, ldu, k, knn, nc;
 };
 
+// Grouped dot product: *out = sum_i x[i] y[i] over n elements (k_dot, one
+// workgroup per problem, fixed summation order).
+struct DotDesc {
+    const double* x;
+    const double* y;
+    double* out;
+    int64_t n;
+};
+
+// All eigenvalues of a real n x n matrix (n <= EIG_NMAX), k_eig_hqr: wr/wi
+// receive the real/imaginary parts (conjugate pairs adjacent), *info 0 or 1
+// when the QR iteration did not converge.
+constexpr int EIG_NMAX = 128;
+struct EigDesc {
+    const double* A;
+    double* wr;
+    double* wi;
+    int* info;
+    int32_t n, lda;
+};
+
 // Device-side stopping rule of the block iteration (one thread per problem):
 // with sigma the Ritz values of the current basis (descending), kk the
 // number of the first kmax above null_rtol * sigma_0 and
@@ -180,6 +201,9 @@ void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rt
 void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd);
 // kmax: the largest k of the problems (sizes the LDS); tie: relative pivot tie
 void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, double tie);
+void launch_dot(hipStream_t st, const DotDesc* d_descs, int nd);
+// nmax: the largest n of the problems (sizes the LDS)
+void launch_eig(hipStream_t st, const EigDesc* d_descs, int nd, int nmax);
 void launch_randfill(hipStream_t st, const FillDesc* d_descs, int nd, int64_t total);
 void launch_gather(hipStream_t st, const GatherDesc* d_descs, int nd, int64_t total);
 

@@ -1046,6 +1046,319 @@ __global__ __launch_bounds__(CNT) void k_canon(const CanonDesc* __restrict__ des
     }
 }
 
+// ------------------------------------------------------------- dot product
+__global__ __launch_bounds__(1024) void k_dot(const DotDesc* __restrict__ descs, int nd) {
+    __shared__ double part[16];
+    const DotDesc D = descs[blockIdx.x];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (int64_t i = t; i < D.n; i += 1024) acc += D.x[i] * D.y[i];
+    acc = wave_sum(acc);
+    if ((t & 63) == 0) part[t >> 6] = acc;
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; ++w) s += part[w];
+        D.out[0] = s;
+    }
+}
+
+// ------------------------------------------- eigenvalues of a real matrix
+// One 64-lane wavefront per matrix, the matrix in LDS (column-major, ld n+1):
+//  1. balancing by powers of two (Parlett & Reinsch): row i / f, column i * f
+//     while that brings the off-diagonal row and column norms closer;
+//  2. Householder reduction to upper Hessenberg form;
+//  3. Francis double-shift QR on the Hessenberg matrix, eigenvalues only
+//     (the EISPACK hqr scheme: deflation on a negligible subdiagonal entry,
+//     a 2x2 block giving a real pair or a conjugate pair, exceptional shifts
+//     after 10 and 20 iterations).
+// Every lane runs the same control flow on values read from LDS (the wave
+// sums are xor butterflies, identical in every lane); the row and column
+// updates of each step are spread over the lanes.
+constexpr int EIGT = 64;
+constexpr int EIG_MAXITS = 60;   // QR iterations per eigenvalue before giving up
+
+__global__ __launch_bounds__(EIGT) void k_eig_hqr(const EigDesc* __restrict__ descs, int nd) {
+    extern __shared__ double lds[];
+    const EigDesc D = descs[blockIdx.x];
+    const int nf = D.n, t = threadIdx.x, ld = nf + 1;
+    double* hf = lds;
+    double* v = hf + (size_t)nf * ld;
+#define HF(i, j) hf[(i) + (j) * ld]
+    for (int e = t; e < nf * nf; e += EIGT) {
+        const int j = e / nf, i = e - j * nf;
+        HF(i, j) = D.A[i + (int64_t)j * D.lda];
+    }
+    __syncthreads();
+    // 0. permutation (as LAPACK's balancing 'P'): a row with no off-diagonal
+    //    entry inside the active window [ilo, ihi] is swapped to ihi, a
+    //    column with none to ilo; its diagonal entry is an eigenvalue and
+    //    the window shrinks (exact zeros stay exact)
+    int ilo = 0, ihi = nf - 1;
+    auto swap_idx = [&](int a, int b) {
+        if (a == b) return;
+        for (int j = t; j < nf; j += EIGT) {
+            const double x = HF(a, j);
+            HF(a, j) = HF(b, j);
+            HF(b, j) = x;
+        }
+        __syncthreads();
+        for (int i = t; i < nf; i += EIGT) {
+            const double x = HF(i, a);
+            HF(i, a) = HF(i, b);
+            HF(i, b) = x;
+        }
+        __syncthreads();
+    };
+    for (bool moved = true; moved && ihi >= ilo;) {
+        moved = false;
+        for (int j = ihi; j >= ilo && !moved; --j) {
+            int nz = 0;
+            for (int i = ilo + t; i <= ihi; i += EIGT)
+                if (i != j && HF(j, i) != 0.0) nz = 1;
+            if (!__any(nz)) {
+                swap_idx(j, ihi);
+                if (t == 0) {
+                    D.wr[ihi] = HF(ihi, ihi);
+                    D.wi[ihi] = 0.0;
+                }
+                --ihi;
+                moved = true;
+            }
+        }
+        for (int j = ilo; j <= ihi && !moved; ++j) {
+            int nz = 0;
+            for (int i = ilo + t; i <= ihi; i += EIGT)
+                if (i != j && HF(i, j) != 0.0) nz = 1;
+            if (!__any(nz)) {
+                swap_idx(j, ilo);
+                if (t == 0) {
+                    D.wr[ilo] = HF(ilo, ilo);
+                    D.wi[ilo] = 0.0;
+                }
+                ++ilo;
+                moved = true;
+            }
+        }
+        __syncthreads();
+    }
+#undef HF
+    // the rest works on the window: n x n at (ilo, ilo), eigenvalues to wr/wi + ilo
+    const int n = ihi - ilo + 1;
+    double* h = hf + (size_t)ilo * (ld + 1);
+    double* const wro = D.wr + ilo;
+    double* const wio = D.wi + ilo;
+#define HH(i, j) h[(i) + (j) * ld]
+    // 1. balancing
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        bool done = true;
+        for (int i = 0; i < n; ++i) {
+            double c = 0.0, r = 0.0;
+            for (int j = t; j < n; j += EIGT)
+                if (j != i) {
+                    c += fabs(HH(j, i));
+                    r += fabs(HH(i, j));
+                }
+            c = wave_sum(c);
+            r = wave_sum(r);
+            if (c != 0.0 && r != 0.0) {
+                double g = r * 0.5, f = 1.0;
+                const double s0 = c + r;
+                while (c < g) {
+                    f *= 2.0;
+                    c *= 4.0;
+                }
+                g = r * 2.0;
+                while (c > g) {
+                    f *= 0.5;
+                    c *= 0.25;
+                }
+                if ((c + r) / f < 0.95 * s0) {
+                    done = false;
+                    const double gi = 1.0 / f;
+                    for (int j = t; j < n; j += EIGT) HH(i, j) *= gi;
+                    __syncthreads();
+                    for (int j = t; j < n; j += EIGT) HH(j, i) *= f;
+                }
+            }
+            __syncthreads();
+        }
+        if (done) break;
+    }
+    // 2. Hessenberg reduction
+    for (int k = 0; k + 2 < n; ++k) {
+        double ss = 0.0;
+        for (int i = k + 2 + t; i < n; i += EIGT) ss += HH(i, k) * HH(i, k);
+        ss = wave_sum(ss);
+        if (ss == 0.0) continue;
+        const double x0 = HH(k + 1, k);
+        const double alpha = -copysign(sqrt(x0 * x0 + ss), x0);
+        const double v0 = x0 - alpha, beta = 2.0 / (v0 * v0 + ss);
+        for (int i = k + 1 + t; i < n; i += EIGT) v[i] = (i == k + 1) ? v0 : HH(i, k);
+        __syncthreads();
+        for (int j = k + t; j < n; j += EIGT) {
+            double d = 0.0;
+            for (int i = k + 1; i < n; ++i) d += v[i] * HH(i, j);
+            d *= beta;
+            for (int i = k + 1; i < n; ++i) HH(i, j) -= d * v[i];
+        }
+        __syncthreads();
+        for (int i = t; i < n; i += EIGT) {
+            double d = 0.0;
+            for (int j = k + 1; j < n; ++j) d += HH(i, j) * v[j];
+            d *= beta;
+            for (int j = k + 1; j < n; ++j) HH(i, j) -= d * v[j];
+        }
+        __syncthreads();
+        if (t == 0) HH(k + 1, k) = alpha;
+        for (int i = k + 2 + t; i < n; i += EIGT) HH(i, k) = 0.0;
+        __syncthreads();
+    }
+    // 3. double-shift QR
+    double anorm = 0.0;
+    for (int j = t; j < n; j += EIGT)
+        for (int i = 0; i <= min(j + 1, n - 1); ++i) anorm += fabs(HH(i, j));
+    anorm = wave_sum(anorm);
+    int nn = n - 1, info = 0;
+    double tsh = 0.0;
+    double p = 0.0, q = 0.0, r = 0.0, s = 0.0, w = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    while (nn >= 0 && !info) {
+        int its = 0, l = 0;
+        do {
+            for (l = nn; l >= 1; --l) {
+                s = fabs(HH(l - 1, l - 1)) + fabs(HH(l, l));
+                if (s == 0.0) s = anorm;
+                if (fabs(HH(l, l - 1)) + s == s) break;
+            }
+            __syncthreads();
+            if (l >= 1 && t == 0) HH(l, l - 1) = 0.0;
+            __syncthreads();
+            x = HH(nn, nn);
+            if (l == nn) {
+                if (t == 0) {
+                    wro[nn] = x + tsh;
+                    wio[nn] = 0.0;
+                }
+                --nn;
+            } else {
+                y = HH(nn - 1, nn - 1);
+                w = HH(nn, nn - 1) * HH(nn - 1, nn);
+                if (l == nn - 1) {
+                    p = 0.5 * (y - x);
+                    q = p * p + w;
+                    z = sqrt(fabs(q));
+                    x += tsh;
+                    if (q >= 0.0) z = p + copysign(z, p);
+                    if (t == 0) {
+                        if (q >= 0.0) {
+                            wro[nn - 1] = wro[nn] = x + z;
+                            if (z != 0.0) wro[nn] = x - w / z;
+                            wio[nn - 1] = wio[nn] = 0.0;
+                        } else {
+                            wro[nn - 1] = wro[nn] = x + p;
+                            wio[nn - 1] = z;
+                            wio[nn] = -z;
+                        }
+                    }
+                    nn -= 2;
+                } else {
+                    if (its == EIG_MAXITS) {
+                        info = 1;
+                        break;
+                    }
+                    if (its == 10 || its == 20) {
+                        tsh += x;
+                        for (int i = t; i <= nn; i += EIGT) HH(i, i) -= x;
+                        __syncthreads();
+                        s = fabs(HH(nn, nn - 1)) + fabs(HH(nn - 1, nn - 2));
+                        y = x = 0.75 * s;
+                        w = -0.4375 * s * s;
+                    }
+                    ++its;
+                    int m = nn - 2;
+                    for (; m >= l; --m) {
+                        z = HH(m, m);
+                        r = x - z;
+                        s = y - z;
+                        p = (r * s - w) / HH(m + 1, m) + HH(m, m + 1);
+                        q = HH(m + 1, m + 1) - z - r - s;
+                        r = HH(m + 2, m + 1);
+                        s = fabs(p) + fabs(q) + fabs(r);
+                        p /= s;
+                        q /= s;
+                        r /= s;
+                        if (m == l) break;
+                        const double u = fabs(HH(m, m - 1)) * (fabs(q) + fabs(r));
+                        const double vv = fabs(p) * (fabs(HH(m - 1, m - 1)) + fabs(z) +
+                                                     fabs(HH(m + 1, m + 1)));
+                        if (u + vv == vv) break;
+                    }
+                    __syncthreads();
+                    for (int i = m + 2 + t; i <= nn; i += EIGT) {
+                        HH(i, i - 2) = 0.0;
+                        if (i != m + 2) HH(i, i - 3) = 0.0;
+                    }
+                    __syncthreads();
+                    for (int k = m; k <= nn - 1; ++k) {
+                        if (k != m) {
+                            p = HH(k, k - 1);
+                            q = HH(k + 1, k - 1);
+                            r = (k != nn - 1) ? HH(k + 2, k - 1) : 0.0;
+                            x = fabs(p) + fabs(q) + fabs(r);
+                            if (x != 0.0) {
+                                p /= x;
+                                q /= x;
+                                r /= x;
+                            }
+                        }
+                        s = copysign(sqrt(p * p + q * q + r * r), p);
+                        __syncthreads();
+                        if (s != 0.0) {
+                            if (t == 0) {
+                                if (k == m) {
+                                    if (l != m) HH(k, k - 1) = -HH(k, k - 1);
+                                } else {
+                                    HH(k, k - 1) = -s * x;
+                                }
+                            }
+                            p += s;
+                            x = p / s;
+                            y = q / s;
+                            z = r / s;
+                            q /= p;
+                            r /= p;
+                            const bool three = k != nn - 1;
+                            for (int j = k + t; j <= nn; j += EIGT) {
+                                double pp = HH(k, j) + q * HH(k + 1, j);
+                                if (three) {
+                                    pp += r * HH(k + 2, j);
+                                    HH(k + 2, j) -= pp * z;
+                                }
+                                HH(k + 1, j) -= pp * y;
+                                HH(k, j) -= pp * x;
+                            }
+                            __syncthreads();
+                            const int mmin = nn < k + 3 ? nn : k + 3;
+                            for (int i = l + t; i <= mmin; i += EIGT) {
+                                double pp = x * HH(i, k) + y * HH(i, k + 1);
+                                if (three) {
+                                    pp += z * HH(i, k + 2);
+                                    HH(i, k + 2) -= pp * r;
+                                }
+                                HH(i, k + 1) -= pp * q;
+                                HH(i, k) -= pp;
+                            }
+                            __syncthreads();
+                        }
+                    }
+                }
+            }
+        } while (l < nn - 1 && !info);
+    }
+#undef HH
+    if (t == 0) D.info[0] = info;
+}
+
 // stopping rule of the block iteration, one thread per problem
 __global__ void k_conv(const ConvDesc* __restrict__ descs, int nd, double null_rtol,
                        double target, int min_iters, int max_iters) {
@@ -1390,6 +1703,25 @@ void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, do
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
     const size_t shm = ((size_t)kmax * kmax + 2 * (size_t)kmax) * sizeof(double);
     hipLaunchKernelGGL(k_canon, dim3(nd), dim3(CNT), shm, st, d_descs, nd, tie);
+    TNR_HIP(hipGetLastError());
+}
+
+void launch_dot(hipStream_t st, const DotDesc* d_descs, int nd) {
+    if (nd <= 0) return;
+    ProfScope ps(st, PK_MISC, 0.0, 0.0);
+    hipLaunchKernelGGL(k_dot, dim3(nd), dim3(1024), 0, st, d_descs, nd);
+    TNR_HIP(hipGetLastError());
+}
+
+void launch_eig(hipStream_t st, const EigDesc* d_descs, int nd, int nmax) {
+    if (nd <= 0) return;
+    TNR_REQUIRE(nmax >= 1 && nmax <= EIG_NMAX, TNR_ERR_INTERNAL, "eigenproblem above EIG_NMAX");
+    auto bytes = [](int n) { return ((size_t)n * (n + 1) + (size_t)n) * sizeof(double); };
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void*)k_eig_hqr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes(EIG_NMAX));
+    TNR_HIP(attr);
+    ProfScope ps(st, PK_MISC, 0.0, 0.0);
+    hipLaunchKernelGGL(k_eig_hqr, dim3(nd), dim3(EIGT), bytes(nmax), st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 

@@ -18,6 +18,9 @@ getcontext().prec = 50
 ising_betac = math.log(1.0 + math.sqrt(2.0)) / 2.0
 #: src/models/ising.jl:2
 f_onsager = -2.1096511446082074596677792835110847808254932754354
+# src/models/ising.jl:3-7: the exact Ising CFT scaling dimensions
+ising_cft_exact = [1 / 8, 1, 9 / 8, 9 / 8, 2, 2, 2, 2, 17 / 8, 17 / 8, 17 / 8, 3, 3, 3, 3, 3,
+                   25 / 8, 25 / 8, 25 / 8, 25 / 8, 25 / 8, 25 / 8]
 
 
 def _bond(beta, exact_betac):
