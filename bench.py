@@ -307,6 +307,55 @@ def workload_imp(tk, maxiter, cpu_seconds):
             "cases": cases}
 
 
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def workload_obs(tk, reps, with_cpu):
+    """Observables on the fixed-point tensor after a run (cft_data,
+    ground_state_degeneracy, gu_wen_ratio, central_charge, the 2x2 finalize):
+    median seconds per call on the device at the reference's CFT test
+    configurations and at larger Trivial chi, the single-threaded oracle on
+    the same state beside the small ones."""
+    import os as _os
+    cases = []
+    configs = (("TRG", "Z2", 24, 10), ("BTRG", "Z2", 24, 10), ("HOTRG", "Z2", 16, 4),
+               ("TRG", "Trivial", 64, 10), ("BTRG", "Trivial", 80, 8))
+    for name, sym, chi, n in configs:
+        sc = getattr(tk, name)(tk.classical_ising(symmetry=sym))
+        d = tk.run(sc, tk.truncrank(chi), tk.maxiter(n), verbosity=0)
+        c = {"scheme": name, "symmetry": sym, "chi": chi, "maxiter": n}
+        tk.cft_data(sc)                                          # warm-up
+        c["cft_data_s"] = _median_time(lambda: tk.cft_data(sc), reps)
+        c["ground_state_degeneracy_s"] = _median_time(lambda: tk.ground_state_degeneracy(sc), reps)
+        c["gu_wen_ratio_s"] = _median_time(lambda: tk.gu_wen_ratio(sc), reps)
+        c["central_charge_s"] = _median_time(lambda: tk.central_charge(sc, d[-1]), reps)
+        c["finalize_two_by_two_s"] = _median_time(lambda: tk.finalize_two_by_two(sc), reps)
+        cft = tk.cft_data(sc)
+        c["cft_1_2"] = [float(cft[1]), float(cft[2])] if len(cft) > 2 else None
+        if with_cpu and chi <= 24:
+            _os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+            from oracle import models as om
+            from oracle import observables as ob
+            from oracle import schemes as osch
+            T = om.classical_ising_z2(om.ising_betac_mp) if sym == "Z2" else \
+                om.classical_ising_trivial(om.ising_betac_mp)
+            o = getattr(osch, name)(T)
+            osch.run(o, chi, n)
+            c["oracle_cft_data_s"] = _median_time(lambda: ob.cft_data(o), reps)
+            c["oracle_gu_wen_ratio_s"] = _median_time(lambda: ob.gu_wen_ratio(o), reps)
+            c["oracle_finalize_two_by_two_s"] = _median_time(lambda: ob.finalize_two_by_two(o), reps)
+        sc.close()
+        cases.append(c)
+    return {"workload": "Observables on the Ising fixed point, per call", "reps": reps,
+            "cases": cases}
+
+
 def workload_c4(tk, n, maxiter, check):
     """512 BTRG runs, beta ~ U[0.35, 0.55], chi ~ U{16, 24, 32, 48}."""
     rng = np.random.default_rng(SEED + 4)
@@ -402,9 +451,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrivals per second")
-    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp"], default="scan",
-                    help="scan = the headline beta scan; c2-c4 and imp print a report line "
-                         "of their own")
+    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs"], default="scan",
+                    help="scan = the headline beta scan; c2-c4, imp and obs print a report "
+                         "line of their own")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -419,6 +468,8 @@ def main():
             r = workload_c3(tk, a.maxiter)
         elif a.workload == "imp":
             r = workload_imp(tk, a.maxiter, 0.0 if a.no_cpu else a.cpu_seconds)
+        elif a.workload == "obs":
+            r = workload_obs(tk, 5, not a.no_cpu)
         else:
             r = workload_c4(tk, 512 if a.batch == 128 else a.batch, a.maxiter, a.check)
         print(json.dumps(r))

@@ -174,3 +174,20 @@ def test_reference_gsd_gu_wen_gates(tk, cls_name, chi):
         g.close()
         for v in (gsd, x1, x2):
             assert abs(v / want - 1) < 1e-2
+
+
+@pytest.mark.parametrize("cls_name", ["TRG", "BTRG", "HOTRG"])
+def test_observables_z3_potts_vs_oracle(tk, cls_name):
+    """Z3 sectors are not self-dual: the traces pair a leg with the dual of
+    another (the torus through charge-conjugated permutes)."""
+    beta, chi, n = 1.0, 12, 4
+    o = getattr(osch, cls_name)(om.classical_potts_zn(3, beta))
+    osch.run(o, chi, n)
+    g = getattr(tk, cls_name)(tk.classical_potts(3, beta))
+    d = tk.run(g, tk.truncrank(chi), tk.maxiter(n), verbosity=0)
+    assert same_multiset(_big(tk.transfer_spectrum(g)), _big(ob.transfer_spectrum(o)), 1e-8)
+    assert abs(tk.ground_state_degeneracy(g) / ob.ground_state_degeneracy(o) - 1) < 1e-8
+    assert np.allclose(tk.gu_wen_ratio(g), ob.gu_wen_ratio(o), rtol=1e-8)
+    assert abs(tk.central_charge(g, d[-1]) - ob.central_charge(o, d[-1])) < 1e-8
+    assert abs(tk.finalize_two_by_two(g) / ob.finalize_two_by_two(o) - 1) < 1e-8
+    g.close()

@@ -120,12 +120,15 @@ struct CanonDesc {
     int32_t m, ldu, k, knn, nc;
 };
 
-// Grouped dot product: *out = sum_i x[i] y[i] over n elements (k_dot, one
-// workgroup per problem, fixed summation order).
+// Grouped dot product: *out = sum_i x[i] y[i] over n elements.  DOT_PARTS
+// workgroups per problem each sum one fixed contiguous chunk into part[],
+// then the parts are added in order (the result does not depend on timing).
+constexpr int DOT_PARTS = 256;
 struct DotDesc {
     const double* x;
     const double* y;
     double* out;
+    double* part;      // DOT_PARTS doubles of scratch
     int64_t n;
 };
 

@@ -1047,18 +1047,27 @@ __global__ __launch_bounds__(CNT) void k_canon(const CanonDesc* __restrict__ des
 }
 
 // ------------------------------------------------------------- dot product
-__global__ __launch_bounds__(1024) void k_dot(const DotDesc* __restrict__ descs, int nd) {
-    __shared__ double part[16];
-    const DotDesc D = descs[blockIdx.x];
+// grid (DOT_PARTS, problems): workgroup b sums the chunk [b c, (b+1) c) with
+// c = ceil(n / DOT_PARTS); k_dot_final adds the parts in order
+__global__ __launch_bounds__(256) void k_dot_part(const DotDesc* __restrict__ descs, int nd) {
+    __shared__ double wpart[4];
+    const DotDesc D = descs[blockIdx.y];
     const int t = threadIdx.x;
+    const int64_t c = (D.n + DOT_PARTS - 1) / DOT_PARTS;
+    const int64_t i0 = (int64_t)blockIdx.x * c, i1 = min(D.n, i0 + c);
     double acc = 0.0;
-    for (int64_t i = t; i < D.n; i += 1024) acc += D.x[i] * D.y[i];
+    for (int64_t i = i0 + t; i < i1; i += 256) acc += D.x[i] * D.y[i];
     acc = wave_sum(acc);
-    if ((t & 63) == 0) part[t >> 6] = acc;
+    if ((t & 63) == 0) wpart[t >> 6] = acc;
     __syncthreads();
-    if (t == 0) {
+    if (t == 0) D.part[blockIdx.x] = (wpart[0] + wpart[1]) + (wpart[2] + wpart[3]);
+}
+
+__global__ __launch_bounds__(64) void k_dot_final(const DotDesc* __restrict__ descs, int nd) {
+    const DotDesc D = descs[blockIdx.x];
+    if (threadIdx.x == 0) {
         double s = 0.0;
-        for (int w = 0; w < 16; ++w) s += part[w];
+        for (int b = 0; b < DOT_PARTS; ++b) s += D.part[b];
         D.out[0] = s;
     }
 }
@@ -1708,8 +1717,11 @@ void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, do
 
 void launch_dot(hipStream_t st, const DotDesc* d_descs, int nd) {
     if (nd <= 0) return;
+    TNR_REQUIRE(nd <= 65535, TNR_ERR_INTERNAL, "too many dot products in one launch");
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    hipLaunchKernelGGL(k_dot, dim3(nd), dim3(1024), 0, st, d_descs, nd);
+    hipLaunchKernelGGL(k_dot_part, dim3(DOT_PARTS, nd), dim3(256), 0, st, d_descs, nd);
+    TNR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_dot_final, dim3(nd), dim3(64), 0, st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 

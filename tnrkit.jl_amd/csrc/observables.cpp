@@ -41,12 +41,13 @@ BT identity_leg(Ctx& ctx, int N, const Leg& a) {
 // one readback
 std::vector<double> dots(Ctx& ctx, const std::vector<std::pair<const BT*, const BT*>>& pairs) {
     Buf out(&ctx.pool, std::max<size_t>(1, pairs.size()) * sizeof(double));
+    Buf part(&ctx.pool, std::max<size_t>(1, pairs.size()) * DOT_PARTS * sizeof(double));
     std::vector<DotDesc> d;
     for (size_t i = 0; i < pairs.size(); ++i) {
         TNR_REQUIRE(pairs[i].first->L.same(pairs[i].second->L), TNR_ERR_INTERNAL,
                     "dot of tensors with different layouts");
         d.push_back(DotDesc{pairs[i].first->d(), pairs[i].second->d(), out.d() + i,
-                            pairs[i].first->L.size});
+                            part.d() + i * DOT_PARTS, pairs[i].first->L.size});
     }
     DotDesc* dd = ctx.stage->put(ctx.stream, d);
     launch_dot(ctx.stream, dd, (int)d.size());
@@ -94,13 +95,15 @@ BT transfer_matrix(Ctx& ctx, const Scheme& s, int unitcell) {
                                                     {-1, -3, -2, -4}, 2}})[0]);
 }
 
-// sum A[a,b,c,d] B[d,c,b,a] as tr of Y[a,b,a',b'] = sum_{c,d} A[a,b,c,d] B[d,c,b',a']
-double trace_pair(Ctx& ctx, const Operand& A, const Operand& B) {
-    Operand a = A, b = B;
-    a.labels = {-1, -2, 1, 2};
-    b.labels = {2, 1, -4, -3};
-    BT Y = std::move(contract_many(ctx, {ContractJob{a, b, {-1, -2, -3, -4}, 2}})[0]);
-    return trace_of(ctx, Y);
+// sum A[a,b,c,d] B[d,c,b,a] in O(size): B' = B with its legs reversed and
+// its charges negated (conj) has A's layout when A's legs 3, 2 carry the
+// duals of legs 0, 1, and B'[a,b,c,d] = B[d,c,b,a] sits where A[a,b,c,d]
+// does; then one dot product
+double trace_pair(Ctx& ctx, const BT& A, const BT& B) {
+    BT Br = permute(ctx, B, {3, 2, 1, 0}, 2, /*conj=*/true);
+    TNR_REQUIRE(Br.L.same(A.L), TNR_ERR_ARG,
+                "the torus needs equal left/right and top/bottom spaces");
+    return dots(ctx, {{&A, &Br}})[0];
 }
 
 bool by_modulus(const cplx& a, const cplx& b) {
@@ -202,7 +205,7 @@ void gu_wen_ratio(Ctx& ctx, const Scheme& s, double* x1, double* x2) {
                                                      1}})[0]);
     const double two1 = std::fabs(trace_of(ctx, NN));
     // T[1 2; 3 4] T[4 3; 2 1]
-    const double two2 = std::fabs(trace_pair(ctx, Operand{&su.T, {}}, Operand{&su.T, {}}));
+    const double two2 = std::fabs(trace_pair(ctx, su.T, su.T));
     *x1 = one * one / two1;
     *x2 = one * one / two2;
 }
@@ -247,7 +250,7 @@ double finalize_two_by_two(Ctx& ctx, Scheme& s) {
                           {-1, -2, -3, -4}, 2},
               ContractJob{{&T, {3, -1, -2, 8}, false, wc}, {&T, {8, -3, -4, 3}},
                           {-1, -2, -3, -4}, 2}});
-    const double n = std::fabs(trace_pair(ctx, Operand{&PQ[0], {}}, Operand{&PQ[1], {}}));
+    const double n = std::fabs(trace_pair(ctx, PQ[0], PQ[1]));
     const double r = std::pow(n, 0.25);
     lincomb_many(ctx, {LinJob{&s.T, {{1.0 / r, &s.T}}}});
     return r;
