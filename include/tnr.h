@@ -49,6 +49,7 @@ extern "C" {
 #define TNR_HOTRG  2   /* src/schemes/hotrg.jl:27-160 */
 #define TNR_IMPURITY_TRG    3   /* src/schemes/impuritytrg.jl:40-103 */
 #define TNR_IMPURITY_HOTRG  4   /* src/schemes/impurityhotrg.jl:27-95 */
+#define TNR_ATRG  5   /* src/schemes/atrg.jl:26-76 */
 
 #define TNR_PREC_FP64     0  /* all contractions FP64 MFMA */
 #define TNR_PREC_SPLIT32  1  /* contractions on FP32 MFMA, hi/lo split
@@ -163,7 +164,7 @@ const char* tnr_last_error(void);
  * covering all runs and all symmetry blocks).  Results equal the one-run
  * path within the documented tolerance. */
 typedef struct {
-    int kind;                 /* TNR_TRG / TNR_BTRG / TNR_HOTRG */
+    int kind;                 /* TNR_TRG / TNR_BTRG / TNR_HOTRG / TNR_ATRG */
     int zn;                   /* 1 = Trivial, N = Z_N            */
     const int32_t* leg_dims;  /* 4*zn                            */
     const double* data;       /* initial tensor                  */

@@ -143,6 +143,43 @@ class BTRG:
         return n
 
 
+class ATRG(TRG):
+    """src/schemes/atrg.jl:35-76 (anisotropic TRG): two half-steps, each
+    three truncated SVDs; the tensor is rotated between and after them."""
+    scalefactor = 4.0
+
+    def _half(self, chi):
+        T = self.T
+        # A, S, B = svd_trunc(permute(T, ((1, 3), (2, 4)))); C, D = copies
+        A, s, D, info = svd_trunc(permute(T, (0, 2, 1, 3)), 2, chi)
+        self.log.append(info)
+        B = scale_leg(D, 0, s)                   # B[-1; -2 -3] = S[-1; 1] B[1; -2 -3]
+        C = scale_leg(A, 2, s)                   # C[-1 -2; -3] = C[-1 -2; 1] S[1; -3]
+        # M[-1 -2; -3 -4] := B[-3; 1 -4] * C[-1 1; -2]
+        M = contract([-1, -2, -3, -4], (B, [-3, 1, -4], False), (C, [-1, 1, -2], False))
+        X, s2, Y, info = svd_trunc(permute(M, (0, 2, 1, 3)), 2, chi)
+        self.log.append(info)
+        sq = np.sqrt(s2)
+        X, Y = scale_leg(X, 2, sq), scale_leg(Y, 0, sq)
+        # Q[-1 -2; -3 -4] := A[3 -3; 2] * D[1; -2 4] * X[4 2; -4] * Y[-1 1; 3]
+        AX = contract([-1, -2, -3, -4], (A, [-1, -2, 1], False), (X, [-3, 1, -4], False))
+        DY = contract([-1, -2, -3, -4], (D, [1, -1, -2], False), (Y, [-3, 1, -4], False))
+        Q = contract([-1, -2, -3, -4], (AX, [3, -3, 4, -4], False), (DY, [-2, 4, -1, 3], False))
+        H, s3, G, info = svd_trunc(Q, 2, chi)
+        self.log.append(info)
+        sq = np.sqrt(s3)
+        H, G = scale_leg(H, 2, sq), scale_leg(G, 0, sq)
+        # T[-1 -2; -3 -4] := G[-1; -3 1] * H[1 -2; -4]
+        self.T = contract([-1, -2, -3, -4], (G, [-1, -3, 1], False), (H, [1, -2, -4], False))
+
+    def step(self, chi):
+        self._half(chi)
+        self.T = permute(self.T, (1, 3, 0, 2))   # permute(T, ((2, 4), (1, 3)))
+        self._half(chi)
+        self.T = permute(self.T, (2, 0, 3, 1))   # permute(T, ((3, 1), (4, 2)))
+        return self
+
+
 # ---------------------------------------------------------------- HOTRG parts
 
 def _pick(A, B, chi, log):

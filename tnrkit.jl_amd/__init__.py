@@ -15,7 +15,7 @@ from ._lib import TNRError, lib  # noqa: F401
 from .models import (classical_clock, classical_ising, classical_ising_impurity,  # noqa: F401
                      classical_potts, f_onsager,
                      ising_betac, ising_cft_exact, potts_betac)
-from .schemes import (BTRG, FP64, HOTRG, SPLIT32, TRG, Finalizer, ImpurityHOTRG,  # noqa: F401
+from .schemes import (ATRG, BTRG, FP64, HOTRG, SPLIT32, TRG, Finalizer, ImpurityHOTRG,  # noqa: F401
                       ImpurityTRG, MultipleCrit,
                       convcrit, default_Finalizer, finalize, free_energy, maxiter, run, step,
                       trivial_convcrit, truncrank)

@@ -21,6 +21,7 @@ TNR_ERR_BUSY = -10
 
 TNR_TRG, TNR_BTRG, TNR_HOTRG = 0, 1, 2
 TNR_IMPURITY_TRG, TNR_IMPURITY_HOTRG = 3, 4
+TNR_ATRG = 5
 TNR_PREC_FP64, TNR_PREC_SPLIT32 = 0, 1
 
 

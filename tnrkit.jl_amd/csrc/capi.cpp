@@ -90,8 +90,8 @@ int tnr_create(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims, cons
     return guarded([&] {
         if (!out) return fail(TNR_ERR_ARG, "out is null");
         *out = nullptr;
-        if (kind != TNR_TRG && kind != TNR_BTRG && kind != TNR_HOTRG)
-            return fail(TNR_ERR_ARG, "kind must be TNR_TRG, TNR_BTRG or TNR_HOTRG");
+        if (!single_tensor_kind(kind))
+            return fail(TNR_ERR_ARG, "kind must be TNR_TRG, TNR_BTRG, TNR_HOTRG or TNR_ATRG");
         if (zn < 1 || zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
         if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
             return fail(TNR_ERR_ARG, "unknown precision");
@@ -223,8 +223,9 @@ int tnr_central_charge(tnr_scheme* s, double n, double* c) {
 int tnr_finalize_two_by_two(tnr_scheme* s, double* out) {
     return guarded([&] {
         if (!s || !out) return fail(TNR_ERR_ARG, "null pointer");
-        if (s->s.kind != TNR_TRG && s->s.kind != TNR_BTRG && s->s.kind != TNR_HOTRG)
-            return fail(TNR_ERR_STATE, "finalize_two_by_two! is defined for TRG, BTRG and HOTRG");
+        if (!single_tensor_kind(s->s.kind))
+            return fail(TNR_ERR_STATE,
+                        "finalize_two_by_two! is defined for TRG, BTRG, HOTRG and ATRG");
         *out = finalize_two_by_two(*s->ctx, s->s);
         check_norms(out, 1);
         return TNR_OK;
@@ -605,8 +606,7 @@ int tnr_server_create(tnr_server** out, size_t hbm_budget_bytes, int max_slots, 
 int tnr_server_submit(tnr_server* s, const tnr_run_desc* req, int64_t* id) {
     return guarded([&] {
         if (!s || !req || !id) return fail(TNR_ERR_ARG, "null pointer");
-        if (req->kind != TNR_TRG && req->kind != TNR_BTRG && req->kind != TNR_HOTRG)
-            return fail(TNR_ERR_ARG, "unknown scheme kind");
+        if (!single_tensor_kind(req->kind)) return fail(TNR_ERR_ARG, "unknown scheme kind");
         if (req->zn < 1 || req->zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
         if (req->chi < 1) return fail(TNR_ERR_ARG, "chi must be >= 1");
         if (!req->leg_dims || !req->data) return fail(TNR_ERR_ARG, "null tensor");

@@ -651,6 +651,99 @@ void step_impurity_trg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vecto
     }
 }
 
+// ------------------------------------------------------------------ ATRG
+// src/schemes/atrg.jl:48-76, one half-step for every scheme of the group:
+//   A, S, B = svd_trunc(permute(T, ((1, 3), (2, 4)))); D = B; B = S B; C = A S
+//   M[-1 -2; -3 -4] := B[-3; 1 -4] C[-1 1; -2]
+//   X, S, Y = svd_trunc(permute(M, ((1, 3), (2, 4)))); X = X sqrt(S); Y = sqrt(S) Y
+//   Q[-1 -2; -3 -4] := A[3 -3; 2] D[1; -2 4] X[4 2; -4] Y[-1 1; 3]
+//                    = AX[3, -3, 4, -4] DY[-2, 4, -1, 3]
+//   H, S, G = svd_trunc(Q); H = H sqrt(S); G = sqrt(S) G
+//   T[-1 -2; -3 -4] := G[-1; -3 1] H[1 -2; -4]
+void atrg_half(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis, int tag) {
+    const size_t n = G.size();
+    std::vector<PermJob> pj;
+    for (auto* s : G) pj.push_back(PermJob{&s->T, {0, 2, 1, 3}, 2, false, {}});
+    auto P = permute_many(ctx, pj);
+    std::vector<const BT*> Ts;
+    std::vector<int> tags;
+    for (size_t i = 0; i < n; ++i) { Ts.push_back(&P[i]); tags.push_back(tag); }
+    auto unit = [](size_t, const std::vector<double>& s, std::vector<double>& wu,
+                   std::vector<double>& wv) {
+        wu.assign(s.size(), 1.0);
+        wv = wu;
+    };
+    auto F1 = svd_trunc_many(ctx, Ts, chis, tags, unit);
+    // S on the new legs: V's is a codomain leg (q = c), U's a domain leg (q = -c)
+    std::vector<int> Ns;
+    std::vector<const int*> dims;
+    std::vector<const std::vector<double>*> vals;
+    for (size_t i = 0; i < n; ++i) {
+        Ns.push_back(G[i]->N);
+        dims.push_back(F1[i].dim);
+        vals.push_back(&F1[i].s);
+    }
+    auto S = make_bonds(ctx, Ns, dims, vals);
+    std::vector<ContractJob> cj;
+    for (size_t i = 0; i < n; ++i) {
+        Operand b{&F1[i].V, {-3, 1, -4}, false, {WeightSpec{0, S[i].same_side()}}};
+        Operand c{&F1[i].U, {-1, 1, -2}, false, {WeightSpec{2, S[i].dual_side()}}};
+        cj.push_back(ContractJob{b, c, {-1, -2, -3, -4}, 2});
+    }
+    auto M = contract_many(ctx, cj);
+    pj.clear();
+    for (size_t i = 0; i < n; ++i) pj.push_back(PermJob{&M[i], {0, 2, 1, 3}, 2, false, {}});
+    auto PM = permute_many(ctx, pj);
+    Ts.clear();
+    tags.clear();
+    for (size_t i = 0; i < n; ++i) { Ts.push_back(&PM[i]); tags.push_back(tag + 1); }
+    auto F2 = svd_trunc_many(ctx, Ts, chis, tags, sqrt_weights());
+    cj.clear();
+    for (size_t i = 0; i < n; ++i) {
+        cj.push_back(ContractJob{{&F1[i].U, {-1, -2, 1}}, {&F2[i].U, {-3, 1, -4}},
+                                 {-1, -2, -3, -4}, 2});
+        cj.push_back(ContractJob{{&F1[i].V, {1, -1, -2}}, {&F2[i].V, {-3, 1, -4}},
+                                 {-1, -2, -3, -4}, 2});
+    }
+    auto AD = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)
+        cj.push_back(ContractJob{{&AD[2 * i], {3, -3, 4, -4}}, {&AD[2 * i + 1], {-2, 4, -1, 3}},
+                                 {-1, -2, -3, -4}, 2});
+    auto Q = contract_many(ctx, cj);
+    AD.clear();
+    Ts.clear();
+    tags.clear();
+    for (size_t i = 0; i < n; ++i) { Ts.push_back(&Q[i]); tags.push_back(tag + 2); }
+    auto F3 = svd_trunc_many(ctx, Ts, chis, tags, sqrt_weights());
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)
+        cj.push_back(ContractJob{{&F3[i].V, {-1, -3, 1}}, {&F3[i].U, {1, -2, -4}},
+                                 {-1, -2, -3, -4}, 2});
+    auto Tn = contract_many(ctx, cj);
+    for (size_t i = 0; i < n; ++i) {
+        for (const Factor* f : {&F1[i], &F2[i], &F3[i]})
+            log_factor(*G[i], f->kept_min, f->dropped_max, f->iters, f->conv);
+        G[i]->T = std::move(Tn[i]);
+    }
+}
+
+// src/schemes/atrg.jl:35-43: half-step, T = permute(T, ((2, 4), (1, 3))),
+// half-step, T = permute(T, ((3, 1), (4, 2)))
+void step_atrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
+    for (auto* s : G) s->tlog.clear();
+    auto rotate = [&](std::vector<int> perm) {
+        std::vector<PermJob> pj;
+        for (auto* s : G) pj.push_back(PermJob{&s->T, perm, 2, false, {}});
+        auto R = permute_many(ctx, pj);
+        for (size_t i = 0; i < G.size(); ++i) G[i]->T = std::move(R[i]);
+    };
+    atrg_half(ctx, G, chis, 71);
+    rotate({1, 3, 0, 2});
+    atrg_half(ctx, G, chis, 74);
+    rotate({2, 0, 3, 1});
+}
+
 // ---------------------------------------------------------- ImpurityHOTRG
 // src/schemes/impurityhotrg.jl:48-84: projector from (T, T), then nine
 // placements of the tensors through it, averaged:
@@ -739,9 +832,13 @@ void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T) {
     }
 }
 
+bool single_tensor_kind(int kind) {
+    return kind == TNR_TRG || kind == TNR_BTRG || kind == TNR_HOTRG || kind == TNR_ATRG;
+}
+
 int scheme_tensors(int kind) {
     switch (kind) {
-        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: return 1;
+        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: case TNR_ATRG: return 1;
         case TNR_IMPURITY_TRG: return 5;
         case TNR_IMPURITY_HOTRG: return 4;
         default: return 0;
@@ -750,7 +847,7 @@ int scheme_tensors(int kind) {
 
 int scheme_components(int kind) {
     switch (kind) {
-        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: return 1;
+        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: case TNR_ATRG: return 1;
         case TNR_IMPURITY_TRG: return 2;
         case TNR_IMPURITY_HOTRG: return 4;
         default: return 0;
@@ -775,8 +872,7 @@ void scheme_init_multi(Ctx& ctx, Scheme& s, int kind, std::vector<BT>&& Ts) {
 }
 
 void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k) {
-    TNR_REQUIRE(kind == TNR_TRG || kind == TNR_BTRG || kind == TNR_HOTRG, TNR_ERR_ARG,
-                "unknown scheme kind");
+    TNR_REQUIRE(single_tensor_kind(kind), TNR_ERR_ARG, "unknown scheme kind");
     s.kind = kind;
     s.k = k;
     s.steps = 0;
@@ -784,7 +880,8 @@ void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k) {
 }
 
 void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<int>& chis) {
-    for (int kind : {TNR_TRG, TNR_BTRG, TNR_HOTRG, TNR_IMPURITY_TRG, TNR_IMPURITY_HOTRG}) {
+    for (int kind : {TNR_TRG, TNR_BTRG, TNR_HOTRG, TNR_ATRG, TNR_IMPURITY_TRG,
+                     TNR_IMPURITY_HOTRG}) {
         std::vector<Scheme*> g;
         std::vector<int> c;
         for (size_t i = 0; i < group.size(); ++i)
@@ -796,6 +893,7 @@ void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<in
         if (kind == TNR_TRG) step_trg(ctx, g, c);
         else if (kind == TNR_BTRG) step_btrg(ctx, g, c);
         else if (kind == TNR_HOTRG) step_hotrg(ctx, g, c);
+        else if (kind == TNR_ATRG) step_atrg(ctx, g, c);
         else if (kind == TNR_IMPURITY_TRG) step_impurity_trg(ctx, g, c);
         else step_impurity_hotrg(ctx, g, c);
         for (auto* s : g) s->steps++;
@@ -823,7 +921,7 @@ void finalize_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vecto
 void finalize_host(Ctx& ctx, Scheme& s, double* out) {
     Buf slots(&ctx.pool, 8 * sizeof(double));
     double* sl = slots.d();
-    if (s.kind == TNR_TRG || s.kind == TNR_BTRG || s.kind == TNR_HOTRG) {
+    if (single_tensor_kind(s.kind)) {
         finalize_many(ctx, {&s}, {sl});
         ctx.fetch(out, sl, sizeof(double));
         return;

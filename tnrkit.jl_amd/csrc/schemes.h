@@ -29,6 +29,8 @@ struct Scheme {
 void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k);
 // impurity schemes: Ts = {T, impurity tensors...}, all on T's spaces
 void scheme_init_multi(Ctx& ctx, Scheme& s, int kind, std::vector<BT>&& Ts);
+// TRG, BTRG, HOTRG, ATRG: one tensor, one norm per finalize!
+bool single_tensor_kind(int kind);
 int scheme_tensors(int kind);      // tensors a scheme holds (0: unknown kind)
 int scheme_components(int kind);   // values one finalize! returns
 // finalize! of one scheme with its values on the host (scheme_components)

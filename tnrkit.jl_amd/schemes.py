@@ -181,6 +181,17 @@ class HOTRG(TNRScheme):
         super().__init__(T, 0.0, precision)
 
 
+class ATRG(TNRScheme):
+    """ATRG(T) (src/schemes/atrg.jl:26-33): anisotropic TRG, one step = two
+    half-steps of three truncated SVDs (free energies with scalefactor 4,
+    as in test/schemes.jl:131-136)."""
+    kind = _lib.TNR_ATRG
+    scalefactor = 4.0
+
+    def __init__(self, T, precision=FP64):
+        super().__init__(T, 0.0, precision)
+
+
 class _MultiScheme(TNRScheme):
     """A scheme holding the pure tensor and impurity tensors on its spaces."""
     ntensors = 0

@@ -14,7 +14,8 @@ from . import _lib
 from ._lib import check, lib
 from .tensors import SymTensor
 
-KINDS = {"TRG": _lib.TNR_TRG, "BTRG": _lib.TNR_BTRG, "HOTRG": _lib.TNR_HOTRG}
+KINDS = {"TRG": _lib.TNR_TRG, "BTRG": _lib.TNR_BTRG, "HOTRG": _lib.TNR_HOTRG,
+         "ATRG": _lib.TNR_ATRG}
 
 
 class Request:
