@@ -356,6 +356,58 @@ def workload_obs(tk, reps, with_cpu):
             "cases": cases}
 
 
+def _cpu_atrg(chi, maxiter, budget_s):
+    from oracle import models as om
+    from oracle import schemes as osch
+    sc = osch.ATRG(om.classical_ising_z2(om.ising_betac_mp))
+    t0 = time.perf_counter()
+    sc.finalize()
+    last, steps = 0.0, 0
+    while steps < maxiter:
+        t1 = time.perf_counter()
+        sc.step(chi)
+        sc.finalize()
+        last = time.perf_counter() - t1
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    return (time.perf_counter() - t0) + (maxiter - steps) * last, steps
+
+
+def workload_atrg(tk, maxiter, batch, cpu_seconds):
+    """ATRG on Z2 Ising at beta_c (one run at a time, chi 24 / 32 / 48) and a
+    batched beta scan of Trivial ATRG runs at chi 24; the single-threaded
+    oracle on the chi-24 run beside it."""
+    cases = []
+    for chi in (24, 32, 48):
+        w = tk.ATRG(tk.classical_ising(symmetry="Z2"))
+        tk.run(w, tk.truncrank(chi), tk.maxiter(2), verbosity=0)
+        w.close()
+        sc = tk.ATRG(tk.classical_ising(symmetry="Z2"))
+        d, dt = _timed(lambda: tk.run(sc, tk.truncrank(chi), tk.maxiter(maxiter), verbosity=0))
+        sc.close()
+        f = tk.free_energy(d, tk.ising_betac, scalefactor=4.0)
+        c = {"chi": chi, "maxiter": maxiter, "s_per_run": dt, "steps_per_s": maxiter / dt,
+             "rel_df_vs_onsager": abs(f - tk.f_onsager) / abs(tk.f_onsager)}
+        if chi == 24 and cpu_seconds > 0:
+            import multiprocessing as mp
+            with mp.get_context("spawn").Pool(1) as pool:
+                cs, steps = pool.apply(_cpu_atrg, (chi, maxiter, cpu_seconds))
+            c["oracle_cpu_s_per_run"] = cs
+            c["oracle_cpu_steps_timed"] = steps
+            c["gpu_speedup_vs_oracle_1core"] = cs / dt
+        cases.append(c)
+    rng = np.random.default_rng(SEED + 7)
+    betas = rng.uniform(0.35, 0.55, batch)
+    reqs = [tk.Request("ATRG", tk.classical_ising(float(b), symmetry="Trivial"), 24, maxiter)
+            for b in betas]
+    tk.run_batch(reqs[:4])
+    _, dt = _timed(lambda: tk.run_batch(reqs))
+    return {"workload": "ATRG: Z2 Ising beta_c single runs; Trivial beta scan batch at chi 24",
+            "cases": cases, "batch": {"runs": batch, "chi": 24, "maxiter": maxiter,
+                                      "free_energies_per_s": batch / dt, "s": dt}}
+
+
 def workload_c4(tk, n, maxiter, check):
     """512 BTRG runs, beta ~ U[0.35, 0.55], chi ~ U{16, 24, 32, 48}."""
     rng = np.random.default_rng(SEED + 4)
@@ -451,9 +503,10 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrivals per second")
-    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs"], default="scan",
-                    help="scan = the headline beta scan; c2-c4, imp and obs print a report "
-                         "line of their own")
+    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs", "atrg"],
+                    default="scan",
+                    help="scan = the headline beta scan; c2-c4, imp, obs and atrg print a "
+                         "report line of their own")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -470,6 +523,9 @@ def main():
             r = workload_imp(tk, a.maxiter, 0.0 if a.no_cpu else a.cpu_seconds)
         elif a.workload == "obs":
             r = workload_obs(tk, 5, not a.no_cpu)
+        elif a.workload == "atrg":
+            r = workload_atrg(tk, a.maxiter, 64 if a.batch == 128 else a.batch,
+                              0.0 if a.no_cpu else a.cpu_seconds)
         else:
             r = workload_c4(tk, 512 if a.batch == 128 else a.batch, a.maxiter, a.check)
         print(json.dumps(r))
