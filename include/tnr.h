@@ -50,6 +50,7 @@ extern "C" {
 #define TNR_IMPURITY_TRG    3   /* src/schemes/impuritytrg.jl:40-103 */
 #define TNR_IMPURITY_HOTRG  4   /* src/schemes/impurityhotrg.jl:27-95 */
 #define TNR_ATRG  5   /* src/schemes/atrg.jl:26-76 */
+#define TNR_HOTRG_3D  6   /* src/schemes/hotrg3d.jl:27-141 (6-leg tensors) */
 
 #define TNR_PREC_FP64     0  /* all contractions FP64 MFMA */
 #define TNR_PREC_SPLIT32  1  /* contractions on FP32 MFMA, hi/lo split
@@ -87,7 +88,19 @@ int tnr_run(tnr_scheme* s, int chi, int maxiter, int finalize_beginning,
 int tnr_free_energy(const double* norms, int n, double beta,
                     double scalefactor, double initial_size, double* f_out);
 
-/* Current sector dimensions (4*zn int32) and element count of T. */
+/* HOTRG_3D(T) (src/schemes/hotrg3d.jl:27-34) for a 3D tensor
+ * T in V1 (x) V2 <- V3 (x) V4 (x) V5 (x) V6 (legs z, z', Y, X, y, x):
+ * leg_dims holds 6*zn sector dimensions, data the blocks per coupled charge
+ * c with rows over codomain pairs (a1, a2) and columns over domain 4-tuples
+ * (a3, a4, a5, a6), a3 + a4 + a5 + a6 = c, both colexicographic (first leg
+ * fastest).  tnr_step, tnr_finalize (n = |T[1 1; 2 3 2 3]|,
+ * src/utility/finalize.jl:62-66) and tnr_run work as for the 2D kinds;
+ * free energies use scalefactor 8. */
+int tnr_create_3d(tnr_scheme** out, int zn, const int32_t* leg_dims,
+                  const double* data, size_t ndata, int precision);
+/* Number of legs (4, or 6 for HOTRG_3D) and codomain legs of T. */
+int tnr_get_nlegs(const tnr_scheme* s, int* nleg, int* ncod);
+/* Current sector dimensions (nleg*zn int32) and element count of T. */
 int tnr_get_dims(const tnr_scheme* s, int32_t* leg_dims, size_t* ndata);
 /* Copy T out (format above) / replace T (checkpoint, resume, debugging). */
 int tnr_get_tensor(tnr_scheme* s, double* data, size_t ndata);

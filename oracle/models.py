@@ -161,3 +161,37 @@ def classical_clock_trivial(q, beta) -> ZT:
 def classical_clock_zn(q, beta) -> ZT:
     """src/models/clock.jl:32-49"""
     return _fourier_zn(clock_tensor(q, beta), q)
+
+
+# ------------------------------------------------------------ 3D Ising
+#: src/models/ising.jl:8 and test/schemes.jl:10 (Fig. 5 of PRB 102, 054432)
+ising_betac_3d = 1.0 / 4.51152469
+f_benchmark3d = -3.507
+
+
+def classical_ising_3d_z2(beta) -> ZT:
+    """src/models/ising.jl:149-165: t[i..n] = sum_a prod_k W[a, i_k] with
+    W = [[sqrt x, sqrt y], [sqrt x, -sqrt y]] (x = cosh b, y = sinh b) on
+    S^3 <- S^3, permuted to ((1, 4), (5, 6, 2, 3)); Z2 charges = index
+    parity on every leg (self-dual)."""
+    b = float(beta)
+    x, y = np.cosh(b), np.sinh(b)
+    W = np.array([[np.sqrt(x), np.sqrt(y)], [np.sqrt(x), -np.sqrt(y)]])
+    t = np.einsum("ai,aj,ak,al,am,an->ijklmn", W, W, W, W, W, W)
+    t = np.transpose(t, (0, 3, 4, 5, 1, 2))
+    qa = np.array([0, 1])
+    return ZT(t, [qa, qa, qa, qa, qa, qa], 2)
+
+
+def classical_ising_3d_trivial(beta) -> ZT:
+    """src/models/ising.jl:130-148: the spin delta O with q = sqrt of the
+    bond Boltzmann matrix [[e^K, e^-K], [e^-K, e^K]] on each of its six legs."""
+    K = float(beta)
+    t = np.array([[np.exp(K), np.exp(-K)], [np.exp(-K), np.exp(K)]])
+    w, v = np.linalg.eigh(t)
+    q = (v * np.sqrt(w)) @ v.T
+    O = np.zeros((2,) * 6)
+    O[(0,) * 6] = 1.0
+    O[(1,) * 6] = 1.0
+    o = np.einsum("abcdef,Aa,Bb,Cc,Dd,Ee,Ff->ABCDEF", O, q, q, q, q, q, q)
+    return ZT(o)

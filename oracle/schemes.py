@@ -237,6 +237,100 @@ def step_hotrg_x(A1: ZT, A2: ZT, Uy: ZT) -> ZT:
     return contract([-1, -2, -3, -4], (S, [-1, 3, -2, 4, -4], False), (Uy, [3, 4, -3], False))
 
 
+# ---------------------------------------------------------------- HOTRG_3D
+# tensors T[z_up z_down; Y X y x] (2|4); bosonic sectors, so the fermionic
+# twists of src/schemes/hotrg3d.jl:36-46 are identities
+
+def _mmdag_3d(A1: ZT, A2: ZT) -> ZT:
+    """src/schemes/hotrg3d.jl:48-65:
+    MM[x2 z z' x2'] := A2[z z2; Y2 X2 y2 x2] conj(A2[z' z2; Y2 X2 y2 x2'])
+    MM[x1 x2; x1' x2'] := MM[x2 z z' x2'] A1[z1 z; Y1 X1 y1 x1] conj(A1[z1 z'; Y1 X1 y1 x1'])"""
+    E2 = contract([-1, -2, -3, -4], (A2, [-2, 1, 2, 3, 4, -1], False), (A2, [-3, 1, 2, 3, 4, -4], True))
+    E1 = contract([-1, -2, -3, -4], (A1, [1, -1, 2, 3, 4, -2], False), (A1, [1, -3, 2, 3, 4, -4], True))
+    MM = contract([-1, -2, -3, -4], (E2, [-2, 5, 6, -4], False), (E1, [5, -1, 6, -3], False))
+    return project_hermitian(MM, 2)
+
+
+def _mdagm_3d(A1: ZT, A2: ZT) -> ZT:
+    """src/schemes/hotrg3d.jl:67-84:
+    MM[x2 z z' x2'] := conj(A2[z z2; Y2 x2 y2 X2]) A2[z' z2; Y2 x2' y2 X2]
+    MM[x1 x2; x1' x2'] := MM[x2 z z' x2'] conj(A1[z1 z; Y1 x1 y1 X1]) A1[z1 z'; Y1 x1' y1 X1]"""
+    E2 = contract([-1, -2, -3, -4], (A2, [-2, 1, 2, -1, 3, 4], True), (A2, [-3, 1, 2, -4, 3, 4], False))
+    E1 = contract([-1, -2, -3, -4], (A1, [1, -1, 2, -2, 3, 4], True), (A1, [1, -3, 2, -4, 3, 4], False))
+    MM = contract([-1, -2, -3, -4], (E2, [-2, 5, 6, -4], False), (E1, [5, -1, 6, -3], False))
+    return project_hermitian(MM, 2)
+
+
+def get_hotrg3d_xproj(A1: ZT, A2: ZT, chi, log=None):
+    """src/schemes/hotrg3d.jl:86-99: U of MM unless eps > eps' (then U')."""
+    MM = _mmdag_3d(A1, A2)
+    _, U, eps, info = eigh_trunc(MM, 2, chi)
+    MM2 = _mdagm_3d(A1, A2)
+    _, U2, eps2, info2 = eigh_trunc(MM2, 2, chi)
+    if log is not None:
+        log.append(info)
+        log.append(info2)
+    if eps > eps2:
+        return U2, eps2
+    return U, eps
+
+
+def get_hotrg3d_yproj(A1: ZT, A2: ZT, chi, log=None):
+    """src/schemes/hotrg3d.jl:101-107: the x projector of ((1, 2), (4, 3, 6, 5))."""
+    perm = (0, 1, 3, 2, 5, 4)
+    return get_hotrg3d_xproj(permute(A1, perm), permute(A2, perm), chi, log)
+
+
+def step_hotrg3d(A1: ZT, A2: ZT, Ux: ZT, Uy: ZT) -> ZT:
+    """src/schemes/hotrg3d.jl:109-119:
+    T[-1 -2; -3 -4 -5 -6] := conj(Ux[x1 x2; -6]) Ux[x1' x2'; -4] conj(Uy[y1 y2; -5])
+                             Uy[y1' y2'; -3] A1[-1 z; y1' x1' y1 x1] A2[z -2; y2' x2' y2 x2]
+    pairwise, at most eight legs at a time."""
+    # B1[a z y1' y1 x1 x2' n4] = A1[a z; y1' x1' y1 x1] Ux[x1' x2'; n4]
+    B1 = contract([-1, -2, -3, -4, -5, -6, -7], (A1, [-1, -2, -3, 1, -4, -5], False),
+                  (Ux, [1, -6, -7], False))
+    # B2[a z y1' y1 x2' n4 x2 n6] = B1 conj(Ux[x1 x2; n6])
+    B2 = contract([-1, -2, -3, -4, -5, -6, -7, -8], (B1, [-1, -2, -3, -4, 1, -5, -6], False),
+                  (Ux, [1, -7, -8], True))
+    # B3[a y1' y1 n4 n6 b y2' y2] = B2 A2[z b; y2' x2' y2 x2]
+    B3 = contract([-1, -2, -3, -4, -5, -6, -7, -8], (B2, [-1, 1, -2, -3, 2, -4, 3, -5], False),
+                  (A2, [1, -6, -7, 2, -8, 3], False))
+    # B4[a y1 n4 n6 b y2 n3] = B3 Uy[y1' y2'; n3]
+    B4 = contract([-1, -2, -3, -4, -5, -6, -7], (B3, [-1, 1, -2, -3, -4, -5, 2, -6], False),
+                  (Uy, [1, 2, -7], False))
+    # T[a b; n3 n4 n5 n6] = B4 conj(Uy[y1 y2; n5])
+    return contract([-1, -2, -3, -4, -5, -6], (B4, [-1, 1, -4, -6, -2, 2, -3], False),
+                    (Uy, [1, 2, -5], True))
+
+
+class HOTRG_3D:
+    """src/schemes/hotrg3d.jl:27-141: three sub-steps per step, each
+    compressing the z direction, the tensor rotated by ((6, 4), (2, 3, 1, 5))
+    after each."""
+    scalefactor = 8.0
+
+    def __init__(self, T: ZT):
+        self.T = T.copy()
+        self.log = []
+
+    def substep(self, chi):
+        Ux, _ = get_hotrg3d_xproj(self.T, self.T, chi, self.log)
+        Uy, _ = get_hotrg3d_yproj(self.T, self.T, chi, self.log)
+        self.T = step_hotrg3d(self.T, self.T, Ux, Uy)
+
+    def step(self, chi):
+        for _ in range(3):
+            self.substep(chi)
+            self.T = permute(self.T, (5, 3, 1, 2, 0, 4))
+        return self
+
+    def finalize(self):
+        """src/utility/finalize.jl:62-66: n = |T[1 1; 2 3 2 3]|, T /= n."""
+        n = abs(float(np.einsum("aabcbc->", self.T.data)))
+        self.T.data /= n
+        return n
+
+
 # ---------------------------------------------------------------- impurities
 
 def _trg_net(D, B, C, A) -> ZT:

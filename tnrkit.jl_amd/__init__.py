@@ -12,10 +12,11 @@ The directory name is not a valid Python identifier; load it with
 ``__graft_entry__.package()`` (it registers the module as ``tnrkit_jl_amd``).
 """
 from ._lib import TNRError, lib  # noqa: F401
-from .models import (classical_clock, classical_ising, classical_ising_impurity,  # noqa: F401
+from .models import (classical_clock, classical_ising, classical_ising_3D,  # noqa: F401
+                     classical_ising_impurity, f_benchmark3D, ising_betac_3D,
                      classical_potts, f_onsager,
                      ising_betac, ising_cft_exact, potts_betac)
-from .schemes import (ATRG, BTRG, FP64, HOTRG, SPLIT32, TRG, Finalizer, ImpurityHOTRG,  # noqa: F401
+from .schemes import (ATRG, BTRG, FP64, HOTRG, HOTRG_3D, SPLIT32, TRG, Finalizer, ImpurityHOTRG,  # noqa: F401
                       ImpurityTRG, MultipleCrit,
                       convcrit, default_Finalizer, finalize, free_energy, maxiter, run, step,
                       trivial_convcrit, truncrank)
@@ -25,4 +26,4 @@ from .observables import (GSDegeneracy_Finalizer, central_charge, cft_data,  # n
                           gu_wen_ratio, guwenratio_Finalizer, transfer_spectrum,
                           two_by_two_Finalizer)
 from .serving import Request, Server, run_batch  # noqa: F401
-from .tensors import SymTensor, trivial  # noqa: F401
+from .tensors import NTensor, SymTensor, trivial  # noqa: F401

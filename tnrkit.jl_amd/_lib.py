@@ -22,6 +22,7 @@ TNR_ERR_BUSY = -10
 TNR_TRG, TNR_BTRG, TNR_HOTRG = 0, 1, 2
 TNR_IMPURITY_TRG, TNR_IMPURITY_HOTRG = 3, 4
 TNR_ATRG = 5
+TNR_HOTRG_3D = 6
 TNR_PREC_FP64, TNR_PREC_SPLIT32 = 0, 1
 
 
@@ -81,6 +82,8 @@ SIGNATURES = {
     "tnr_kernel_svd": (C.c_int, [C.c_int, C.c_int, _DP, C.c_int, _DP, _DP, _DP, _IP]),
     "tnr_kernel_pivoted_gauge": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP]),
     "tnr_kernel_eig": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP]),
+    "tnr_create_3d": (C.c_int, [_PP, C.c_int, C.POINTER(C.c_int32), _DP, C.c_size_t, C.c_int]),
+    "tnr_get_nlegs": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "tnr_transfer_spectrum": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP, C.c_size_t,
                                         C.POINTER(C.c_size_t)]),
     "tnr_cft_data": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, _DP, _DP, C.c_size_t,

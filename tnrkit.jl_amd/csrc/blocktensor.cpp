@@ -244,9 +244,8 @@ static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t t
             if (d.sstr[j] == 1) s0 = j;
             if (d.dstr[j] == 1) d0 = j;
         }
-        const int64_t cnt = (int64_t)d.dim[0] * (d.nd > 1 ? d.dim[1] : 1) *
-                            (d.nd > 2 ? d.dim[2] : 1) * (d.nd > 3 ? d.dim[3] : 1) *
-                            (d.nd > 4 ? d.dim[4] : 1) * (d.nd > 5 ? d.dim[5] : 1);
+        int64_t cnt = 1;
+        for (int j = 0; j < d.nd; ++j) cnt *= d.dim[j];
         const bool weighted = d.wleg[0] >= 0 || d.wleg[1] >= 0;
         if (!weighted && s0 >= 0 && d0 >= 0 && s0 != d0 && d.dim[s0] >= 8 && d.dim[d0] >= 8 &&
             d.nd - 2 <= 4) {
@@ -734,12 +733,42 @@ void trace_normalize_many(Ctx& ctx, const std::vector<TraceJob>& jobs) {
         BT& t = *jobs[x].t;
         const Layout& L = t.L;
         const int N = L.N;
-        TNR_REQUIRE(L.nleg == 4 && L.ncod == 2, TNR_ERR_INTERNAL, "finalize needs a (2,2) tensor");
-        TNR_REQUIRE(L.legs[0] == L.legs[3].dual(N) && L.legs[1] == L.legs[2].dual(N),
-                    TNR_ERR_ARG, "legs 1/4 and 2/3 must carry the same spaces");
         TraceProb P;
         P.first = (int)descs.size();
         int64_t start = 0;
+        if (L.nleg == 6) {
+            // n = sum T[a a; b c b c] (src/utility/finalize.jl:62-66), no weights
+            TNR_REQUIRE(L.ncod == 2 && L.legs[1] == L.legs[0].dual(N) &&
+                            L.legs[4] == L.legs[2].dual(N) && L.legs[5] == L.legs[3].dual(N),
+                        TNR_ERR_ARG, "legs 1/2, 3/5 and 4/6 must carry the same spaces");
+            TNR_REQUIRE(!jobs[x].w_x1 && !jobs[x].w_x2, TNR_ERR_INTERNAL, "weighted 3D trace");
+            for (int q0 = 0; q0 < N; ++q0)
+                for (int q2 = 0; q2 < N; ++q2)
+                    for (int q3 = 0; q3 < N; ++q3) {
+                        int q[6] = {q0, (N - q0) % N, q2, q3, (N - q2) % N, (N - q3) % N};
+                        if (!L.legs[0].d[q0] || !L.legs[2].d[q2] || !L.legs[3].d[q3]) continue;
+                        int64_t st[6];
+                        int64_t base = subblock(L, q, st);
+                        TraceDesc d;
+                        d.T = t.d() + base;
+                        d.s1 = st[0] + st[1];
+                        d.s2 = st[2] + st[4];
+                        d.s3 = st[3] + st[5];
+                        d.d1 = L.legs[0].d[q0];
+                        d.d2 = L.legs[2].d[q2];
+                        d.d3 = L.legs[3].d[q3];
+                        TNR_REQUIRE(base + (int64_t)(d.d1 - 1) * d.s1 + (int64_t)(d.d2 - 1) * d.s2 +
+                                            (int64_t)(d.d3 - 1) * d.s3 < L.size,
+                                    TNR_ERR_INTERNAL, "trace descriptor out of bounds");
+                        d.w1 = d.w2 = nullptr;
+                        d.start = start;
+                        start += (int64_t)d.d1 * d.d2 * d.d3;
+                        descs.push_back(d);
+                    }
+        } else {
+        TNR_REQUIRE(L.nleg == 4 && L.ncod == 2, TNR_ERR_INTERNAL, "finalize needs a (2,2) tensor");
+        TNR_REQUIRE(L.legs[0] == L.legs[3].dual(N) && L.legs[1] == L.legs[2].dual(N),
+                    TNR_ERR_ARG, "legs 1/4 and 2/3 must carry the same spaces");
         for (int q1 = 0; q1 < N; ++q1)
             for (int q2 = 0; q2 < N; ++q2) {
                 int q[4] = {q1, q2, (N - q2) % N, (N - q1) % N};
@@ -757,10 +786,13 @@ void trace_normalize_many(Ctx& ctx, const std::vector<TraceJob>& jobs) {
                 d.d2 = L.legs[1].d[q2];
                 d.w1 = jobs[x].w_x1 ? jobs[x].w_x1->w + jobs[x].w_x1->off[q1] : nullptr;
                 d.w2 = jobs[x].w_x2 ? jobs[x].w_x2->w + jobs[x].w_x2->off[q2] : nullptr;
+                d.s3 = 0;
+                d.d3 = 1;
                 d.start = start;
                 start += (int64_t)d.d1 * d.d2;
                 descs.push_back(d);
             }
+        }
         TNR_REQUIRE(start > 0, TNR_ERR_ARG, "empty tensor");
         P.count = (int)descs.size() - P.first;
         P.total = start;
@@ -854,32 +886,35 @@ void readback_vectors(Ctx& ctx, const std::vector<const double*>& src,
 }
 
 // ---------------------------------------------------------- ABI transfer
-Layout abi_layout(int N, const int32_t* leg_dims) {
+Layout abi_layout(int N, const int32_t* leg_dims, int nleg, int ncod) {
     TNR_REQUIRE(N >= 1 && N <= MAXN, TNR_ERR_ARG, "zn must be in 1..8");
-    std::vector<Leg> legs(4);
-    for (int i = 0; i < 4; ++i)
+    TNR_REQUIRE(nleg >= 2 && nleg <= MAXLEG && ncod >= 1 && ncod < nleg, TNR_ERR_ARG,
+                "bad leg counts");
+    std::vector<Leg> legs(nleg);
+    for (int i = 0; i < nleg; ++i)
         for (int a = 0; a < N; ++a) {
             int d = leg_dims[i * N + a];
             TNR_REQUIRE(d >= 0, TNR_ERR_ARG, "negative sector dimension");
             // codomain legs: q = a; domain legs: q = -a
-            int q = (i < 2) ? a : (N - a) % N;
+            int q = (i < ncod) ? a : (N - a) % N;
             legs[i].d[q] = d;
         }
-    return Layout(N, 2, legs);
+    return Layout(N, ncod, legs);
 }
 
 void abi_dims(const Layout& L, int32_t* leg_dims) {
     const int N = L.N;
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < L.nleg; ++i)
         for (int a = 0; a < N; ++a) {
-            int q = (i < 2) ? a : (N - a) % N;
+            int q = (i < L.ncod) ? a : (N - a) % N;
             leg_dims[i * N + a] = L.legs[i].d[q];
         }
 }
 
-BT upload(Ctx& ctx, int N, const int32_t* leg_dims, const double* data, size_t n) {
+BT upload(Ctx& ctx, int N, const int32_t* leg_dims, const double* data, size_t n, int nleg,
+          int ncod) {
     TNR_REQUIRE(leg_dims && data, TNR_ERR_ARG, "null pointer");
-    Layout L = abi_layout(N, leg_dims);
+    Layout L = abi_layout(N, leg_dims, nleg, ncod);
     TNR_REQUIRE((int64_t)n == L.size, TNR_ERR_ARG,
                 "ndata = " + std::to_string(n) + " does not match the sector dimensions (" +
                     std::to_string(L.size) + ")");

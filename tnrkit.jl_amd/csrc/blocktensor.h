@@ -220,11 +220,12 @@ void gather_many(Ctx& ctx, std::vector<GatherDesc> descs);
 void readback_vectors(Ctx& ctx, const std::vector<const double*>& src,
                       const std::vector<int>& cnt, const std::vector<std::vector<double>*>& dst);
 
-// Upload / download in the ABI format (ncod = 2, 4 legs, sector dims per
-// space label).
-Layout abi_layout(int N, const int32_t* leg_dims);
-BT upload(Ctx& ctx, int N, const int32_t* leg_dims, const double* data, size_t n);
+// Upload / download in the ABI format (sector dims per space label, first
+// ncod legs in the codomain; 2D tensors have 4 legs, 3D tensors 6).
+Layout abi_layout(int N, const int32_t* leg_dims, int nleg = 4, int ncod = 2);
+BT upload(Ctx& ctx, int N, const int32_t* leg_dims, const double* data, size_t n, int nleg = 4,
+          int ncod = 2);
 void download(Ctx& ctx, const BT& t, double* data);
-void abi_dims(const Layout& L, int32_t* leg_dims);
+void abi_dims(const Layout& L, int32_t* leg_dims);     // nleg * N values
 
 }  // namespace tnr

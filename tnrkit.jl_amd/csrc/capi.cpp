@@ -90,8 +90,9 @@ int tnr_create(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims, cons
     return guarded([&] {
         if (!out) return fail(TNR_ERR_ARG, "out is null");
         *out = nullptr;
-        if (!single_tensor_kind(kind))
-            return fail(TNR_ERR_ARG, "kind must be TNR_TRG, TNR_BTRG, TNR_HOTRG or TNR_ATRG");
+        if (!single_tensor_kind(kind) || kind == TNR_HOTRG_3D)
+            return fail(TNR_ERR_ARG, "kind must be TNR_TRG, TNR_BTRG, TNR_HOTRG or TNR_ATRG "
+                                     "(TNR_HOTRG_3D: tnr_create_3d)");
         if (zn < 1 || zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
         if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
             return fail(TNR_ERR_ARG, "unknown precision");
@@ -102,6 +103,33 @@ int tnr_create(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims, cons
         BT T = upload(*h->ctx, zn, leg_dims, data, ndata);
         scheme_init(*h->ctx, h->s, kind, std::move(T), k);
         *out = h.release();
+        return TNR_OK;
+    });
+}
+
+int tnr_create_3d(tnr_scheme** out, int zn, const int32_t* leg_dims, const double* data,
+                  size_t ndata, int precision) {
+    return guarded([&] {
+        if (!out) return fail(TNR_ERR_ARG, "out is null");
+        *out = nullptr;
+        if (zn < 1 || zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
+        if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
+            return fail(TNR_ERR_ARG, "unknown precision");
+        std::unique_ptr<tnr_scheme> h(new tnr_scheme());
+        h->ctx.reset(new Ctx());
+        h->ctx->precision = precision;
+        BT T = upload(*h->ctx, zn, leg_dims, data, ndata, 6, 2);
+        scheme_init(*h->ctx, h->s, TNR_HOTRG_3D, std::move(T), 0.0);
+        *out = h.release();
+        return TNR_OK;
+    });
+}
+
+int tnr_get_nlegs(const tnr_scheme* s, int* nleg, int* ncod) {
+    return guarded([&] {
+        if (!s) return fail(TNR_ERR_ARG, "null handle");
+        if (nleg) *nleg = s->s.T.L.nleg;
+        if (ncod) *ncod = s->s.T.L.ncod;
         return TNR_OK;
     });
 }
@@ -223,7 +251,7 @@ int tnr_central_charge(tnr_scheme* s, double n, double* c) {
 int tnr_finalize_two_by_two(tnr_scheme* s, double* out) {
     return guarded([&] {
         if (!s || !out) return fail(TNR_ERR_ARG, "null pointer");
-        if (!single_tensor_kind(s->s.kind))
+        if (!single_tensor_kind(s->s.kind) || s->s.kind == TNR_HOTRG_3D)
             return fail(TNR_ERR_STATE,
                         "finalize_two_by_two! is defined for TRG, BTRG, HOTRG and ATRG");
         *out = finalize_two_by_two(*s->ctx, s->s);
@@ -321,7 +349,7 @@ int tnr_get_tensor(tnr_scheme* s, double* data, size_t ndata) {
 int tnr_set_tensor(tnr_scheme* s, const int32_t* leg_dims, const double* data, size_t ndata) {
     return guarded([&] {
         if (!s) return fail(TNR_ERR_ARG, "null handle");
-        BT T = upload(*s->ctx, s->s.N, leg_dims, data, ndata);
+        BT T = upload(*s->ctx, s->s.N, leg_dims, data, ndata, s->s.T.L.nleg, s->s.T.L.ncod);
         scheme_set_tensor(*s->ctx, s->s, std::move(T));
         return TNR_OK;
     });
@@ -606,7 +634,8 @@ int tnr_server_create(tnr_server** out, size_t hbm_budget_bytes, int max_slots, 
 int tnr_server_submit(tnr_server* s, const tnr_run_desc* req, int64_t* id) {
     return guarded([&] {
         if (!s || !req || !id) return fail(TNR_ERR_ARG, "null pointer");
-        if (!single_tensor_kind(req->kind)) return fail(TNR_ERR_ARG, "unknown scheme kind");
+        if (!single_tensor_kind(req->kind) || req->kind == TNR_HOTRG_3D)
+            return fail(TNR_ERR_ARG, "unknown scheme kind (HOTRG_3D runs through tnr_create_3d)");
         if (req->zn < 1 || req->zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
         if (req->chi < 1) return fail(TNR_ERR_ARG, "chi must be >= 1");
         if (!req->leg_dims || !req->data) return fail(TNR_ERR_ARG, "null tensor");

@@ -16,7 +16,7 @@
 namespace tnr {
 
 constexpr int MAXN = 8;      // largest Z_N order supported
-constexpr int MAXLEG = 6;    // largest tensor rank (HOTRG intermediates have 5)
+constexpr int MAXLEG = 8;    // largest tensor rank (HOTRG_3D intermediates have 8)
 constexpr int LMAX = 96;     // largest Rayleigh-Ritz block / direct-solve size
 
 struct Error : std::runtime_error {

@@ -59,12 +59,14 @@ struct GemmDesc {
 
 // Weighted partial trace sum_{x1,x2} T[x1*s1 + x2*s2] w1[x1] w2[x2] over one
 // sub-block; problem `prob` owns descriptors [first, first + count).
+// sum_{x1, x2, x3} T[x1 s1 + x2 s2 + x3 s3] w1[x1] w2[x2] over one sub-block
+// (d3 = 1 for the 2D traces, 3 index pairs for the 3D one)
 struct TraceDesc {
     const double* T;
     const double* w1;             // already offset, or null
     const double* w2;
-    int64_t s1, s2;
-    int32_t d1, d2;
+    int64_t s1, s2, s3;
+    int32_t d1, d2, d3;
     int64_t start;                // prefix within its problem
 };
 struct TraceProb {

@@ -513,8 +513,10 @@ __global__ __launch_bounds__(256) void k_trace_partial(const TraceDesc* __restri
         }
         const TraceDesc& D = ds[lo];
         int64_t r = e - D.start;
-        int x1 = (int)(r % D.d1), x2 = (int)(r / D.d1);
-        double v = D.T[x1 * D.s1 + x2 * D.s2];
+        const int x1 = (int)(r % D.d1);
+        r /= D.d1;
+        const int x2 = (int)(r % D.d2), x3 = (int)(r / D.d2);
+        double v = D.T[x1 * D.s1 + x2 * D.s2 + x3 * D.s3];
         if (D.w1) v *= D.w1[x1];
         if (D.w2) v *= D.w2[x2];
         acc += v;

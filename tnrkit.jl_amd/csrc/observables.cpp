@@ -76,6 +76,7 @@ BT unit_tensor(Ctx& ctx, const Scheme& s) {
 // Transfer matrix of `unitcell` tensors in a row, periodic (cft.jl:7-16,
 // 280-295): M[(b_1..b_u), (t_1..t_u)] = sum T[a_1 b_1 t_1 a_2] ... T[a_u b_u t_u a_1]
 BT transfer_matrix(Ctx& ctx, const Scheme& s, int unitcell) {
+    TNR_REQUIRE(s.T.L.nleg == 4, TNR_ERR_ARG, "observables are defined for 2D (4-leg) tensors");
     TNR_REQUIRE(unitcell == 1 || unitcell == 2, TNR_ERR_ARG,
                 "unitcell must be 1 or 2 (tensors of at most six legs)");
     const BT& T = s.T;
@@ -189,6 +190,7 @@ double ground_state_degeneracy(Ctx& ctx, const Scheme& s, int unitcell) {
 }
 
 void gu_wen_ratio(Ctx& ctx, const Scheme& s, double* x1, double* x2) {
+    TNR_REQUIRE(s.T.L.nleg == 4, TNR_ERR_ARG, "observables are defined for 2D (4-leg) tensors");
     // cft.jl:374-383 on T (BTRG: T_unit, cft.jl:386-387)
     BT Tu = unit_tensor(ctx, s);
     const int N = Tu.L.N;
@@ -211,6 +213,7 @@ void gu_wen_ratio(Ctx& ctx, const Scheme& s, double* x1, double* x2) {
 }
 
 double central_charge(Ctx& ctx, const Scheme& s, double n) {
+    TNR_REQUIRE(s.T.L.nleg == 4, TNR_ERR_ARG, "observables are defined for 2D (4-leg) tensors");
     // M[-1; -2] = T[1 -1; -2 1] (cft.jl:257), BTRG T[1 -1; 3 2] S1[3; -2]
     // S2[2; 1] (cft.jl:263-266); the largest singular value of the first
     // non-empty block (S.data[1]) of M / n
@@ -235,6 +238,7 @@ double central_charge(Ctx& ctx, const Scheme& s, double n) {
 }
 
 double finalize_two_by_two(Ctx& ctx, Scheme& s) {
+    TNR_REQUIRE(s.T.L.nleg == 4, TNR_ERR_ARG, "observables are defined for 2D (4-leg) tensors");
     // n = |T[7 1; 5 4] T[4 2; 6 7] T[3 6; 2 8] T[8 5; 1 3]| (finalize.jl:18-21),
     // BTRG with S2 and S1 on the eight inner bonds (finalize.jl:29-38): as
     // P[1,5,2,6] = Ta Tb, Q[6,2,5,1] = Tc Td and sum P[a,b,c,d] Q[d,c,b,a]

@@ -744,6 +744,122 @@ void step_atrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& 
     rotate({2, 0, 3, 1});
 }
 
+// ------------------------------------------------------------- HOTRG_3D
+// Tensors T[z z'; Y X y x] (2|4); bosonic sectors, so the fermionic twists of
+// src/schemes/hotrg3d.jl:36-46 are identities.
+// Projector of the x pair (hotrg3d.jl:86-99) of each tensor A (= A1 = A2;
+// for the y pair A is permuted by ((1, 2), (4, 3, 6, 5)) first, :101-107):
+//   E2[x2 z z' x2'] = A2[z z2; Y2 X2 y2 x2] conj(A2[z' z2; Y2 X2 y2 x2'])
+//   E1[z x1 z' x1'] = A1[z1 z; Y1 X1 y1 x1] conj(A1[z1 z'; Y1 X1 y1 x1'])
+//   MM[x1 x2; x1' x2'] = E2[x2 z z' x2'] E1[z x1 z' x1']        (:48-65)
+// and MM' from the conjugated pair with x and X exchanged (:67-84); U of MM
+// unless eps > eps'.
+std::vector<BT> hotrg3d_proj(Ctx& ctx, const std::vector<const BT*>& As, const std::vector<Scheme*>& G,
+                             const std::vector<int>& chis, int tag) {
+    const size_t n = As.size();
+    std::vector<ContractJob> cj;
+    for (size_t i = 0; i < n; ++i) {
+        const BT* A = As[i];
+        cj.push_back(ContractJob{{A, {-2, 1, 2, 3, 4, -1}}, {A, {-3, 1, 2, 3, 4, -4}, true},
+                                 {-1, -2, -3, -4}, 2});
+        cj.push_back(ContractJob{{A, {1, -1, 2, 3, 4, -2}}, {A, {1, -3, 2, 3, 4, -4}, true},
+                                 {-1, -2, -3, -4}, 2});
+        cj.push_back(ContractJob{{A, {-2, 1, 2, -1, 3, 4}, true}, {A, {-3, 1, 2, -4, 3, 4}},
+                                 {-1, -2, -3, -4}, 2});
+        cj.push_back(ContractJob{{A, {1, -1, 2, -2, 3, 4}, true}, {A, {1, -3, 2, -4, 3, 4}},
+                                 {-1, -2, -3, -4}, 2});
+    }
+    auto E = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 2; ++k)
+            cj.push_back(ContractJob{{&E[4 * i + 2 * k], {-2, 5, 6, -4}},
+                                     {&E[4 * i + 2 * k + 1], {5, -1, 6, -3}}, {-1, -2, -3, -4}, 2});
+    auto MM = contract_many(ctx, cj);
+    E.clear();
+    std::vector<BT*> Ms;
+    std::vector<int> ch;
+    for (size_t i = 0; i < n; ++i) {
+        Ms.push_back(&MM[2 * i]);
+        Ms.push_back(&MM[2 * i + 1]);
+        ch.push_back(chis[i]);
+        ch.push_back(chis[i]);
+    }
+    auto F = eigh_trunc_many(ctx, Ms, ch, tag);
+    std::vector<BT> U;
+    for (size_t i = 0; i < n; ++i) {
+        EFactor& a = F[2 * i];
+        EFactor& b = F[2 * i + 1];
+        log_factor(*G[i], a.kept_min, a.dropped_max, a.iters, a.conv);
+        log_factor(*G[i], b.kept_min, b.dropped_max, b.iters, b.conv);
+        U.push_back(a.eps > b.eps ? std::move(b.U) : std::move(a.U));
+    }
+    return U;
+}
+
+// one z compression (hotrg3d.jl:121-126) for every scheme:
+//   T[-1 -2; -3 -4 -5 -6] := conj(Ux[x1 x2; -6]) Ux[x1' x2'; -4] conj(Uy[y1 y2; -5])
+//        Uy[y1' y2'; -3] A1[-1 z; y1' x1' y1 x1] A2[z -2; y2' x2' y2 x2]
+// (hotrg3d.jl:109-119) pairwise, at most eight legs at a time
+void hotrg3d_substep(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
+    const size_t n = G.size();
+    std::vector<const BT*> As;
+    std::vector<PermJob> pj;
+    for (auto* s : G) {
+        As.push_back(&s->T);
+        pj.push_back(PermJob{&s->T, {0, 1, 3, 2, 5, 4}, 2, false, {}});
+    }
+    auto Ux = hotrg3d_proj(ctx, As, G, chis, 81);
+    auto Ay = permute_many(ctx, pj);
+    std::vector<const BT*> Ays;
+    for (auto& a : Ay) Ays.push_back(&a);
+    auto Uy = hotrg3d_proj(ctx, Ays, G, chis, 82);
+    Ay.clear();
+    std::vector<ContractJob> cj;
+    for (size_t i = 0; i < n; ++i)   // B1[a z y1' y1 x1 x2' n4]
+        cj.push_back(ContractJob{{&G[i]->T, {-1, -2, -3, 1, -4, -5}}, {&Ux[i], {1, -6, -7}},
+                                 {-1, -2, -3, -4, -5, -6, -7}, 2});
+    auto B1 = contract_many(ctx, cj);
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // B2[a z y1' y1 x2' n4 x2 n6]
+        cj.push_back(ContractJob{{&B1[i], {-1, -2, -3, -4, 1, -5, -6}}, {&Ux[i], {1, -7, -8}, true},
+                                 {-1, -2, -3, -4, -5, -6, -7, -8}, 2});
+    auto B2 = contract_many(ctx, cj);
+    B1.clear();
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // B3[a y1' y1 n4 n6 b y2' y2]
+        cj.push_back(ContractJob{{&B2[i], {-1, 1, -2, -3, 2, -4, 3, -5}},
+                                 {&G[i]->T, {1, -6, -7, 2, -8, 3}},
+                                 {-1, -2, -3, -4, -5, -6, -7, -8}, 2});
+    auto B3 = contract_many(ctx, cj);
+    B2.clear();
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // B4[a y1 n4 n6 b y2 n3]
+        cj.push_back(ContractJob{{&B3[i], {-1, 1, -2, -3, -4, -5, 2, -6}}, {&Uy[i], {1, 2, -7}},
+                                 {-1, -2, -3, -4, -5, -6, -7}, 2});
+    auto B4 = contract_many(ctx, cj);
+    B3.clear();
+    cj.clear();
+    for (size_t i = 0; i < n; ++i)   // T[a b; n3 n4 n5 n6]
+        cj.push_back(ContractJob{{&B4[i], {-1, 1, -4, -6, -2, 2, -3}}, {&Uy[i], {1, 2, -5}, true},
+                                 {-1, -2, -3, -4, -5, -6}, 2});
+    auto Tn = contract_many(ctx, cj);
+    for (size_t i = 0; i < n; ++i) G[i]->T = std::move(Tn[i]);
+}
+
+// hotrg3d.jl:128-136: three z compressions, T permuted by ((6, 4), (2, 3, 1, 5))
+// after each
+void step_hotrg3d(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& chis) {
+    for (auto* s : G) s->tlog.clear();
+    for (int k = 0; k < 3; ++k) {
+        hotrg3d_substep(ctx, G, chis);
+        std::vector<PermJob> pj;
+        for (auto* s : G) pj.push_back(PermJob{&s->T, {5, 3, 1, 2, 0, 4}, 2, false, {}});
+        auto R = permute_many(ctx, pj);
+        for (size_t i = 0; i < G.size(); ++i) G[i]->T = std::move(R[i]);
+    }
+}
+
 // ---------------------------------------------------------- ImpurityHOTRG
 // src/schemes/impurityhotrg.jl:48-84: projector from (T, T), then nine
 // placements of the tensors through it, averaged:
@@ -819,6 +935,16 @@ int gauge_clusters(const double* sv, int k, std::vector<int>& cl, int& knn) {
 
 // ------------------------------------------------------------------ public
 void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T) {
+    if (s.kind == TNR_HOTRG_3D) {
+        const int N = T.L.N;
+        TNR_REQUIRE(T.L.nleg == 6 && T.L.ncod == 2, TNR_ERR_ARG, "3D tensors have 2+4 legs");
+        TNR_REQUIRE(T.L.legs[1] == T.L.legs[0].dual(N) && T.L.legs[4] == T.L.legs[2].dual(N) &&
+                        T.L.legs[5] == T.L.legs[3].dual(N),
+                    TNR_ERR_ARG, "legs 1/2, 3/5 and 4/6 must carry the same spaces");
+        s.N = N;
+        s.T = std::move(T);
+        return;
+    }
     TNR_REQUIRE(T.L.nleg == 4 && T.L.ncod == 2, TNR_ERR_ARG, "scheme tensors have 2+2 legs");
     TNR_REQUIRE(T.L.legs[0] == T.L.legs[3].dual(T.L.N) && T.L.legs[1] == T.L.legs[2].dual(T.L.N),
                 TNR_ERR_ARG, "legs 1/4 and 2/3 must carry the same spaces");
@@ -833,12 +959,13 @@ void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T) {
 }
 
 bool single_tensor_kind(int kind) {
-    return kind == TNR_TRG || kind == TNR_BTRG || kind == TNR_HOTRG || kind == TNR_ATRG;
+    return kind == TNR_TRG || kind == TNR_BTRG || kind == TNR_HOTRG || kind == TNR_ATRG ||
+           kind == TNR_HOTRG_3D;
 }
 
 int scheme_tensors(int kind) {
     switch (kind) {
-        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: case TNR_ATRG: return 1;
+        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: case TNR_ATRG: case TNR_HOTRG_3D: return 1;
         case TNR_IMPURITY_TRG: return 5;
         case TNR_IMPURITY_HOTRG: return 4;
         default: return 0;
@@ -847,7 +974,7 @@ int scheme_tensors(int kind) {
 
 int scheme_components(int kind) {
     switch (kind) {
-        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: case TNR_ATRG: return 1;
+        case TNR_TRG: case TNR_BTRG: case TNR_HOTRG: case TNR_ATRG: case TNR_HOTRG_3D: return 1;
         case TNR_IMPURITY_TRG: return 2;
         case TNR_IMPURITY_HOTRG: return 4;
         default: return 0;
@@ -880,7 +1007,7 @@ void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k) {
 }
 
 void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<int>& chis) {
-    for (int kind : {TNR_TRG, TNR_BTRG, TNR_HOTRG, TNR_ATRG, TNR_IMPURITY_TRG,
+    for (int kind : {TNR_TRG, TNR_BTRG, TNR_HOTRG, TNR_ATRG, TNR_HOTRG_3D, TNR_IMPURITY_TRG,
                      TNR_IMPURITY_HOTRG}) {
         std::vector<Scheme*> g;
         std::vector<int> c;
@@ -894,6 +1021,7 @@ void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<in
         else if (kind == TNR_BTRG) step_btrg(ctx, g, c);
         else if (kind == TNR_HOTRG) step_hotrg(ctx, g, c);
         else if (kind == TNR_ATRG) step_atrg(ctx, g, c);
+        else if (kind == TNR_HOTRG_3D) step_hotrg3d(ctx, g, c);
         else if (kind == TNR_IMPURITY_TRG) step_impurity_trg(ctx, g, c);
         else step_impurity_hotrg(ctx, g, c);
         for (auto* s : g) s->steps++;
@@ -975,6 +1103,7 @@ size_t scheme_hbm_estimate(int kind, int N, int chi) {
     const double c5 = std::pow((double)chi, 5) / std::max(1, N) * 8.0;
     double b = 8.0 * c4 + 4.0 * (double)chi * chi * LMAX * 8.0 * 6.0;
     if (kind == TNR_HOTRG) b += 3.0 * c5;
+    if (kind == TNR_HOTRG_3D) b += 4.0 * std::pow((double)chi, 8) / std::max(1, N) * 8.0;
     return (size_t)b + (size_t(8) << 20);
 }
 

@@ -11,7 +11,7 @@ from decimal import Decimal, getcontext
 
 import numpy as np
 
-from .tensors import SymTensor, trivial
+from .tensors import NTensor, SymTensor, trivial
 
 getcontext().prec = 50
 #: src/models/ising.jl:1 (beta_c = ln(1 + sqrt 2) / 2)
@@ -154,3 +154,36 @@ def classical_clock(q, beta, symmetry="ZN"):
     if symmetry == "Trivial":
         return trivial(A)
     return _fourier(A, q)
+
+
+# ------------------------------------------------------------ 3D Ising
+#: src/models/ising.jl:8 (beta_c of the 3D Ising model)
+ising_betac_3D = 1.0 / 4.51152469
+#: test/schemes.jl:10 (Fig. 5 of PRB 102, 054432): the free energy at beta_c
+f_benchmark3D = -3.507
+
+
+def classical_ising_3D(beta=None, symmetry="Z2"):
+    """3D Ising tensor T[z z'; Y X y x] (2 + 4 legs) for HOTRG_3D.
+
+    "Z2" follows src/models/ising.jl:149-165 (t = sum_a prod W[a, i_k], W =
+    [[sqrt x, sqrt y], [sqrt x, -sqrt y]], permuted to ((1, 4), (5, 6, 2, 3));
+    Z2 charge = index on every leg), "Trivial" :130-148 (the spin delta with
+    the square root of the bond Boltzmann matrix on each leg)."""
+    b = ising_betac_3D if beta is None else float(beta)
+    if symmetry == "Z2":
+        x, y = np.cosh(b), np.sinh(b)
+        W = np.array([[np.sqrt(x), np.sqrt(y)], [np.sqrt(x), -np.sqrt(y)]])
+        t = np.einsum("ai,aj,ak,al,am,an->ijklmn", W, W, W, W, W, W)
+        t = np.transpose(t, (0, 3, 4, 5, 1, 2))
+        return NTensor.from_dense(t, [[0, 1]] * 6, 2, 2)
+    if symmetry == "Trivial":
+        t = np.array([[np.exp(b), np.exp(-b)], [np.exp(-b), np.exp(b)]])
+        w, v = np.linalg.eigh(t)
+        q = (v * np.sqrt(w)) @ v.T
+        O = np.zeros((2,) * 6)
+        O[(0,) * 6] = 1.0
+        O[(1,) * 6] = 1.0
+        o = np.einsum("abcdef,Aa,Bb,Cc,Dd,Ee,Ff->ABCDEF", O, q, q, q, q, q, q)
+        return NTensor(1, np.full((6, 1), 2, dtype=np.int32), o.ravel(order="F"), 2)
+    raise ValueError("symmetry must be 'Z2' or 'Trivial'")

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .tensors import SymTensor
+from .tensors import NTensor, SymTensor
 
 log = logging.getLogger("tnrkit_amd")
 
@@ -190,6 +190,42 @@ class ATRG(TNRScheme):
 
     def __init__(self, T, precision=FP64):
         super().__init__(T, 0.0, precision)
+
+
+class HOTRG_3D(TNRScheme):
+    """HOTRG_3D(T) (src/schemes/hotrg3d.jl:27-141) for a 6-leg tensor
+    T[z z'; Y X y x] (``NTensor`` with 2 + 4 legs); each step compresses z,
+    rotates, three times (free energies with scalefactor 8)."""
+    kind = _lib.TNR_HOTRG_3D
+    scalefactor = 8.0
+
+    def __init__(self, T: NTensor, precision: int = FP64):
+        if not isinstance(T, NTensor) or T.nleg != 6 or T.ncod != 2:
+            raise TypeError("T must be a 2 + 4 leg NTensor")
+        self._T0 = T
+        self.k = 0.0
+        self.precision = int(precision)
+        self._h = C.c_void_p()
+        dims = np.ascontiguousarray(T.leg_dims, dtype=np.int32).ravel()
+        check(lib().tnr_create_3d(C.byref(self._h), T.N, dims.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  T.data.ctypes.data_as(C.POINTER(C.c_double)), T.data.size,
+                                  self.precision))
+
+    @property
+    def T(self) -> NTensor:
+        N = self._T0.N
+        dims = np.zeros(6 * N, dtype=np.int32)
+        n = C.c_size_t()
+        check(lib().tnr_get_dims(self._h, dims.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n)))
+        data = np.zeros(n.value)
+        check(lib().tnr_get_tensor(self._h, data.ctypes.data_as(C.POINTER(C.c_double)), n.value))
+        return NTensor(N, dims.reshape(6, N), data, 2)
+
+    @T.setter
+    def T(self, t: NTensor):
+        dims = np.ascontiguousarray(t.leg_dims, dtype=np.int32).ravel()
+        check(lib().tnr_set_tensor(self._h, dims.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   t.data.ctypes.data_as(C.POINTER(C.c_double)), t.data.size))
 
 
 class _MultiScheme(TNRScheme):

@@ -124,3 +124,118 @@ def trivial(T):
     """SymTensor of a dense (Trivial) 4D array."""
     T = np.asarray(T, dtype=np.float64)
     return SymTensor(1, np.array(T.shape, dtype=np.int32).reshape(4, 1), T.ravel(order="F"))
+
+
+# ------------------------------------------------------------ n-leg tensors
+def _tuples_k(N, dims, c):
+    """Colex sector tuples (first leg fastest) over the legs whose per-sector
+    dimensions are ``dims`` (a list of arrays), labels summing to c mod N,
+    with their element counts."""
+    k = len(dims)
+    out = []
+    for t in range(N ** k):
+        labels, x = [], t
+        for _ in range(k):
+            labels.append(x % N)
+            x //= N
+        if sum(labels) % N == c:
+            size = 1
+            for j, a in enumerate(labels):
+                size *= int(dims[j][a])
+            out.append((tuple(labels), size))
+    return out
+
+
+class NTensor:
+    """A map V1 (x) .. (x) V_ncod <- V_{ncod+1} (x) .. (x) V_nleg in the ABI
+    block format (the 3D tensors of HOTRG_3D: 2 + 4 legs).  ``leg_dims[i, a]``
+    is the dimension of sector a in V_{i+1}; ``data`` holds, for coupled
+    charge c = 0..N-1, the column-major block with rows over codomain sector
+    tuples and columns over domain sector tuples summing to c, both
+    colexicographic, elements column-major inside a tuple."""
+    __slots__ = ("N", "leg_dims", "data", "ncod")
+
+    def __init__(self, N, leg_dims, data, ncod):
+        self.N = int(N)
+        self.leg_dims = np.ascontiguousarray(np.asarray(leg_dims, dtype=np.int32).reshape(-1, self.N))
+        self.ncod = int(ncod)
+        self.data = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
+        if self.data.size != self._size():
+            raise ValueError("data size does not match the sector dimensions")
+
+    @property
+    def nleg(self):
+        return self.leg_dims.shape[0]
+
+    def _size(self):
+        cod, dom = list(self.leg_dims[:self.ncod]), list(self.leg_dims[self.ncod:])
+        n = 0
+        for c in range(self.N):
+            R = sum(s for _, s in _tuples_k(self.N, cod, c))
+            Cc = sum(s for _, s in _tuples_k(self.N, dom, c))
+            n += R * Cc
+        return n
+
+    def copy(self):
+        return NTensor(self.N, self.leg_dims.copy(), self.data.copy(), self.ncod)
+
+    @staticmethod
+    def from_dense(T, leg_labels, N, ncod):
+        """From a dense array whose index i of leg k carries space label
+        leg_labels[k][i] (entries that violate charge conservation must be 0)."""
+        T = np.asarray(T, dtype=np.float64)
+        nleg = T.ndim
+        labels = [np.asarray(x, dtype=int) % N for x in leg_labels]
+        dims = np.zeros((nleg, N), dtype=np.int32)
+        order = []
+        for k in range(nleg):
+            for a in range(N):
+                dims[k, a] = int(np.sum(labels[k] == a))
+            order.append([np.nonzero(labels[k] == a)[0] for a in range(N)])
+
+        def index_lists(legs, tup):
+            grids = np.meshgrid(*[order[l][a] for l, a in zip(legs, tup)], indexing="ij")
+            return [g.ravel(order="F") for g in grids]
+
+        chunks = []
+        cod, dom = list(range(ncod)), list(range(ncod, nleg))
+        for c in range(N):
+            rows = [index_lists(cod, t) for t, _ in _tuples_k(N, [dims[l] for l in cod], c)]
+            cols = [index_lists(dom, t) for t, _ in _tuples_k(N, [dims[l] for l in dom], c)]
+            if not rows or not cols:
+                continue
+            r = [np.concatenate([x[j] for x in rows]) for j in range(ncod)]
+            cc = [np.concatenate([x[j] for x in cols]) for j in range(nleg - ncod)]
+            idx = tuple([ri[:, None] for ri in r] + [ci[None, :] for ci in cc])
+            chunks.append(T[idx].ravel(order="F"))
+        data = np.concatenate(chunks) if chunks else np.zeros(0)
+        return NTensor(N, dims, data, ncod)
+
+    def to_dense(self):
+        N, ncod, nleg = self.N, self.ncod, self.nleg
+        off = [_sector_index(self.leg_dims[k]) for k in range(nleg)]
+        T = np.zeros([int(self.leg_dims[k].sum()) for k in range(nleg)])
+        cod, dom = list(range(ncod)), list(range(ncod, nleg))
+        pos = 0
+        for c in range(N):
+            rows = _tuples_k(N, [self.leg_dims[l] for l in cod], c)
+            cols = _tuples_k(N, [self.leg_dims[l] for l in dom], c)
+            R = sum(s for _, s in rows)
+            Cc = sum(s for _, s in cols)
+            if R == 0 or Cc == 0:
+                continue
+            B = self.data[pos:pos + R * Cc].reshape(R, Cc, order="F")
+            pos += R * Cc
+            r = 0
+            for rt, sr in rows:
+                cc = 0
+                for ct, sc in cols:
+                    shape = [self.leg_dims[l][a] for l, a in zip(cod, rt)] + \
+                            [self.leg_dims[l][a] for l, a in zip(dom, ct)]
+                    sub = B[r:r + sr, cc:cc + sc].reshape(shape, order="F")
+                    sl = tuple(slice(off[l][a], off[l][a + 1]) for l, a in
+                               zip(cod + dom, rt + ct))
+                    T[sl] = sub
+                    cc += sc
+                r += sr
+        return T
