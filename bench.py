@@ -408,6 +408,50 @@ def workload_atrg(tk, maxiter, batch, cpu_seconds):
                                       "free_energies_per_s": batch / dt, "s": dt}}
 
 
+def _cpu_hotrg3d(chi, maxiter, budget_s):
+    from oracle import models as om
+    from oracle import schemes as osch
+    sc = osch.HOTRG_3D(om.classical_ising_3d_z2(om.ising_betac_3d))
+    t0 = time.perf_counter()
+    sc.finalize()
+    last, steps = 0.0, 0
+    while steps < maxiter:
+        t1 = time.perf_counter()
+        sc.step(chi)
+        sc.finalize()
+        last = time.perf_counter() - t1
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    return (time.perf_counter() - t0) + (maxiter - steps) * last, steps
+
+
+def workload_3d(tk, maxiter, cpu_seconds):
+    """HOTRG_3D on the Z2 3D Ising tensor at beta_c, chi 8 / 10 / 12, one
+    run at a time; free energy against -3.507; the single-threaded oracle
+    on the chi-8 run beside it."""
+    cases = []
+    for chi in (8, 10, 12):
+        w = tk.HOTRG_3D(tk.classical_ising_3D())
+        tk.run(w, tk.truncrank(chi), tk.maxiter(1), verbosity=0)
+        w.close()
+        sc = tk.HOTRG_3D(tk.classical_ising_3D())
+        d, dt = _timed(lambda: tk.run(sc, tk.truncrank(chi), tk.maxiter(maxiter), verbosity=0))
+        sc.close()
+        f = tk.free_energy(d, tk.ising_betac_3D, scalefactor=8.0)
+        c = {"chi": chi, "maxiter": maxiter, "s_per_run": dt, "steps_per_s": maxiter / dt,
+             "f": f, "rel_df_vs_benchmark": abs(f / tk.f_benchmark3D - 1)}
+        if chi == 8 and cpu_seconds > 0:
+            import multiprocessing as mp
+            with mp.get_context("spawn").Pool(1) as pool:
+                cs, steps = pool.apply(_cpu_hotrg3d, (chi, maxiter, cpu_seconds))
+            c["oracle_cpu_s_per_run"] = cs
+            c["oracle_cpu_steps_timed"] = steps
+            c["gpu_speedup_vs_oracle_1core"] = cs / dt
+        cases.append(c)
+    return {"workload": "HOTRG_3D: Z2 3D Ising beta_c", "cases": cases}
+
+
 def workload_c4(tk, n, maxiter, check):
     """512 BTRG runs, beta ~ U[0.35, 0.55], chi ~ U{16, 24, 32, 48}."""
     rng = np.random.default_rng(SEED + 4)
@@ -503,9 +547,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrivals per second")
-    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs", "atrg"],
+    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs", "atrg", "3d"],
                     default="scan",
-                    help="scan = the headline beta scan; c2-c4, imp, obs and atrg print a "
+                    help="scan = the headline beta scan; c2-c4, imp, obs, atrg and 3d print a "
                          "report line of their own")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -523,6 +567,8 @@ def main():
             r = workload_imp(tk, a.maxiter, 0.0 if a.no_cpu else a.cpu_seconds)
         elif a.workload == "obs":
             r = workload_obs(tk, 5, not a.no_cpu)
+        elif a.workload == "3d":
+            r = workload_3d(tk, a.maxiter, 0.0 if a.no_cpu else a.cpu_seconds)
         elif a.workload == "atrg":
             r = workload_atrg(tk, a.maxiter, 64 if a.batch == 128 else a.batch,
                               0.0 if a.no_cpu else a.cpu_seconds)
