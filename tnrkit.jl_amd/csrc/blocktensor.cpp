@@ -378,9 +378,9 @@ struct TuneTrial {
     int shape = 0;
 };
 struct TuneEntry {
-    float ms[GEMM_NSHAPES] = {0, 0, 0, 0};
-    int samples[GEMM_NSHAPES] = {0, 0, 0, 0};
-    int requested[GEMM_NSHAPES] = {0, 0, 0, 0};
+    float ms[GEMM_NSHAPES] = {};
+    int samples[GEMM_NSHAPES] = {};
+    int requested[GEMM_NSHAPES] = {};
     std::vector<TuneTrial> pending;
     int best = -1;
 };
@@ -496,8 +496,7 @@ GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int
     if (force_shape >= 0 && !split32) P.shape = force_shape;
     else if (split32) P.shape = GEMM_64x64;
     else P.shape = tuned_shape(ctx, jobs, pick_shape(jobs, false), &P.tune_key, &P.trial);
-    const int BMs = (P.shape == GEMM_64x64 || P.shape == GEMM_64x64_K32) ? 64 : 128;
-    const int BNs = P.shape == GEMM_128x128 ? 128 : 64;
+    const int BMs = gemm_shape_bm(P.shape), BNs = gemm_shape_bn(P.shape);
     // split K over workgroups only when the launch is too small to fill
     // the chip without it (results are the same either way)
     int64_t whole = 0;

@@ -183,8 +183,15 @@ void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t chunks
 void launch_transpose(hipStream_t st, const TransDesc* d_descs, int nd, int64_t tiles,
                       int64_t elems);
 // tile shapes of the FP64 GEMM kernel (the split-FP32 kernel is 64 x 64)
+// FP64 tile shapes: workgroup tile BM x BN, waves, k-tile depth
+//   GEMM_64x64       64x64,  4 waves of 32x32, k 16
+//   GEMM_128x64     128x64,  8 waves of 32x32, k 16
+//   GEMM_128x128   128x128,  8 waves of 32x64, k 16
+//   GEMM_64x64_K32   64x64,  4 waves of 32x32, k 32
 enum GemmShape { GEMM_64x64 = 0, GEMM_128x64 = 1, GEMM_128x128 = 2, GEMM_64x64_K32 = 3 };
 constexpr int GEMM_NSHAPES = 4;
+inline int gemm_shape_bm(int s) { return (s == GEMM_64x64 || s == GEMM_64x64_K32) ? 64 : 128; }
+inline int gemm_shape_bn(int s) { return s == GEMM_128x128 ? 128 : 64; }
 void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32,
                  double flops, int shape);
 void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const int64_t* d_starts,

@@ -172,10 +172,24 @@ struct TileCoord {
 template <int BM, int BN>
 __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local) {
     TileCoord c;
-    int tm = (int)(local % g.tiles_m);
-    int64_t rest = local / g.tiles_m;
-    int tn = (int)(rest % g.tiles_n);
-    int kz = (int)(rest / g.tiles_n);
+    // Workgroups go round-robin over the 8 XCDs (each with its own L2): the
+    // physical ids of one residue class mod 8 get one contiguous range of
+    // the tile order, which walks GROUP_M tile rows at a time (then the next
+    // tile column), so the tiles an XCD runs together share A and B slices.
+    // A bijection on each K chunk's tiles; every tile's arithmetic is the
+    // same wherever it runs.
+    constexpr int NXCD = 8, GROUP_M = 8;
+    const int64_t per_k = (int64_t)g.tiles_m * g.tiles_n;
+    const int kz = (int)(local / per_k);
+    int64_t r = local - (int64_t)kz * per_k;
+    const int64_t F = per_k / NXCD;
+    if (r < NXCD * F) r = (r % NXCD) * F + r / NXCD;
+    const int64_t width = (int64_t)GROUP_M * g.tiles_n;
+    const int64_t grp = r / width, inner = r - grp * width;
+    const int first_m = (int)(grp * GROUP_M);
+    const int gsz = min(g.tiles_m - first_m, GROUP_M);
+    const int tm = first_m + (int)(inner % gsz);
+    const int tn = (int)(inner / gsz);
     c.m0 = tm * BM;
     c.n0 = tn * BN;
     if (g.ksplit > 1) {           // one K chunk per workgroup
