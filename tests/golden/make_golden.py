@@ -9,6 +9,11 @@ python tests/golden/make_golden.py).
 * oracle_norms.json - per-step norm vectors of the FP64 oracle on fixed
   models and configurations, so the GPU parity tests compare against stored
   vectors and the CPU suite notices any drift of the oracle.
+* oracle_extensions.json - the same for the schemes and observables beyond
+  TRG / BTRG / HOTRG: ATRG and HOTRG_3D norm vectors, ImpurityTRG (pivoted
+  SVD gauge) and ImpurityHOTRG rows, and cft_data / ground_state_degeneracy /
+  gu_wen_ratio / central_charge after fixed runs
+  (python tests/golden/make_golden.py extensions writes only this file).
 """
 from __future__ import annotations
 
@@ -93,5 +98,41 @@ def main():
         json.dump(out, f, indent=1)
 
 
+def extensions():
+    from oracle import observables as ob
+    out = {}
+    sc = osch.ATRG(om.classical_ising_z2(om.ising_betac_mp))
+    out["atrg_z2_ising_betac_chi12_8"] = {"chi": 12, "maxiter": 8,
+                                          "norms": [float(x) for x in osch.run(sc, 12, 8)]}
+    sc = osch.HOTRG_3D(om.classical_ising_3d_z2(om.ising_betac_3d))
+    out["hotrg3d_z2_ising3d_betac_chi6_3"] = {"chi": 6, "maxiter": 3,
+                                              "norms": [float(x) for x in osch.run(sc, 6, 3)]}
+    T, Ti = om.classical_ising_trivial(0.44, 0.1), om.classical_ising_impurity(0.44, 0.1)
+    d = osch.run(osch.ImpurityTRG(T, Ti, T, T, T, gauge="pivoted"), 8, 6)
+    out["impurity_trg_b0.44_h0.1_chi8_6"] = {"beta": 0.44, "h": 0.1, "chi": 8, "maxiter": 6,
+                                             "rows": [[float(v) for v in r] for r in d]}
+    T, Ti = om.classical_ising_trivial(0.5, 0.1), om.classical_ising_impurity(0.5, 0.1)
+    d = osch.run(osch.ImpurityHOTRG(T, Ti, Ti, T), 8, 4)
+    out["impurity_hotrg_b0.5_h0.1_chi8_4"] = {"beta": 0.5, "h": 0.1, "chi": 8, "maxiter": 4,
+                                              "rows": [[float(v) for v in r] for r in d]}
+    for kind in ("TRG", "BTRG"):
+        sc = {"TRG": osch.TRG, "BTRG": osch.BTRG}[kind](om.classical_ising_z2(om.ising_betac_mp))
+        d = osch.run(sc, 16, 6)
+        x1, x2 = ob.gu_wen_ratio(sc)
+        out[f"observables_{kind.lower()}_z2_ising_betac_chi16_6"] = {
+            "scheme": kind, "chi": 16, "maxiter": 6,
+            "cft_data_first6": [float(v) for v in ob.cft_data(sc)[:6]],
+            "ground_state_degeneracy": ob.ground_state_degeneracy(sc),
+            "gu_wen_ratio": [x1, x2],
+            "central_charge_n_last": ob.central_charge(sc, d[-1]),
+            "n_last": float(d[-1])}
+    with open(os.path.join(HERE, "oracle_extensions.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote", len(out), "cases")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["extensions"]:
+        extensions()
+    else:
+        main()

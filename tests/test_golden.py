@@ -61,3 +61,33 @@ def test_oracle_reproduces_stored_norms(name):
     assert len(d) == len(ref)
     # the oracle is deterministic up to the host BLAS's threading
     assert np.max(np.abs(np.array(d) - ref) / np.abs(ref)) < 1e-12
+
+
+# ------------------------------------------------- schemes beyond TRG/BTRG/HOTRG
+def test_oracle_reproduces_stored_extensions():
+    """oracle_extensions.json: ATRG and HOTRG_3D norms, ImpurityTRG (pivoted
+    gauge) and ImpurityHOTRG rows, observables after fixed runs."""
+    from oracle import observables as ob
+    g = load("oracle_extensions.json")
+    c = g["atrg_z2_ising_betac_chi12_8"]
+    d = osch.run(osch.ATRG(om.classical_ising_z2(om.ising_betac_mp)), c["chi"], c["maxiter"])
+    assert np.max(np.abs(np.array(d) / np.array(c["norms"]) - 1)) < 1e-12
+    c = g["hotrg3d_z2_ising3d_betac_chi6_3"]
+    d = osch.run(osch.HOTRG_3D(om.classical_ising_3d_z2(om.ising_betac_3d)), c["chi"], c["maxiter"])
+    assert np.max(np.abs(np.array(d) / np.array(c["norms"]) - 1)) < 1e-12
+    c = g["impurity_trg_b0.44_h0.1_chi8_6"]
+    T, Ti = om.classical_ising_trivial(c["beta"], c["h"]), om.classical_ising_impurity(c["beta"], c["h"])
+    d = osch.run(osch.ImpurityTRG(T, Ti, T, T, T, gauge="pivoted"), c["chi"], c["maxiter"])
+    assert np.max(np.abs(np.array(d) / np.array(c["rows"]) - 1)) < 1e-12
+    c = g["impurity_hotrg_b0.5_h0.1_chi8_4"]
+    T, Ti = om.classical_ising_trivial(c["beta"], c["h"]), om.classical_ising_impurity(c["beta"], c["h"])
+    d = osch.run(osch.ImpurityHOTRG(T, Ti, Ti, T), c["chi"], c["maxiter"])
+    assert np.max(np.abs(np.array(d) / np.array(c["rows"]) - 1)) < 1e-12
+    for kind in ("TRG", "BTRG"):
+        c = g[f"observables_{kind.lower()}_z2_ising_betac_chi16_6"]
+        sc = {"TRG": osch.TRG, "BTRG": osch.BTRG}[kind](om.classical_ising_z2(om.ising_betac_mp))
+        d = osch.run(sc, c["chi"], c["maxiter"])
+        assert np.max(np.abs(ob.cft_data(sc)[:6] - np.array(c["cft_data_first6"]))) < 1e-10
+        assert abs(ob.ground_state_degeneracy(sc) / c["ground_state_degeneracy"] - 1) < 1e-10
+        assert np.allclose(ob.gu_wen_ratio(sc), c["gu_wen_ratio"], rtol=1e-12)
+        assert abs(ob.central_charge(sc, d[-1]) - c["central_charge_n_last"]) < 1e-12

@@ -270,3 +270,50 @@ def test_impurity_host_loop_equals_device_run(tk):
     b = tk.run(tk.ImpurityTRG(T, Ti, T, T, T), tk.truncrank(8), tk.maxiter(4),
                tk.Finalizer(tk.finalize), verbosity=0)
     assert a == b
+
+
+
+def test_against_golden_extensions(tk):
+    """tests/golden/oracle_extensions.json on the device: ATRG and HOTRG_3D
+    norms (rtol 1e-10), ImpurityTRG / ImpurityHOTRG rows, observables after
+    the stored TRG / BTRG runs (the tolerances of test_gpu_observables)."""
+    import json
+    import os
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_extensions.json")
+    with open(p) as f:
+        g = json.load(f)
+    c = g["atrg_z2_ising_betac_chi12_8"]
+    sc = tk.ATRG(tk.classical_ising(symmetry="Z2"))
+    d = tk.run(sc, tk.truncrank(c["chi"]), tk.maxiter(c["maxiter"]), verbosity=0)
+    sc.close()
+    assert np.max(np.abs(np.array(d) / np.array(c["norms"]) - 1)) < 1e-10
+    c = g["hotrg3d_z2_ising3d_betac_chi6_3"]
+    sc = tk.HOTRG_3D(tk.classical_ising_3D())
+    d = tk.run(sc, tk.truncrank(c["chi"]), tk.maxiter(c["maxiter"]), verbosity=0)
+    sc.close()
+    assert np.max(np.abs(np.array(d) / np.array(c["norms"]) - 1)) < 1e-10
+    c = g["impurity_trg_b0.44_h0.1_chi8_6"]
+    T = tk.classical_ising(c["beta"], "Trivial", c["h"])
+    Ti = tk.classical_ising_impurity(c["beta"], c["h"])
+    sc = tk.ImpurityTRG(T, Ti, T, T, T)
+    d = tk.run(sc, tk.truncrank(c["chi"]), tk.maxiter(c["maxiter"]), verbosity=0)
+    sc.close()
+    ok, worst = _close_rows(d, c["rows"])
+    assert ok, worst
+    c = g["impurity_hotrg_b0.5_h0.1_chi8_4"]
+    T = tk.classical_ising(c["beta"], "Trivial", c["h"])
+    Ti = tk.classical_ising_impurity(c["beta"], c["h"])
+    sc = tk.ImpurityHOTRG(T, Ti, Ti, T)
+    d = tk.run(sc, tk.truncrank(c["chi"]), tk.maxiter(c["maxiter"]), verbosity=0)
+    sc.close()
+    ok, worst = _close_rows(d, c["rows"])
+    assert ok, worst
+    for kind in ("TRG", "BTRG"):
+        c = g[f"observables_{kind.lower()}_z2_ising_betac_chi16_6"]
+        sc = getattr(tk, kind)(tk.classical_ising(symmetry="Z2"))
+        d = tk.run(sc, tk.truncrank(c["chi"]), tk.maxiter(c["maxiter"]), verbosity=0)
+        assert np.max(np.abs(tk.cft_data(sc)[:4] - np.array(c["cft_data_first6"][:4]))) < 1e-7
+        assert abs(tk.ground_state_degeneracy(sc) / c["ground_state_degeneracy"] - 1) < 1e-8
+        assert np.allclose(tk.gu_wen_ratio(sc), c["gu_wen_ratio"], rtol=1e-8)
+        assert abs(tk.central_charge(sc, d[-1]) - c["central_charge_n_last"]) < 1e-8
+        sc.close()
