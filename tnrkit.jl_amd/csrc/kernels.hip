@@ -203,25 +203,36 @@ __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local
     return c;
 }
 
+// Pointers taken from descriptor arrays are generic to the compiler, so plain
+// reads through them become flat loads, which also count in lgkmcnt: the
+// LDS wait before each MFMA group would then wait for the next k-tile's
+// prefetch as well.  Reading through a global-address-space view makes them
+// global loads, tracked by vmcnt alone.
+using gconst_double = const __attribute__((address_space(1))) double;
+using gmut_double = __attribute__((address_space(1))) double;
+__device__ __forceinline__ gconst_double* as_global(const double* p) { return (gconst_double*)p; }
+__device__ __forceinline__ gmut_double* as_global(double* p) { return (gmut_double*)p; }
+
 // global -> registers: the tile's BM x KB slab of op(A), KB BM / NT values
 // per thread of an NT-thread workgroup
 template <int BM, int NT, int KB>
 __device__ __forceinline__ void load_a(const GemmDesc& g, int m0, int k0, int t,
                                        double (&ra)[KB * BM / NT]) {
     constexpr int R = KB * BM / NT;
+    gconst_double* A = as_global(g.A);
     if (!g.ta) {  // A[i + k*lda], M x K: thread -> (i, k-group)
         const int i = m0 + (t % BM), kk = t / BM;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int k = k0 + kk + (NT / BM) * r;
-            ra[r] = (i < g.M && k < g.K) ? g.A[(int64_t)i + (int64_t)k * g.lda] : 0.0;
+            ra[r] = (i < g.M && k < g.K) ? A[(int64_t)i + (int64_t)k * g.lda] : 0.0;
         }
     } else {      // stored K x M: A[k + i*lda]
         const int kk = t % KB, i0 = t / KB;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = m0 + i0 + (NT / KB) * r, k = k0 + kk;
-            ra[r] = (i < g.M && k < g.K) ? g.A[(int64_t)k + (int64_t)i * g.lda] : 0.0;
+            ra[r] = (i < g.M && k < g.K) ? A[(int64_t)k + (int64_t)i * g.lda] : 0.0;
         }
     }
 }
@@ -245,19 +256,20 @@ template <int BN, int NT, int KB>
 __device__ __forceinline__ void load_b(const GemmDesc& g, int n0, int k0, int t,
                                        double (&rb)[KB * BN / NT]) {
     constexpr int R = KB * BN / NT;
+    gconst_double* B = as_global(g.B);
     if (!g.tb) {  // B[k + j*ldb], K x N
         const int kk = t % KB, j0 = t / KB;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = n0 + j0 + (NT / KB) * r, k = k0 + kk;
-            rb[r] = (j < g.N && k < g.K) ? g.B[(int64_t)k + (int64_t)j * g.ldb] : 0.0;
+            rb[r] = (j < g.N && k < g.K) ? B[(int64_t)k + (int64_t)j * g.ldb] : 0.0;
         }
     } else {      // stored N x K: B[j + k*ldb]
         const int j = n0 + (t % BN), kk = t / BN;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int k = k0 + kk + (NT / BN) * r;
-            rb[r] = (j < g.N && k < g.K) ? g.B[(int64_t)j + (int64_t)k * g.ldb] : 0.0;
+            rb[r] = (j < g.N && k < g.K) ? B[(int64_t)j + (int64_t)k * g.ldb] : 0.0;
         }
     }
 }
@@ -281,7 +293,7 @@ template <bool F64_LAYOUT, int MI, int NI>
 __device__ __forceinline__ void write_tile(const GemmDesc& g, const TileCoord& tc, int lane,
                                            int wm0, int wn0, const dbl4 (&acc)[MI][NI]) {
     const bool part = g.ksplit > 1;
-    double* out = part ? g.work + (int64_t)tc.kz * g.M * g.N : g.C;
+    gmut_double* out = as_global(part ? g.work + (int64_t)tc.kz * g.M * g.N : g.C);
     const int64_t ld = part ? g.M : g.ldc;
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi)
@@ -295,7 +307,7 @@ __device__ __forceinline__ void write_tile(const GemmDesc& g, const TileCoord& t
                 const int row = tc.m0 + wm0 + mi * 16 + ri;
                 if (row >= g.M) continue;
                 const double v = acc[mi][ni][r];
-                double* p = out + (int64_t)row + (int64_t)col * ld;
+                gmut_double* p = out + (int64_t)row + (int64_t)col * ld;
                 if (part) *p = v;
                 else *p = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * *p;
             }
@@ -341,20 +353,31 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
             load_a<BM, NT, KB>(g, tc.m0, k0 + KB, t, ra);
             load_b<BN, NT, KB>(g, tc.n0, k0 + KB, t, rb);
         }
+        // operands of k-step ks + 1 are read from LDS while the MFMAs of
+        // k-step ks run (two register sets)
+        double a[2][MI], b[2][NI];
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi) a[0][mi] = As[buf][lane >> 4][wm0 + mi * 16 + (lane & 15)];
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) b[0][ni] = Bs[buf][lane >> 4][wn0 + ni * 16 + (lane & 15)];
 #pragma unroll
         for (int ks = 0; ks < KB / 4; ++ks) {
-            const int k = ks * 4 + (lane >> 4);
-            double a[MI], b[NI];
+            const int cur = ks & 1;
+            if (ks + 1 < KB / 4) {
+                const int k = (ks + 1) * 4 + (lane >> 4);
 #pragma unroll
-            for (int mi = 0; mi < MI; ++mi) a[mi] = As[buf][k][wm0 + mi * 16 + (lane & 15)];
+                for (int mi = 0; mi < MI; ++mi)
+                    a[cur ^ 1][mi] = As[buf][k][wm0 + mi * 16 + (lane & 15)];
 #pragma unroll
-            for (int ni = 0; ni < NI; ++ni) b[ni] = Bs[buf][k][wn0 + ni * 16 + (lane & 15)];
+                for (int ni = 0; ni < NI; ++ni)
+                    b[cur ^ 1][ni] = Bs[buf][k][wn0 + ni * 16 + (lane & 15)];
+            }
 #pragma unroll
             for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
                 for (int ni = 0; ni < NI; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[ni], acc[mi][ni],
-                                                                       0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][mi], b[cur][ni],
+                                                                       acc[mi][ni], 0, 0, 0);
         }
         // end of a K chunk: add its sum to the running total (chunk order)
         if ((k0 + KB) % g.kchunk == 0 || !more) {
