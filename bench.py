@@ -11,8 +11,11 @@ Reported beside the throughput:
 * accuracy gate: |f_batch - f_plain| / |f_plain| over a sample of runs,
   where f_plain is the plain one-run-at-a-time FP64 GPU path (must be
   <= 1e-9), and that path's own throughput;
-* roofline of the dominant kernel class (device time from HIP events on the
-  library's stream, algorithmic flops / bytes per launch);
+* roofline of the dominant kernel class over the timed steps: device time
+  from each launch's dispatch start / end timestamps (HIP events of
+  hipExtLaunchKernelGGL on the launching stream), algorithmic flops / bytes
+  per launch.  Beside it the GEMM's rate on one untimed batch run on a
+  single lane;
 * latency p50/p99 from a seeded Poisson stream of BTRG solves (Z2 Ising,
   Z3 Potts, Z6 clock; chi 8-64) through the continuous-batching server;
 * cpu_baseline: the FP64 numpy oracle (OpenBLAS dgemm / LAPACK gesdd, the
@@ -46,10 +49,11 @@ def device_sync(tk):
     tk.lib().tnr_device_sync()
 
 
-def profile_batch(tk, reqs, precision):
+def read_profile(tk):
+    """Per kernel class: launches, summed device time (each launch's dispatch
+    start / end timestamps since tnr_profile_enable(1)), algorithmic flops
+    and bytes; switches the recording off."""
     L = tk.lib()
-    L.tnr_profile_enable(1)
-    tk.run_batch(reqs, precision)
     out = {}
     for k, name in enumerate(KIND_NAMES):
         n, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
@@ -57,6 +61,12 @@ def profile_batch(tk, reqs, precision):
         out[name] = {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
     L.tnr_profile_enable(0)
     return out
+
+
+def profile_batch(tk, reqs, precision):
+    tk.lib().tnr_profile_enable(1)
+    tk.run_batch(reqs, precision)
+    return read_profile(tk)
 
 
 def _gemm_roofline(prof, name):
@@ -590,11 +600,16 @@ def main():
     for _ in range(a.warmup):
         tk.run_batch(reqs, prec)
     device_sync(tk)
+    if not a.no_profile:
+        # every launch of the timed steps carries its dispatch's start / end
+        # timestamps; nothing synchronises the host
+        tk.lib().tnr_profile_enable(1)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         res = tk.run_batch(reqs, prec)
     device_sync(tk)
     dt = time.perf_counter() - t0
+    prof = None if a.no_profile else read_profile(tk)
     ms_per_step = dt / a.steps * 1e3
     value = a.batch * a.steps / dt
     f_batch = [tk.free_energy(r, float(b)) for r, b in zip(res, betas)]
@@ -634,12 +649,27 @@ def main():
         sv = serving_stream(tk, SEED + 1, a.serve_requests, a.serve_rate, prec)
         out["latency_s"] = {"p50": sv["p50_s"], "p99": sv["p99_s"], "source": "serving stream"}
         out["serving"] = sv
-    if not a.no_profile:
-        prof = profile_batch(tk, reqs, prec)
+    if prof is not None:
         out["roofline"] = roofline_from(prof, a.precision, load_traffic())
         gname = "gemm_split32" if a.precision == "split32" else "gemm_f64"
         out["roofline_gemm"] = _gemm_roofline(prof, gname)
-        out["kernel_time_ms"] = {k: round(v["ms"], 3) for k, v in prof.items() if v["launches"]}
+        out["kernel_time_ms_per_step"] = {k: round(v["ms"] / a.steps, 3)
+                                          for k, v in prof.items() if v["launches"]}
+        # the same batch once more on a single lane: no launch then shares the
+        # device with the other lane's kernels, so this is the GEMM's own rate
+        prev = os.environ.get("TNR_LANES")
+        os.environ["TNR_LANES"] = "1"
+        try:
+            p1 = profile_batch(tk, reqs, prec)
+        finally:
+            if prev is None:
+                del os.environ["TNR_LANES"]
+            else:
+                os.environ["TNR_LANES"] = prev
+        g1 = _gemm_roofline(p1, gname)
+        g1["note"] = ("one untimed batch on one lane (TNR_LANES=1); the timed steps run two "
+                      "lanes whose kernels overlap, which lengthens each launch")
+        out["roofline_gemm_one_lane"] = g1
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))

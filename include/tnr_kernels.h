@@ -68,13 +68,17 @@ int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam,
 int tnr_kernel_jacobi(int count, int n, const double* G, double* W,
                       double* lam, int* sweeps, double* ms_out);
 
-/* Device-time accounting per kernel class (HIP events around every launch
- * on the launching stream).  tnr_profile_enable(1) clears and starts it,
- * (0) stops it.  Classes: 0 FP64 GEMM, 1 split-FP32 GEMM, 2 split-K reduce,
- * 3 Jacobi, 4 permute/copy, 5 column norms, 6 gather, 7 trace/normalise,
- * 8 random fill, 9 other.  flops/bytes are the algorithmic work the
- * launches were asked to do (GEMM: 2MNK summed over the grouped problems;
- * copies: bytes read + written). */
+/* Device-time accounting per kernel class.  A launch of one kernel is timed
+ * by its dispatch's start and end timestamps (the events of
+ * hipExtLaunchKernelGGL, no marker packets between launches); a class call
+ * of two kernels by events recorded around both.  tnr_profile_enable(1)
+ * clears and starts it without any host synchronisation, (2) also reads
+ * back after each Jacobi launch the sweeps it took (its flops), (0) stops
+ * it.  Classes: 0 FP64 GEMM, 1 split-FP32 GEMM, 2 split-K reduce, 3 Jacobi,
+ * 4 permute/copy, 5 column norms, 6 gather, 7 trace/normalise, 8 random
+ * fill, 9 other.  flops/bytes are the algorithmic work the launches were
+ * asked to do (GEMM: 2MNK summed over the grouped problems; copies: bytes
+ * read + written). */
 int tnr_profile_enable(int on);
 /* GEMM tile-shape autotuner: on = 1 (default; TNR_AUTOTUNE=0 in the
  * environment starts it off) times each tile shape on the first launches of

@@ -204,7 +204,8 @@ int tnr_kernel_jacobi(int count, int n, const double* G, double* W, double* lam,
 
 int tnr_profile_enable(int on) {
     return kguard([&] {
-        prof_enable(on != 0);
+        TNR_REQUIRE(on >= 0 && on <= 2, TNR_ERR_ARG, "profiling level must be 0, 1 or 2");
+        prof_enable(on);
         return TNR_OK;
     });
 }

@@ -226,19 +226,42 @@ enum ProfKind {
     PK_GEMM_F64 = 0, PK_GEMM_SPLIT32, PK_GEMM_REDUCE, PK_JACOBI, PK_COPY, PK_COLNORM,
     PK_GATHER, PK_TRACE, PK_FILL, PK_MISC, PK_COUNT
 };
-void prof_enable(bool on);
+// level 0 off; 1 device time per launch, no host synchronisation; 2 also
+// the work of launches whose work is only known on the device (the Jacobi
+// sweeps), read back after each such launch
+void prof_enable(int level);
 bool prof_enabled();
+int prof_level();
 struct ProfScope {
     ProfScope(hipStream_t st, int kind, double flops, double bytes);
     ~ProfScope();
-    void end();                       // record the closing event now
+    void bracket();                   // time the kernels launched from here on by events
+                                      // recorded around them (a scope of several kernels)
+    void end();                       // record the closing event of a bracket now
     hipStream_t st;
     int kind;
     bool active;
     bool ended = false;
+    int mode = 0;                     // 0 nothing launched, 1 bracket, 2 dispatch timestamps
     double flops, bytes;
     hipEvent_t ea = nullptr, eb = nullptr;
 };
+
+// Launch one kernel inside a profiling scope.  While profiling, a lone kernel
+// is timed by its dispatch's own start and end timestamps (the events of
+// hipExtLaunchKernelGGL), so no marker packets go between launches; a
+// bracketed scope launches plainly.
+#define TNR_LAUNCH(ps, kernel, grid, block, shm, st, ...)                                   \
+    do {                                                                                    \
+        if ((ps).active && (ps).mode == 0) {                                                \
+            hipExtLaunchKernelGGL(kernel, grid, block, shm, st, (ps).ea, (ps).eb, 0u,       \
+                                  __VA_ARGS__);                                             \
+            (ps).mode = 2;                                                                  \
+            (ps).ended = true;                                                              \
+        } else {                                                                            \
+            hipLaunchKernelGGL(kernel, grid, block, shm, st, __VA_ARGS__);                  \
+        }                                                                                   \
+    } while (0)
 struct ProfTotals { int64_t launches = 0; double ms = 0, flops = 0, bytes = 0; };
 ProfTotals prof_read(int kind);   // synchronises the device
 

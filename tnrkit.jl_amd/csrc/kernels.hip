@@ -9,6 +9,8 @@
 // run and alone vs batched.
 #include "kernels.h"
 
+#include <hip/hip_ext.h>
+
 #include <mutex>
 
 #include <algorithm>
@@ -1511,7 +1513,7 @@ __global__ __launch_bounds__(256) void k_frob(const ColDesc* __restrict__ descs,
 namespace {
 struct ProfState {
     std::mutex mu;
-    bool on = false;
+    int level = 0;
     struct Rec { hipEvent_t a, b; int kind; double flops, bytes; };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> spare;
@@ -1522,19 +1524,30 @@ struct ProfState {
         TNR_HIP(hipEventCreate(&e));
         return e;
     }
-    void drain() {      // with mu held
+    void file(const Rec& r) {   // with mu held; r.b has completed
+        float ms = 0;
+        TNR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        tot[r.kind].launches++;
+        tot[r.kind].ms += ms;
+        tot[r.kind].flops += r.flops;
+        tot[r.kind].bytes += r.bytes;
+        spare.push_back(r.a);
+        spare.push_back(r.b);
+    }
+    void drain() {      // with mu held; waits for every pending launch
         for (auto& r : recs) {
             TNR_HIP(hipEventSynchronize(r.b));
-            float ms = 0;
-            TNR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-            tot[r.kind].launches++;
-            tot[r.kind].ms += ms;
-            tot[r.kind].flops += r.flops;
-            tot[r.kind].bytes += r.bytes;
-            spare.push_back(r.a);
-            spare.push_back(r.b);
+            file(r);
         }
         recs.clear();
+    }
+    void drain_ready() {   // with mu held; files the completed launches, never waits
+        size_t keep = 0;
+        for (size_t i = 0; i < recs.size(); ++i) {
+            if (hipEventQuery(recs[i].b) == hipSuccess) file(recs[i]);
+            else recs[keep++] = recs[i];
+        }
+        recs.resize(keep);
     }
 };
 ProfState& prof() {
@@ -1543,18 +1556,20 @@ ProfState& prof() {
 }
 }  // namespace
 
-void prof_enable(bool on) {
+void prof_enable(int level) {
     ProfState& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
     p.drain();
-    p.on = on;
+    p.level = level;
     for (auto& t : p.tot) t = ProfTotals();
 }
 
-bool prof_enabled() {
+bool prof_enabled() { return prof_level() > 0; }
+
+int prof_level() {
     ProfState& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
-    return p.on;
+    return p.level;
 }
 
 // Launches may come from several host threads (one stream each): the
@@ -1563,15 +1578,21 @@ ProfScope::ProfScope(hipStream_t s, int k, double f, double b)
     : st(s), kind(k), active(false), flops(f), bytes(b) {
     ProfState& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
-    if (!p.on) return;
+    if (!p.level) return;
     ea = p.ev();
     eb = p.ev();
     active = true;
-    TNR_HIP(hipEventRecord(ea, st));
+}
+
+void ProfScope::bracket() {
+    if (active && mode == 0) {
+        TNR_HIP(hipEventRecord(ea, st));
+        mode = 1;
+    }
 }
 
 void ProfScope::end() {
-    if (active && !ended) {
+    if (active && mode == 1 && !ended) {
         (void)hipEventRecord(eb, st);
         ended = true;
     }
@@ -1582,10 +1603,15 @@ ProfScope::~ProfScope() {
     end();
     ProfState& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
+    if (mode == 0) {   // nothing was launched
+        p.spare.push_back(ea);
+        p.spare.push_back(eb);
+        return;
+    }
     p.recs.push_back(ProfState::Rec{ea, eb, kind, flops, bytes});
     if (p.recs.size() > 4096) {
         try {
-            p.drain();
+            p.drain_ready();
         } catch (...) {
         }
     }
@@ -1611,7 +1637,7 @@ void launch_copy(hipStream_t st, const CopyDesc* d_descs, int nd, int64_t chunks
     if (chunks <= 0 || nd <= 0) return;
     ProfScope ps(st, PK_COPY, 0.0, 16.0 * (double)elems);
     const int grid = (int)std::min<int64_t>(chunks, 8192);
-    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, d_descs, nd, chunks);
+    TNR_LAUNCH(ps, k_copy, dim3(grid), dim3(256), 0, st, d_descs, nd, chunks);
     TNR_HIP(hipGetLastError());
 }
 
@@ -1620,7 +1646,7 @@ void launch_transpose(hipStream_t st, const TransDesc* d_descs, int nd, int64_t 
     if (tiles <= 0 || nd <= 0) return;
     ProfScope ps(st, PK_COPY, 0.0, 16.0 * (double)elems);
     const int grid = (int)std::min<int64_t>(tiles, 4096);
-    hipLaunchKernelGGL(k_transpose, dim3(grid), dim3(256), 0, st, d_descs, nd, tiles);
+    TNR_LAUNCH(ps, k_transpose, dim3(grid), dim3(256), 0, st, d_descs, nd, tiles);
     TNR_HIP(hipGetLastError());
 }
 
@@ -1629,18 +1655,18 @@ void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles,
     if (tiles <= 0 || nd <= 0) return;
     ProfScope ps(st, split32 ? PK_GEMM_SPLIT32 : PK_GEMM_F64, flops, 0.0);
     if (split32)
-        hipLaunchKernelGGL(k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
+        TNR_LAUNCH(ps, k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
     else if (shape == GEMM_128x128)
-        hipLaunchKernelGGL((k_gemm_f64<128, 128, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
+        TNR_LAUNCH(ps, (k_gemm_f64<128, 128, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
                            st, d_descs, nd);
     else if (shape == GEMM_128x64)
-        hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
+        TNR_LAUNCH(ps, (k_gemm_f64<128, 64, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
                            st, d_descs, nd);
     else if (shape == GEMM_64x64_K32)
-        hipLaunchKernelGGL((k_gemm_f64<64, 64, 2, 2, 32>), dim3((unsigned)tiles), dim3(256), 0,
+        TNR_LAUNCH(ps, (k_gemm_f64<64, 64, 2, 2, 32>), dim3((unsigned)tiles), dim3(256), 0,
                            st, d_descs, nd);
     else
-        hipLaunchKernelGGL((k_gemm_f64<64, 64, 2, 2, 16>), dim3((unsigned)tiles), dim3(256), 0,
+        TNR_LAUNCH(ps, (k_gemm_f64<64, 64, 2, 2, 16>), dim3((unsigned)tiles), dim3(256), 0,
                            st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
@@ -1649,7 +1675,7 @@ void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const i
                         int64_t total) {
     if (total <= 0 || nd <= 0) return;
     ProfScope ps(st, PK_GEMM_REDUCE, 0.0, 0.0);
-    hipLaunchKernelGGL(k_gemm_reduce, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd,
+    TNR_LAUNCH(ps, k_gemm_reduce, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd,
                        d_starts, total);
     TNR_HIP(hipGetLastError());
 }
@@ -1658,11 +1684,12 @@ void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const Trac
                             int nprob, int64_t max_size) {
     if (nprob <= 0) return;
     ProfScope ps(st, PK_TRACE, 0.0, 0.0);
-    hipLaunchKernelGGL(k_trace_partial, dim3(TRACE_PARTS, nprob), dim3(256), 0, st, d_descs,
+    ps.bracket();
+    TNR_LAUNCH(ps, k_trace_partial, dim3(TRACE_PARTS, nprob), dim3(256), 0, st, d_descs,
                        d_probs);
     TNR_HIP(hipGetLastError());
     int gx = grid_for(max_size, 1024);
-    hipLaunchKernelGGL(k_normalize, dim3(gx, nprob), dim3(256), 0, st, d_probs);
+    TNR_LAUNCH(ps, k_normalize, dim3(gx, nprob), dim3(256), 0, st, d_probs);
     TNR_HIP(hipGetLastError());
 }
 
@@ -1681,10 +1708,10 @@ void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, 
         (int)jacobi_lds_bytes(LMAX));
     TNR_HIP(attr);
     ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
-    hipLaunchKernelGGL(k_jacobi, dim3(nd), dim3(JT), jacobi_lds_bytes(nmax), st, d_descs, nd, tol,
+    TNR_LAUNCH(ps, k_jacobi, dim3(nd), dim3(JT), jacobi_lds_bytes(nmax), st, d_descs, nd, tol,
                        max_sweeps);
     TNR_HIP(hipGetLastError());
-    if (ps.active) {
+    if (ps.active && prof_level() >= 2) {
         // profiling only: the work a launch did depends on the sweeps each
         // problem took, read back after the launch (the timing events are
         // already recorded)
@@ -1721,7 +1748,7 @@ void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax,
         (int)cholqr_lds_bytes(LMAX));
     TNR_HIP(attr);
     ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
-    hipLaunchKernelGGL(k_cholqr, dim3(nd), dim3(CQT), cholqr_lds_bytes(nmax), st, d_descs, nd,
+    TNR_LAUNCH(ps, k_cholqr, dim3(nd), dim3(CQT), cholqr_lds_bytes(nmax), st, d_descs, nd,
                        null_rel2);
     TNR_HIP(hipGetLastError());
 }
@@ -1729,7 +1756,7 @@ void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax,
 void launch_lincomb(hipStream_t st, const LinDesc* d_descs, int nd, int64_t total) {
     if (total <= 0 || nd <= 0) return;
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    hipLaunchKernelGGL(k_lincomb, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd,
+    TNR_LAUNCH(ps, k_lincomb, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd,
                        total);
     TNR_HIP(hipGetLastError());
 }
@@ -1737,7 +1764,7 @@ void launch_lincomb(hipStream_t st, const LinDesc* d_descs, int nd, int64_t tota
 void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, double rtol) {
     if (nd <= 0) return;
     ProfScope ps(st, PK_COLNORM, 0.0, 0.0);
-    hipLaunchKernelGGL(k_colnorm, dim3(nd), dim3(1024), 0, st, d_descs, nd, rtol);
+    TNR_LAUNCH(ps, k_colnorm, dim3(nd), dim3(1024), 0, st, d_descs, nd, rtol);
     TNR_HIP(hipGetLastError());
 }
 
@@ -1750,7 +1777,7 @@ void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, do
     TNR_HIP(attr);
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
     const size_t shm = ((size_t)kmax * kmax + 2 * (size_t)kmax) * sizeof(double);
-    hipLaunchKernelGGL(k_canon, dim3(nd), dim3(CNT), shm, st, d_descs, nd, tie);
+    TNR_LAUNCH(ps, k_canon, dim3(nd), dim3(CNT), shm, st, d_descs, nd, tie);
     TNR_HIP(hipGetLastError());
 }
 
@@ -1758,9 +1785,10 @@ void launch_dot(hipStream_t st, const DotDesc* d_descs, int nd) {
     if (nd <= 0) return;
     TNR_REQUIRE(nd <= 65535, TNR_ERR_INTERNAL, "too many dot products in one launch");
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    hipLaunchKernelGGL(k_dot_part, dim3(DOT_PARTS, nd), dim3(256), 0, st, d_descs, nd);
+    ps.bracket();
+    TNR_LAUNCH(ps, k_dot_part, dim3(DOT_PARTS, nd), dim3(256), 0, st, d_descs, nd);
     TNR_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_dot_final, dim3(nd), dim3(64), 0, st, d_descs, nd);
+    TNR_LAUNCH(ps, k_dot_final, dim3(nd), dim3(64), 0, st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 
@@ -1772,7 +1800,7 @@ void launch_eig(hipStream_t st, const EigDesc* d_descs, int nd, int nmax) {
         (const void*)k_eig_hqr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes(EIG_NMAX));
     TNR_HIP(attr);
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    hipLaunchKernelGGL(k_eig_hqr, dim3(nd), dim3(EIGT), bytes(nmax), st, d_descs, nd);
+    TNR_LAUNCH(ps, k_eig_hqr, dim3(nd), dim3(EIGT), bytes(nmax), st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 
@@ -1780,7 +1808,7 @@ void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax)
     if (nd <= 0) return;
     int64_t tot = (int64_t)nmax * nmax;
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    hipLaunchKernelGGL(k_symmetrize, dim3(grid_for(tot, 1024), nd), dim3(256), 0, st, d_descs, nd);
+    TNR_LAUNCH(ps, k_symmetrize, dim3(grid_for(tot, 1024), nd), dim3(256), 0, st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 
@@ -1788,7 +1816,7 @@ void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rt
                  int min_iters, int max_iters) {
     if (nd <= 0) return;
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    hipLaunchKernelGGL(k_conv, dim3((nd + 63) / 64), dim3(64), 0, st, d_descs, nd, null_rtol, target,
+    TNR_LAUNCH(ps, k_conv, dim3((nd + 63) / 64), dim3(64), 0, st, d_descs, nd, null_rtol, target,
                        min_iters, max_iters);
     TNR_HIP(hipGetLastError());
 }
@@ -1796,21 +1824,21 @@ void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rt
 void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd) {
     if (nd <= 0) return;
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    hipLaunchKernelGGL(k_frob, dim3(nd), dim3(256), 0, st, d_descs, nd);
+    TNR_LAUNCH(ps, k_frob, dim3(nd), dim3(256), 0, st, d_descs, nd);
     TNR_HIP(hipGetLastError());
 }
 
 void launch_randfill(hipStream_t st, const FillDesc* d_descs, int nd, int64_t total) {
     if (total <= 0 || nd <= 0) return;
     ProfScope ps(st, PK_FILL, 0.0, 0.0);
-    hipLaunchKernelGGL(k_randfill, dim3(grid_for(total, 4096)), dim3(256), 0, st, d_descs, nd, total);
+    TNR_LAUNCH(ps, k_randfill, dim3(grid_for(total, 4096)), dim3(256), 0, st, d_descs, nd, total);
     TNR_HIP(hipGetLastError());
 }
 
 void launch_gather(hipStream_t st, const GatherDesc* d_descs, int nd, int64_t total) {
     if (total <= 0 || nd <= 0) return;
     ProfScope ps(st, PK_GATHER, 0.0, 16.0 * (double)total);
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd, total);
+    TNR_LAUNCH(ps, k_gather, dim3(grid_for(total, 8192)), dim3(256), 0, st, d_descs, nd, total);
     TNR_HIP(hipGetLastError());
 }
 

@@ -1,6 +1,7 @@
 """Turn rocprofv3 CSV output into the short summaries kept under profiles/.
 
-  python tnrkit.jl_amd/prof/summarize.py stats KERNEL_STATS_CSV --command CMD > profiles/X.md
+  python tnrkit.jl_amd/prof/summarize.py stats KERNEL_STATS_CSV --command CMD \
+      [--trace KERNEL_TRACE_CSV --kernel NAME --phases name:count,...] > profiles/X.md
   python tnrkit.jl_amd/prof/summarize.py traffic FETCH_CSV WRITE_CSV CAL_FETCH_CSV CAL_WRITE_CSV \
       --command CMD > profiles/traffic_X.json
 
@@ -37,6 +38,26 @@ def stats(path, command):
             100.0 * float(r["TotalDurationNs"]) / total))
     out.append("")
     out.append("Total kernel time: %.1f ms" % (total / 1e6))
+    return "\n".join(out)
+
+
+def phases(trace, kernel, spec):
+    """Average duration of `kernel` over consecutive groups of its dispatches
+    in start order (e.g. warmup, timed steps, a last pass), from the trace."""
+    rows = [r for r in csv.DictReader(open(trace)) if short(r["Kernel_Name"]).split("<")[0] == kernel]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
+    out = ["", "Per phase (`%s`, dispatches in start order, from the kernel trace):" % kernel, "",
+           "| phase | dispatches | avg us |", "|---|---|---|"]
+    i = 0
+    for item in spec.split(","):
+        name, cnt = item.split(":")
+        seg = dur[i:i + int(cnt)]
+        i += int(cnt)
+        out.append("| %s | %d | %.1f |" % (name, len(seg), sum(seg) / max(1, len(seg))))
+    if i != len(dur):
+        out.append("")
+        out.append("(%d dispatches in the trace, %d assigned to phases)" % (len(dur), i))
     return "\n".join(out)
 
 
@@ -80,6 +101,9 @@ def main():
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--command", required=True)
+    s.add_argument("--trace")
+    s.add_argument("--kernel")
+    s.add_argument("--phases")
     t = sub.add_parser("traffic")
     for x in ("fetch", "write", "cal_fetch", "cal_write"):
         t.add_argument(x)
@@ -87,6 +111,8 @@ def main():
     a = ap.parse_args()
     if a.mode == "stats":
         print(stats(a.csv, a.command))
+        if a.trace:
+            print(phases(a.trace, a.kernel, a.phases))
     else:
         print(traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.command))
 
