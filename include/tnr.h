@@ -188,7 +188,11 @@ typedef struct {
     int finalize_beginning;
 } tnr_run_desc;
 
-/* norms: concatenation over runs of (maxiter + finalize_beginning) doubles. */
+/* norms: concatenation over runs of (maxiter + finalize_beginning) doubles.
+ * The batch engine behind this call (its streams, device memory pools and
+ * pinned staging rings) stays alive for the life of the process and is
+ * reused by the next call on the same device with the same precision and
+ * lane count; concurrent calls run one after another. */
 int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision,
                   double* norms);
 

@@ -2,6 +2,7 @@
 // crosses this boundary: every call maps failures to a status code and keeps
 // the message for tnr_last_error().
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -10,6 +11,7 @@
 #include <exception>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <thread>
 
@@ -610,7 +612,33 @@ struct tnr_server {
     void clear() {
         for (auto& l : lanes) l->active.clear();
     }
+
+    // back to an empty server; streams, pools and staging rings are kept
+    void reset(int slots) {
+        clear();
+        for (auto& l : lanes) l->load = 0;
+        reqs.clear();
+        queue.clear();
+        in_use = 0;
+        max_slots = slots;
+        t0 = std::chrono::steady_clock::now();
+    }
 };
+
+// tnr_run_batch keeps one server per (device, precision, lane count) for the
+// life of the process.  A fresh server per call would allocate every working
+// buffer anew and free it again at the end (both synchronise the device) and
+// pin a new staging ring: about half a second of idle device per call at the
+// BTRG chi = 32 batch of 128 runs.
+// The map is never destroyed: at process exit the HIP runtime may already be
+// gone, and the driver reclaims the device memory anyway.
+namespace {
+std::mutex g_batch_mu;
+std::map<std::array<int, 3>, std::unique_ptr<tnr_server>>& batch_servers() {
+    static auto* m = new std::map<std::array<int, 3>, std::unique_ptr<tnr_server>>();
+    return *m;
+}
+}  // namespace
 
 extern "C" {
 
@@ -705,15 +733,27 @@ int tnr_server_destroy(tnr_server* s) {
 }
 
 int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision, double* norms) {
-    return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    std::array<int, 3> key{-1, precision, 0};
+    const int status = guarded([&] {
         if (nruns < 0 || (nruns > 0 && (!runs || !norms))) return fail(TNR_ERR_ARG, "null pointer");
-        tnr_server* s = nullptr;
-        int st = tnr_server_create(&s, 0, nruns > 0 ? nruns : 1, precision);
-        if (st != TNR_OK) return st;
-        std::unique_ptr<tnr_server> hold(s);
+        if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
+            return fail(TNR_ERR_ARG, "unknown precision");
+        ensure_device();
+        TNR_HIP(hipGetDevice(&key[0]));
+        key[2] = default_lanes();
+        std::unique_ptr<tnr_server>& held = batch_servers()[key];
+        if (!held) {
+            tnr_server* s = nullptr;
+            int st = tnr_server_create(&s, 0, 1, precision);
+            if (st != TNR_OK) return st;
+            held.reset(s);
+        }
+        tnr_server* s = held.get();
+        s->reset(nruns > 0 ? nruns : 1);
         std::vector<int64_t> ids(nruns);
         for (int i = 0; i < nruns; ++i) {
-            st = tnr_server_submit(s, &runs[i], &ids[i]);
+            int st = tnr_server_submit(s, &runs[i], &ids[i]);
             if (st != TNR_OK) return st;
         }
         int active = 1;
@@ -728,9 +768,21 @@ int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision, double* no
             std::memcpy(norms + o, r.norms.data(), r.norms.size() * sizeof(double));
             o += r.norms.size();
         }
-        s->clear();
+        s->reset(1);
         return TNR_OK;
     });
+    if (status != TNR_OK && key[0] >= 0) {
+        // a failed batch may leave work or state behind: start the next one afresh
+        auto it = batch_servers().find(key);
+        if (it != batch_servers().end()) {
+            try {
+                if (it->second) it->second->sync_all();
+            } catch (...) {
+            }
+            batch_servers().erase(it);
+        }
+    }
+    return status;
 }
 
 }  // extern "C"
