@@ -14,6 +14,7 @@
 #include <mutex>
 #include <new>
 #include <thread>
+#include <vector>
 
 #include "observables.h"
 #include "schemes.h"
@@ -65,16 +66,68 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 }
 }  // namespace
 
+// Contexts of destroyed handles are kept (a few per process) and handed to
+// the next handle on the same device.  A context per handle from scratch
+// means a new stream, a newly pinned staging ring and an empty pool, so every
+// buffer of the handle's first steps would be allocated (and later freed)
+// with the device synchronised each time.  The map of spares is never
+// destroyed: at process exit the HIP runtime may already be gone.
+namespace {
+constexpr size_t kSpareContexts = 4;
+constexpr size_t kSparePoolBytes = size_t(8) << 30;   // cache kept with a spare context
+std::mutex g_ctx_mu;
+std::vector<std::unique_ptr<Ctx>>& spare_contexts() {
+    static auto* v = new std::vector<std::unique_ptr<Ctx>>();
+    return *v;
+}
+
+std::unique_ptr<Ctx> take_context(int precision) {
+    ensure_device();
+    int dev = 0;
+    TNR_HIP(hipGetDevice(&dev));
+    std::unique_ptr<Ctx> c;
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        auto& v = spare_contexts();
+        for (size_t i = 0; i < v.size(); ++i)
+            if (v[i]->device == dev) {
+                c = std::move(v[i]);
+                v.erase(v.begin() + (long)i);
+                break;
+            }
+    }
+    if (!c) c.reset(new Ctx());
+    c->precision = precision;
+    return c;
+}
+
+// never throws: a context whose stream reports an error is destroyed
+void give_context(std::unique_ptr<Ctx> c) noexcept {
+    if (!c) return;
+    try {
+        c->sync();
+        if (c->pool.cached() > kSparePoolBytes) c->pool.trim();
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        auto& v = spare_contexts();
+        if (v.size() < kSpareContexts) v.push_back(std::move(c));
+    } catch (...) {
+    }
+}
+}  // namespace
+
 struct tnr_scheme {
     std::unique_ptr<Ctx> ctx;
     Scheme s;
     Buf norms;
     int cap = 0;
     ~tnr_scheme() {
-        if (ctx) ctx->sync();
+        try {
+            if (ctx) ctx->sync();
+        } catch (...) {
+        }
         s = Scheme();
         norms.release();
-        ctx.reset();
+        give_context(std::move(ctx));
     }
     double* slots(int n) {
         if (n > cap) {
@@ -100,8 +153,7 @@ int tnr_create(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims, cons
             return fail(TNR_ERR_ARG, "unknown precision");
         if (!std::isfinite(k)) return fail(TNR_ERR_ARG, "k must be finite");
         std::unique_ptr<tnr_scheme> h(new tnr_scheme());
-        h->ctx.reset(new Ctx());
-        h->ctx->precision = precision;
+        h->ctx = take_context(precision);
         BT T = upload(*h->ctx, zn, leg_dims, data, ndata);
         scheme_init(*h->ctx, h->s, kind, std::move(T), k);
         *out = h.release();
@@ -118,8 +170,7 @@ int tnr_create_3d(tnr_scheme** out, int zn, const int32_t* leg_dims, const doubl
         if (precision != TNR_PREC_FP64 && precision != TNR_PREC_SPLIT32)
             return fail(TNR_ERR_ARG, "unknown precision");
         std::unique_ptr<tnr_scheme> h(new tnr_scheme());
-        h->ctx.reset(new Ctx());
-        h->ctx->precision = precision;
+        h->ctx = take_context(precision);
         BT T = upload(*h->ctx, zn, leg_dims, data, ndata, 6, 2);
         scheme_init(*h->ctx, h->s, TNR_HOTRG_3D, std::move(T), 0.0);
         *out = h.release();
@@ -151,8 +202,7 @@ int tnr_create_multi(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims
             return fail(TNR_ERR_ARG, "ImpurityTRG takes 5 tensors, ImpurityHOTRG 4");
         if (!data || !ndata) return fail(TNR_ERR_ARG, "null tensor list");
         std::unique_ptr<tnr_scheme> h(new tnr_scheme());
-        h->ctx.reset(new Ctx());
-        h->ctx->precision = precision;
+        h->ctx = take_context(precision);
         std::vector<BT> Ts;
         for (int i = 0; i < ntensors; ++i)
             Ts.push_back(upload(*h->ctx, zn, leg_dims, data[i], ndata[i]));
