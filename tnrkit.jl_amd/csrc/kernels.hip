@@ -18,6 +18,15 @@
 
 namespace tnr {
 
+// Pointers taken from descriptor arrays are generic to the compiler, so plain
+// reads through them become flat loads, which also count in lgkmcnt (an LDS
+// wait would then also wait for them).  A global-address-space view makes
+// them global loads, tracked by vmcnt alone.
+using gconst_double = const __attribute__((address_space(1))) double;
+using gmut_double = __attribute__((address_space(1))) double;
+__device__ __forceinline__ gconst_double* as_global(const double* p) { return (gconst_double*)p; }
+__device__ __forceinline__ gmut_double* as_global(double* p) { return (gmut_double*)p; }
+
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
@@ -95,46 +104,55 @@ __global__ __launch_bounds__(256) void k_transpose(const TransDesc* __restrict__
     __shared__ double buf[4096 + 64 * 16];
     __shared__ int64_t soff[64], doff[64];           // batch offsets per sub-tile (-1: none)
     const int t = threadIdx.x;
+    int lo = 0;
     for (int64_t tid = blockIdx.x; tid < tiles; tid += gridDim.x) {
-        int lo = 0, hi = nd - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (descs[mid].tile_start <= tid) lo = mid; else hi = mid - 1;
+        // tiles come in increasing order: continue the search from the last
+        // job (usually the same one or the next)
+        if (descs[lo].tile_start > tid) lo = 0;
+        if (lo + 1 < nd && descs[lo + 1].tile_start <= tid) {
+            int hi = nd - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (descs[mid].tile_start <= tid) lo = mid; else hi = mid - 1;
+            }
         }
         const TransDesc& D = descs[lo];
         const int tx = D.tx, ty = D.ty, sub = D.sub, pitch = tx + 1;
         const int lx = __ffs(tx) - 1, ly = __ffs(ty) - 1, la = lx + ly;   // log2 extents
-        int64_t r = tid - D.tile_start;
-        const int i0 = (int)(r % D.t0) * tx;
+        // a job has fewer than 2^31 tiles and batch entries: 32-bit decode
+        int r = (int)(tid - D.tile_start);
+        const int i0 = (r % D.t0) * tx;
         r /= D.t0;
-        const int i1 = (int)(r % D.t1) * ty;
+        const int i1 = (r % D.t1) * ty;
         r /= D.t1;
-        const int64_t b0 = r * sub;                 // first batch index of this tile
+        const int b0 = r * sub;                     // first batch index of this tile
         __syncthreads();                            // previous tile done with buf / offsets
         if (t < sub) {
-            const int64_t bi = b0 + t;
+            const int bi = b0 + t;
             int64_t so = -1, dof = -1;
             if (bi < D.nb) {
-                int64_t q = bi;
+                int q = bi;
                 so = 0;
                 dof = 0;
                 for (int kk = 0; kk < 4; ++kk) {
-                    const int64_t ik = q % D.bdim[kk];
+                    const int ik = q % D.bdim[kk];
                     q /= D.bdim[kk];
-                    so += ik * D.bs[kk];
-                    dof += ik * D.bd[kk];
+                    so += (int64_t)ik * D.bs[kk];
+                    dof += (int64_t)ik * D.bd[kk];
                 }
             }
             soff[t] = so;
             doff[t] = dof;
         }
         __syncthreads();
+        gconst_double* src = as_global(D.src);
+        gmut_double* dst = as_global(D.dst);
         for (int e = t; e < 4096; e += 256) {       // x along n0 (source-contiguous)
             const int s = e >> la, rem = e & ((1 << la) - 1), j = rem >> lx, x = rem & (tx - 1);
             const int a = i0 + x, c = i1 + j;
             const int64_t so = soff[s];
             if (so >= 0 && a < D.n0 && c < D.n1)
-                buf[s * ty * pitch + j * pitch + x] = D.src[so + a + (int64_t)c * D.ss1];
+                buf[s * ty * pitch + j * pitch + x] = src[so + a + (int64_t)c * D.ss1];
         }
         __syncthreads();
         for (int e = t; e < 4096; e += 256) {       // x along n1 (destination-contiguous)
@@ -142,7 +160,7 @@ __global__ __launch_bounds__(256) void k_transpose(const TransDesc* __restrict__
             const int a = i0 + j, c = i1 + x;
             const int64_t dof = doff[s];
             if (dof >= 0 && a < D.n0 && c < D.n1)
-                D.dst[dof + c + (int64_t)a * D.ds0] = buf[s * ty * pitch + x * pitch + j];
+                dst[dof + c + (int64_t)a * D.ds0] = buf[s * ty * pitch + x * pitch + j];
         }
     }
 }
@@ -204,16 +222,6 @@ __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local
     c.kz = kz;
     return c;
 }
-
-// Pointers taken from descriptor arrays are generic to the compiler, so plain
-// reads through them become flat loads, which also count in lgkmcnt: the
-// LDS wait before each MFMA group would then wait for the next k-tile's
-// prefetch as well.  Reading through a global-address-space view makes them
-// global loads, tracked by vmcnt alone.
-using gconst_double = const __attribute__((address_space(1))) double;
-using gmut_double = __attribute__((address_space(1))) double;
-__device__ __forceinline__ gconst_double* as_global(const double* p) { return (gconst_double*)p; }
-__device__ __forceinline__ gmut_double* as_global(double* p) { return (gmut_double*)p; }
 
 // global -> registers: the tile's BM x KB slab of op(A), KB BM / NT values
 // per thread of an NT-thread workgroup
