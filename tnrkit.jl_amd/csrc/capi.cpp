@@ -489,6 +489,13 @@ struct Slot {
 // cut between the SVD and the contractions) and its kernels overlap the
 // other lanes' kernels.  Every computation is local to one scheme, so the
 // lane a scheme lands in changes nothing in its results.
+// The batch (tnr_run_batch) the current thread works for, if any: set on the
+// calling thread and on its lane threads.
+struct tnr_server;
+namespace {
+thread_local const tnr_server* t_batch_running = nullptr;
+}  // namespace
+
 struct Lane {
     Ctx ctx;
     std::vector<std::unique_ptr<Slot>> active;
@@ -605,8 +612,10 @@ struct tnr_server {
         }
         std::vector<std::thread> th;
         std::vector<std::exception_ptr> err(busy.size());
+        const tnr_server* running = t_batch_running;
         for (size_t i = 1; i < busy.size(); ++i)
-            th.emplace_back([&, i] {
+            th.emplace_back([&, i, running] {
+                t_batch_running = running;
                 try {
                     TNR_HIP(hipSetDevice(busy[i]->ctx.device));
                     step_lane(*busy[i]);
@@ -688,6 +697,36 @@ std::map<std::array<int, 3>, std::unique_ptr<tnr_server>>& batch_servers() {
     static auto* m = new std::map<std::array<int, 3>, std::unique_ptr<tnr_server>>();
     return *m;
 }
+
+// With g_batch_mu held by this thread or by the batch it works for; lane
+// threads of one batch may get here together, hence the second lock.
+std::mutex g_trim_mu;
+void trim_idle_batch_servers(const tnr_server* busy) {
+    std::lock_guard<std::mutex> lk(g_trim_mu);
+    for (auto& kv : batch_servers())
+        if (kv.second && kv.second.get() != busy)
+            for (auto& l : kv.second->lanes) l->ctx.pool.trim();
+}
+
+// Pool fallback on a failed allocation: idle spare contexts and batch engines
+// give their cached buffers back.  A batch running on this thread keeps its
+// own (its pools are in use); one running on another thread is waited for.
+void release_idle_memory() {
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        for (auto& c : spare_contexts()) c->pool.trim();
+    }
+    if (t_batch_running) {
+        trim_idle_batch_servers(t_batch_running);   // its batch holds g_batch_mu
+    } else {
+        std::lock_guard<std::mutex> lk(g_batch_mu);
+        trim_idle_batch_servers(nullptr);
+    }
+}
+
+struct ReleaseHook {
+    ReleaseHook() { g_release_idle_memory = &release_idle_memory; }
+} g_release_hook;
 }  // namespace
 
 extern "C" {
@@ -800,6 +839,10 @@ int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision, double* no
             held.reset(s);
         }
         tnr_server* s = held.get();
+        t_batch_running = s;
+        struct Clear {
+            ~Clear() { t_batch_running = nullptr; }
+        } clear_running;
         s->reset(nruns > 0 ? nruns : 1);
         std::vector<int64_t> ids(nruns);
         for (int i = 0; i < nruns; ++i) {

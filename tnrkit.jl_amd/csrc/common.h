@@ -129,6 +129,11 @@ struct Ctx {
 };
 
 void ensure_device();   // throws TNR_ERR_NODEVICE if no gfx950 GPU
+
+// Called when a device allocation fails even after the failing pool dropped
+// its own cache: frees what idle contexts and idle batch engines hold cached.
+// Set once by the C API layer.
+extern void (*g_release_idle_memory)();
 void set_last_error(const std::string& msg);   // message for tnr_last_error()
 
 }  // namespace tnr

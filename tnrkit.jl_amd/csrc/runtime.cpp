@@ -6,6 +6,8 @@
 
 namespace tnr {
 
+void (*g_release_idle_memory)() = nullptr;
+
 // ------------------------------------------------------------------- Pool
 size_t Pool::bucket(size_t bytes) {
     if (bytes == 0) return 0;
@@ -32,6 +34,11 @@ void* Pool::get(size_t bytes) {
         (void)hipGetLastError();
         trim();
         e = hipMalloc(&p, bytes);
+        if (e != hipSuccess && g_release_idle_memory) {
+            (void)hipGetLastError();
+            g_release_idle_memory();
+            e = hipMalloc(&p, bytes);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             throw Error(TNR_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
