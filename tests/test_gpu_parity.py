@@ -317,3 +317,41 @@ def test_against_golden_extensions(tk):
         assert np.allclose(tk.gu_wen_ratio(sc), c["gu_wen_ratio"], rtol=1e-8)
         assert abs(tk.central_charge(sc, d[-1]) - c["central_charge_n_last"]) < 1e-8
         sc.close()
+
+
+def test_reused_engine_and_contexts_repeat_bitwise(tk):
+    """The batch engine and the contexts of destroyed handles are reused with
+    whatever their pools cached from earlier work: a batch, then a batch of
+    other sizes and symmetries, then the first batch again gives the first
+    norms bit for bit; the same holds for single handles created after
+    handles of other sizes were destroyed.  The last check is a batch that
+    fails (chi above the solver's limit) followed by the first batch again."""
+    a = [tk.Request("BTRG", tk.classical_ising(0.41 + 0.02 * i, symmetry="Trivial"), 24, 10)
+         for i in range(6)]
+    b = [tk.Request("HOTRG", tk.classical_ising(0.44, symmetry="Z2"), 12, 6),
+         tk.Request("BTRG", tk.classical_potts(3, 1.0), 32, 7),
+         tk.Request("TRG", tk.classical_clock(6, 1.1), 20, 5)]
+    first = tk.run_batch(a)
+    tk.run_batch(b)
+    again = tk.run_batch(a)
+    for x, y in zip(first, again):
+        assert np.array_equal(x, y)
+
+    def single(T, chi, n):
+        return gpu_run(tk, tk.BTRG, T, chi, n)
+    T1 = tk.classical_ising(0.43, symmetry="Z2")
+    s1 = single(T1, 16, 8)
+    single(tk.classical_ising(0.5, symmetry="Trivial"), 32, 6)
+    single(tk.classical_potts(3, 0.9), 24, 5)
+    assert np.array_equal(single(T1, 16, 8), s1)
+
+    rng = np.random.default_rng(3)
+    D = 10
+    dense = rng.standard_normal((D, D, D, D))
+    big = tk.SymTensor.from_dense(dense + dense.transpose(3, 2, 1, 0),
+                                  [np.zeros(D, dtype=int)] * 4, 1)
+    with pytest.raises(tk.TNRError):
+        tk.run_batch([tk.Request("TRG", big, 90, 1)] + a[:2])
+    after = tk.run_batch(a)
+    for x, y in zip(first, after):
+        assert np.array_equal(x, y)
