@@ -97,6 +97,7 @@ std::unique_ptr<Ctx> take_context(int precision) {
             }
     }
     if (!c) c.reset(new Ctx());
+    c->ensure_stream();
     c->precision = precision;
     return c;
 }
@@ -105,7 +106,7 @@ std::unique_ptr<Ctx> take_context(int precision) {
 void give_context(std::unique_ptr<Ctx> c) noexcept {
     if (!c) return;
     try {
-        c->sync();
+        c->drop_stream();
         if (c->pool.cached() > kSparePoolBytes) c->pool.trim();
         std::lock_guard<std::mutex> lk(g_ctx_mu);
         auto& v = spare_contexts();

@@ -119,6 +119,10 @@ struct Ctx {
     Ctx();
     ~Ctx();
     void sync();
+    // an idle context gives its stream back (streams share the device's few
+    // hardware queues) and makes a new one before it is used again
+    void drop_stream();
+    void ensure_stream();
     // device -> host copy that waits for the stream (through a pinned
     // bounce buffer, so the DMA engine writes host memory directly)
     void fetch(void* dst, const void* src, size_t bytes);

@@ -141,6 +141,17 @@ Ctx::~Ctx() {
 
 void Ctx::sync() { TNR_HIP(hipStreamSynchronize(stream)); }
 
+void Ctx::drop_stream() {
+    if (!stream) return;
+    TNR_HIP(hipStreamSynchronize(stream));
+    TNR_HIP(hipStreamDestroy(stream));
+    stream = nullptr;
+}
+
+void Ctx::ensure_stream() {
+    if (!stream) TNR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+
 void Ctx::fetch(void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
     if (bytes > bounce_cap_) {
