@@ -68,10 +68,12 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 
 // Contexts of destroyed handles are kept (a few per process) and handed to
 // the next handle on the same device.  A context per handle from scratch
-// means a new stream, a newly pinned staging ring and an empty pool, so every
-// buffer of the handle's first steps would be allocated (and later freed)
-// with the device synchronised each time.  The map of spares is never
-// destroyed: at process exit the HIP runtime may already be gone.
+// means a newly pinned staging ring and an empty pool, so every buffer of
+// the handle's first steps would be allocated (and later freed) with the
+// device synchronised each time.  A kept context holds no stream: idle
+// streams would still take the device's few hardware queues.  The list of
+// spares is never destroyed: at process exit the HIP runtime may already be
+// gone.
 namespace {
 constexpr size_t kSpareContexts = 4;
 constexpr size_t kSparePoolBytes = size_t(8) << 30;   // cache kept with a spare context
