@@ -816,7 +816,6 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
     extern __shared__ double lds[];
     __shared__ double bv[CQT / 64];
     __shared__ int bi[CQT / 64];
-    __shared__ int piv_s;
     __shared__ double dmax_s;
     const JacobiDesc D = descs[blockIdx.x];
     if (D.skip && *D.skip) return;
@@ -863,7 +862,11 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
             bi[w] = ii;
         }
         __syncthreads();
-        if (t == 0) {
+        // every thread reduces the per-wave candidates itself (same order,
+        // same pivot everywhere); bv / bi are next written after this step's
+        // last barrier
+        int p;
+        {
             double b = bv[0];
             int bj = bi[0];
             for (int q = 1; q < CQT / 64; ++q)
@@ -871,10 +874,8 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
                     b = bv[q];
                     bj = bi[q];
                 }
-            piv_s = (b > nfloor && dmax_s > 0.0) ? bj : -1;
+            p = (b > nfloor && dmax_s > 0.0) ? bj : -1;
         }
-        __syncthreads();
-        const int p = piv_s;
         if (p < 0) {
             rank = j;
             break;
@@ -899,11 +900,12 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
             }
             __syncthreads();
         }
+        // A[j, j] is read by every thread here and overwritten only after
+        // the next barrier; the trailing update does not read it
         const double r = sqrt(A[j + j * ld]);
-        __syncthreads();
         for (int k = j + 1 + t; k < n; k += CQT) A[j + k * ld] /= r;
-        if (t == 0) A[j + j * ld] = r;
         __syncthreads();
+        if (t == 0) A[j + j * ld] = r;
         // trailing update: rows across the lanes, columns k = j+1+w (mod 4)
         // per wave, four columns per pass so their loads overlap
         constexpr int WS = CQT / 64;
