@@ -640,6 +640,7 @@ def main():
                      "pass": bool(errs) and max(errs) <= 1e-9},
         "plain_fp64_one_at_a_time": {"value": plain_value, "unit": "free energies/s"},
         "speedup_vs_plain": value / plain_value if plain_value else None,
+        "steps_per_s": value * max(a.maxiter, 1),   # coarse-graining steps of all runs
         "batch_latency_s": ms_per_step / 1e3,
         "model_flops_per_free_energy": btrg_model_flops(a.chi, a.maxiter),
         "end_to_end_model_tflops": btrg_model_flops(a.chi, a.maxiter) * value / 1e12,
