@@ -218,6 +218,20 @@ def workload_c2(tk, maxiter):
             "cases": cases}
 
 
+def workload_plain(tk, chi, maxiter, n):
+    """The plain one-run-at-a-time FP64 path on the headline's first n betas:
+    each BTRG run is its own scheme handle, created, run and destroyed."""
+    betas = np.random.default_rng(SEED).uniform(0.35, 0.55, n)
+    times = []
+    for b in betas:
+        sc = tk.BTRG(tk.classical_ising(float(b), symmetry="Trivial"))
+        _, dt = _timed(lambda: tk.run(sc, tk.truncrank(chi), tk.maxiter(maxiter), verbosity=0))
+        sc.close()
+        times.append(dt)
+    return {"workload": f"plain BTRG runs one at a time, chi={chi}, maxiter={maxiter}",
+            "runs": n, "s_per_run": times, "free_energies_per_s": n / sum(times)}
+
+
 def workload_c3(tk, maxiter):
     """BTRG with Z2 Ising at beta_c and Z3 Potts at ln(1+sqrt 3), chi = 64."""
     cases = []
@@ -557,9 +571,10 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrivals per second")
-    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs", "atrg", "3d"],
+    ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs", "atrg", "3d",
+                                           "plain"],
                     default="scan",
-                    help="scan = the headline beta scan; c2-c4, imp, obs, atrg and 3d print a "
+                    help="scan = the headline beta scan; c2-c4, imp, obs, atrg, 3d and plain print a "
                          "report line of their own")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -579,6 +594,8 @@ def main():
             r = workload_obs(tk, 5, not a.no_cpu)
         elif a.workload == "3d":
             r = workload_3d(tk, a.maxiter, 0.0 if a.no_cpu else a.cpu_seconds)
+        elif a.workload == "plain":
+            r = workload_plain(tk, a.chi, a.maxiter, 6)
         elif a.workload == "atrg":
             r = workload_atrg(tk, a.maxiter, 64 if a.batch == 128 else a.batch,
                               0.0 if a.no_cpu else a.cpu_seconds)

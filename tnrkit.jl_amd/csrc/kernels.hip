@@ -619,13 +619,10 @@ __global__ __launch_bounds__(256) void k_lincomb(const LinDesc* __restrict__ des
 // the solve stops when none violates the criterion.  The input is
 // symmetrised on load ((G + G^T)/2).  Output: eigenvalues by |lambda|
 // descending, eigenvectors as columns of W in the same order.
-// JT threads per matrix, n <= NMAX (at most 2 JT pair blocks); every G block
-// and W entry is computed the same way whichever thread owns it, so both
-// instantiations give the same bits.
-template <int JT, int NMAX>
+constexpr int JT = 1024;
+
 __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ descs, int nd,
                                                double tol, int max_sweeps) {
-    static_assert((NMAX / 2) * (NMAX / 2 + 1) / 2 <= 2 * JT, "two pair blocks per thread");
     extern __shared__ double lds[];
     const JacobiDesc D = descs[blockIdx.x];
     if (D.skip && *D.skip) return;
@@ -733,7 +730,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
             // W <- W J: this thread's (pair, row) tasks touch distinct
             // columns, so all their loads are issued before any store
             {
-                constexpr int WT = (NMAX / 2) * NMAX / JT + 1;   // tasks per thread, max
+                constexpr int WT = (LMAX / 2) * LMAX / JT + 1;   // tasks per thread, max
                 int wa[WT], wb[WT];
                 double wp[WT], wq[WT], wc[WT], ws[WT];
                 bool wl[WT];
@@ -1715,24 +1712,14 @@ size_t jacobi_lds_bytes(int n) {
 void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, double tol,
                    int max_sweeps) {
     if (nd <= 0) return;
-    TNR_REQUIRE(nmax >= 1 && nmax <= LMAX, TNR_ERR_INTERNAL, "Jacobi block above LMAX");
     // once per process (thread-safe static initialisation)
     static const hipError_t attr = hipFuncSetAttribute(
-        (const void*)k_jacobi<1024, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
         (int)jacobi_lds_bytes(LMAX));
     TNR_HIP(attr);
-    static const hipError_t attr_small = hipFuncSetAttribute(
-        (const void*)k_jacobi<512, 64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-        (int)jacobi_lds_bytes(64));
-    TNR_HIP(attr_small);
     ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
-    // blocks up to 64: half the threads, so each round's barriers span 8 waves
-    if (nmax <= 64)
-        TNR_LAUNCH(ps, (k_jacobi<512, 64>), dim3(nd), dim3(512), jacobi_lds_bytes(nmax), st,
-                   d_descs, nd, tol, max_sweeps);
-    else
-        TNR_LAUNCH(ps, (k_jacobi<1024, LMAX>), dim3(nd), dim3(1024), jacobi_lds_bytes(nmax), st,
-                   d_descs, nd, tol, max_sweeps);
+    TNR_LAUNCH(ps, k_jacobi, dim3(nd), dim3(JT), jacobi_lds_bytes(nmax), st, d_descs, nd, tol,
+                       max_sweeps);
     TNR_HIP(hipGetLastError());
     if (ps.active && prof_level() >= 2) {
         // profiling only: the work a launch did depends on the sweeps each
