@@ -50,15 +50,20 @@ def device_sync(tk):
 
 
 def read_profile(tk):
-    """Per kernel class: launches, summed device time (each launch's dispatch
-    start / end timestamps since tnr_profile_enable(1)), algorithmic flops
-    and bytes; switches the recording off."""
+    """Per kernel class since tnr_profile_enable(1): launches, summed launch
+    durations (each launch's dispatch start / end timestamps), the union of
+    the launch intervals (device-busy time of the class; launches on the two
+    lanes' streams overlap), algorithmic flops (GEMM: counted on the device
+    for the problems that ran) and bytes; switches the recording off."""
     L = tk.lib()
     out = {}
     for k, name in enumerate(KIND_NAMES):
         n, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
-        L.tnr_profile_read(k, C.byref(n), C.byref(ms), C.byref(fl), C.byref(by))
-        out[name] = {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+        busy, planned = C.c_double(), C.c_double()
+        L.tnr_profile_read(k, C.byref(n), C.byref(ms), C.byref(fl), C.byref(by), C.byref(busy),
+                           C.byref(planned))
+        out[name] = {"launches": n.value, "ms": ms.value, "busy_ms": busy.value,
+                     "flops": fl.value, "flops_planned": planned.value, "bytes": by.value}
     L.tnr_profile_enable(0)
     return out
 
@@ -69,37 +74,50 @@ def profile_batch(tk, reqs, precision):
     return read_profile(tk)
 
 
-def _gemm_roofline(prof, name):
+def _gemm_roofline(prof, name, steps=1):
+    """achieved = device-counted GEMM flops of the problems that ran / the
+    union of the GEMM launch intervals (the time some GEMM was on the device;
+    launches of the two lanes overlap, so summed durations would double-count
+    shared time); beside it the same flops over the summed launch durations,
+    the per-launch figure a kernel trace gives."""
     d = prof[name]
     peak = PEAK["fp32_mfma"] if name == "gemm_split32" else PEAK["fp64_mfma"]
     flops = d["flops"] * (3 if name == "gemm_split32" else 1)   # 3 MFMA products per FMA
-    ach = flops / (d["ms"] * 1e-3) if d["ms"] else 0.0
+    ach = flops / (d["busy_ms"] * 1e-3) if d["busy_ms"] else 0.0
+    per_launch = flops / (d["ms"] * 1e-3) if d["ms"] else 0.0
+    n = max(1, d["launches"])
     return {"kernel": "k_gemm_split32" if name == "gemm_split32" else "k_gemm_f64",
             "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": ach / peak, "launches": d["launches"], "avg_launch_ms": d["ms"] / max(1, d["launches"])}
+            "frac": ach / peak, "launches": d["launches"],
+            "flops_per_launch": d["flops"] / n,
+            "flops_planned_per_launch": d["flops_planned"] / n,
+            "busy_ms_per_step": d["busy_ms"] / steps,
+            "avg_launch_ms": d["ms"] / n,
+            "achieved_over_summed_launch_time": per_launch / 1e12}
 
 
-def roofline_from(prof, precision, traffic):
-    """Roofline of the kernel class with the most device time; achieved =
-    algorithmic work of its launches / their summed duration (HIP events on
-    the library's stream)."""
-    total_ms = sum(v["ms"] for v in prof.values())
-    name, d = max(prof.items(), key=lambda kv: kv[1]["ms"])
+def roofline_from(prof, precision, traffic, steps=1):
+    """Roofline of the kernel class with the most device-busy time (union of
+    its launch intervals, HIP dispatch timestamps on the lanes' streams)."""
+    total = sum(v["busy_ms"] for v in prof.values())
+    name, d = max(prof.items(), key=lambda kv: kv[1]["busy_ms"])
+    n = max(1, d["launches"])
     if name.startswith("gemm"):
-        r = _gemm_roofline(prof, name)
+        r = _gemm_roofline(prof, name, steps)
     elif name == "jacobi":
-        ach = d["flops"] / (d["ms"] * 1e-3) if d["ms"] else 0.0
+        ach = d["flops"] / (d["busy_ms"] * 1e-3) if d["busy_ms"] else 0.0
         r = {"kernel": "k_jacobi", "bound": "mfma", "achieved": ach / 1e12,
              "peak": PEAK["fp64_mfma"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["fp64_mfma"],
-             "launches": d["launches"], "avg_launch_ms": d["ms"] / max(1, d["launches"]),
+             "launches": d["launches"], "avg_launch_ms": d["ms"] / n,
+             "busy_ms_per_step": d["busy_ms"] / steps,
              "note": "FP64 VALU rotations (no MFMA); MI355X FP64 vector and matrix peaks are "
                      "both 78.6 TF/s; work = 18n flops per rotation, sweeps read back"}
     else:
-        ach = d["bytes"] / (d["ms"] * 1e-3) if d["ms"] else 0.0
+        ach = d["bytes"] / (d["busy_ms"] * 1e-3) if d["busy_ms"] else 0.0
         r = {"kernel": name, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK["hbm"] / 1e9,
              "unit": "GB/s", "frac": ach / PEAK["hbm"], "launches": d["launches"],
-             "avg_launch_ms": d["ms"] / max(1, d["launches"])}
-    r["share_of_device_time"] = d["ms"] / total_ms if total_ms else None
+             "avg_launch_ms": d["ms"] / n, "busy_ms_per_step": d["busy_ms"] / steps}
+    r["share_of_device_busy_time"] = d["busy_ms"] / total if total else None
     r["traffic"] = traffic.get(r["kernel"]) if traffic else None
     return r
 
@@ -116,42 +134,104 @@ def load_traffic():
     return {k: v["bytes_per_launch"] for k, v in t.get("kernels", {}).items()}
 
 
+def host_cpu_info():
+    """What the host offers: nproc, the CPUs this process may run on, the
+    cgroup's CPU quota (cores) where one is set, and the CPU model."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity"] = os.cpu_count()
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_quota_cores"] = quota
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info["model"] = model
+    return info
+
+
+def host_cpu_share(info, cap=16):
+    """Worker processes for the CPU leg: the CPUs this process may use, the
+    cgroup quota, and at most `cap` (a GPU box shares its host: one GPU's
+    share is 16 cores, whatever nproc reports)."""
+    n = info["affinity"] or 1
+    if info["cgroup_quota_cores"]:
+        n = min(n, max(1, int(info["cgroup_quota_cores"])))
+    return max(1, min(n, cap))
+
+
 def _cpu_run(args):
-    """One oracle BTRG run on one core: time steps until the budget is spent
-    and extrapolate the remaining steps at the steady-state step time."""
+    """One oracle BTRG run: every step up to the steady bond dimension is
+    timed once; once D = chi every step costs the same, so after one warm-up
+    steady step the best of five steady steps prices the remaining ones
+    (each timed step still advances the run).  Returns seconds per free
+    energy and the steps actually run."""
     chi, maxiter, beta, budget_s = args
     from oracle import models as om
     from oracle import schemes as osch
     sc = osch.BTRG(om.classical_ising_trivial(beta))
+    t0 = time.perf_counter()
     sc.finalize()
-    times = []
+    total = time.perf_counter() - t0
+    D, done, steady = 2, 0, []
     t_all = time.perf_counter()
-    for _ in range(maxiter):
+    while done < maxiter:
         t0 = time.perf_counter()
         sc.step(chi)
         sc.finalize()
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_all > budget_s:
+        dt = time.perf_counter() - t0
+        done += 1
+        D2 = min(chi, D * D)
+        if D2 < chi or D < chi:
+            total += dt            # bond dimension still growing: timed once
+        else:
+            steady.append(dt)      # steady state: warm-up + best of five
+            if len(steady) == 6:
+                break
+        D = D2
+        if time.perf_counter() - t_all > budget_s and len(steady) >= 2:
             break
-    n = len(times)
-    per_fe = sum(times) + (float(np.median(times[-3:])) * (maxiter - n) if n < maxiter else 0.0)
-    return per_fe, n
+    if steady:
+        best = min(steady[1:]) if len(steady) > 1 else steady[0]
+        total += best * (maxiter - (done - len(steady)))
+    return total, done
 
 
 def cpu_baseline(chi, maxiter, betas, budget_s):
-    """FP64 oracle (numpy -> OpenBLAS dgemm / LAPACK gesdd) as a beta scan:
-    one single-threaded process per core, each on its own beta, bounded to
-    budget_s; throughput = sum over processes of 1 / (time per free energy)."""
+    """FP64 oracle (numpy -> OpenBLAS dgemm / LAPACK gesdd) on the host:
+    * beta-scan throughput: one single-threaded process per core of this
+      process's CPU share, each on its own beta; value = sum over processes
+      of 1 / (seconds per free energy);
+    * a single run with OpenBLAS threads = that share (the latency figure)."""
     import multiprocessing as mp
-    procs = max(1, min(16, os.cpu_count() or 1, len(betas)))
+    info = host_cpu_info()
+    procs = min(host_cpu_share(info), len(betas))
     saved = {k: os.environ.get(k) for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS",
                                              "MKL_NUM_THREADS")}
-    for k in saved:
-        os.environ[k] = "1"
     try:
+        for k in saved:
+            os.environ[k] = "1"
         with mp.get_context("spawn").Pool(procs) as pool:
             res = pool.map(_cpu_run, [(chi, maxiter, float(betas[i]), budget_s)
                                       for i in range(procs)])
+        for k in saved:
+            os.environ[k] = str(procs)
+        with mp.get_context("spawn").Pool(1) as pool:
+            single = pool.map(_cpu_run, [(chi, maxiter, float(betas[0]), budget_s)])[0]
     finally:
         for k, v in saved.items():
             if v is None:
@@ -160,11 +240,35 @@ def cpu_baseline(chi, maxiter, betas, budget_s):
                 os.environ[k] = v
     value = sum(1.0 / r[0] for r in res)
     steps = min(r[1] for r in res)
-    sample = (f"{procs} single-threaded oracle BTRG chi={chi} runs on distinct betas, "
-              f"{steps}+ of {maxiter} steps each timed within {budget_s:.0f} s, the rest "
-              f"extrapolated at the steady-state step time")
+    sample = (f"{procs} single-threaded oracle BTRG chi={chi} runs on distinct betas, each "
+              f"timed over its first {steps}+ of {maxiter} steps: steps before the bond "
+              f"dimension reaches chi timed once, then a warm-up steady step and the best of "
+              f"five steady steps pricing the rest")
     return {"value": value, "unit": "free energies/s", "cores": procs, "kind": "port",
-            "sample": sample}
+            "sample": sample, "host": info,
+            "single_run": {"free_energies_per_s": 1.0 / single[0], "openblas_threads": procs,
+                           "seconds_per_free_energy": single[0]}}
+
+
+def oracle_scan_free_energies(chi, maxiter, betas):
+    """(index, f) of the headline scan's runs whose FP64 oracle free energy
+    is stored in tests/golden/oracle_configs.json (matched by beta, chi and
+    maxiter)."""
+    p = os.path.join(ROOT, "tests", "golden", "oracle_configs.json")
+    if not os.path.exists(p):
+        return []
+    with open(p) as f:
+        g = json.load(f)
+    out = []
+    for c in g.values():
+        if c["scheme"] != "BTRG" or c["model"] != "ising_trivial" or c["chi"] != chi or \
+                c["maxiter"] != maxiter:
+            continue
+        for i, b in enumerate(betas):
+            if float(b) == c["beta"]:
+                out.append((i, c["f"]))
+                break
+    return out
 
 
 def btrg_model_flops(chi, maxiter):
@@ -642,6 +746,11 @@ def main():
         fp = tk.free_energy(d, float(betas[i]))
         errs.append(abs(f_batch[i] - fp) / abs(fp))
     plain_value = min(a.check, a.batch) / t_plain if t_plain else None
+    # the FP64 oracle's free energies of the scan's first runs (committed
+    # fixtures, computed on the CPU by tests/golden/make_golden.py configs)
+    oracle_errs = []
+    for i, f_or in oracle_scan_free_energies(a.chi, a.maxiter, betas):
+        oracle_errs.append(abs(f_batch[i] - f_or) / abs(f_or))
 
     out = {
         "metric": METRIC, "value": value, "unit": "free energies/s", "n_gpus": 1,
@@ -653,8 +762,11 @@ def main():
                    "batch": a.batch, "chi": a.chi, "maxiter": a.maxiter, "seed": SEED,
                    "precision": a.precision},
         "accuracy": {"max_rel_df_vs_plain_fp64": max(errs) if errs else None,
-                     "runs_checked": len(errs), "gate": 1e-9,
-                     "pass": bool(errs) and max(errs) <= 1e-9},
+                     "runs_checked": len(errs),
+                     "max_rel_df_vs_fp64_oracle": max(oracle_errs) if oracle_errs else None,
+                     "oracle_runs_checked": len(oracle_errs), "gate": 1e-9,
+                     "pass": bool(errs) and max(errs) <= 1e-9 and
+                             (not oracle_errs or max(oracle_errs) <= 1e-9)},
         "plain_fp64_one_at_a_time": {"value": plain_value, "unit": "free energies/s"},
         "speedup_vs_plain": value / plain_value if plain_value else None,
         "steps_per_s": value * max(a.maxiter, 1),   # coarse-graining steps of all runs
@@ -668,10 +780,10 @@ def main():
         out["latency_s"] = {"p50": sv["p50_s"], "p99": sv["p99_s"], "source": "serving stream"}
         out["serving"] = sv
     if prof is not None:
-        out["roofline"] = roofline_from(prof, a.precision, load_traffic())
+        out["roofline"] = roofline_from(prof, a.precision, load_traffic(), a.steps)
         gname = "gemm_split32" if a.precision == "split32" else "gemm_f64"
-        out["roofline_gemm"] = _gemm_roofline(prof, gname)
-        out["kernel_time_ms_per_step"] = {k: round(v["ms"] / a.steps, 3)
+        out["roofline_gemm"] = _gemm_roofline(prof, gname, a.steps)
+        out["kernel_busy_ms_per_step"] = {k: round(v["busy_ms"] / a.steps, 3)
                                           for k, v in prof.items() if v["launches"]}
         # the same batch once more on a single lane: no launch then shares the
         # device with the other lane's kernels, so this is the GEMM's own rate

@@ -63,12 +63,20 @@ typedef struct tnr_scheme tnr_scheme;
  * BTRG(T, k) (src/schemes/btrg.jl:39-48; k = -0.5 is the reference default)
  * and HOTRG(T) (src/schemes/hotrg.jl:31-33).  Copies T to HBM.  `k` is read
  * for BTRG only.  Legs 1/4 and 2/3 must carry the same spaces (the finalize
- * trace connects them). */
+ * trace connects them).  A tensor with NaN or infinite entries is refused
+ * with TNR_ERR_NONFINITE (here and in every call that takes a tensor). */
 int tnr_create(tnr_scheme** out, int kind, int zn, const int32_t* leg_dims,
                const double* data, size_t ndata, double k, int precision);
 
 /* step!(scheme, truncrank(chi)): src/schemes/trg.jl:35-41,
- * src/schemes/btrg.jl:60-94, src/schemes/hotrg.jl:154-160. */
+ * src/schemes/btrg.jl:60-94, src/schemes/hotrg.jl:154-160.
+ * Where the reference's LAPACK-backed svd_trunc / eigh_trunc! would raise
+ * (src/schemes/btrg.jl:61,72, src/schemes/hotrg.jl:103,110), the step fails:
+ * TNR_ERR_NOCONV when a truncated SVD or eigensolve stops at its iteration
+ * or sweep cap unconverged (tnr_solver_config in tnr_kernels.h sets the
+ * caps), TNR_ERR_NONFINITE when it meets NaN / infinite values.  The handle's
+ * tensor is then undefined, and tnr_step, tnr_finalize and tnr_run return
+ * the same status until tnr_set_tensor replaces the tensor. */
 int tnr_step(tnr_scheme* s, int chi);
 
 /* finalize!(scheme): src/utility/finalize.jl:4-8 (TRG, HOTRG) and :10-14
@@ -189,6 +197,8 @@ typedef struct {
 } tnr_run_desc;
 
 /* norms: concatenation over runs of (maxiter + finalize_beginning) doubles.
+ * If a run fails (see tnr_server_poll) the call returns its status and
+ * tnr_last_error names the run.
  * The batch engine behind this call (its streams, device memory pools and
  * pinned staging rings) stays alive for the life of the process and is
  * reused by the next call on the same device with the same precision and
@@ -206,14 +216,17 @@ typedef struct tnr_server tnr_server;
 int tnr_server_create(tnr_server** out, size_t hbm_budget_bytes, int max_slots,
                       int precision);
 /* Queue one run; *id receives a ticket.  Fails with TNR_ERR_BUDGET if the
- * request alone exceeds the budget. */
+ * request alone exceeds the budget, TNR_ERR_NONFINITE for a tensor with NaN
+ * or infinite entries. */
 int tnr_server_submit(tnr_server* s, const tnr_run_desc* req, int64_t* id);
 /* Advance every resident scheme by one coarse-graining step, admitting
  * queued requests first.  *active receives the number still in flight. */
 int tnr_server_pump(tnr_server* s, int* active);
 /* Fetch a finished result: TNR_ERR_BUSY if not finished.  norms must hold
  * maxiter + finalize_beginning doubles; t_admit/t_done are seconds since
- * server creation. */
+ * server creation.  A run whose step failed (TNR_ERR_NOCONV,
+ * TNR_ERR_NONFINITE, see tnr_step) ends there and returns that status with
+ * the norms written so far; the other resident runs go on unaffected. */
 int tnr_server_poll(tnr_server* s, int64_t id, double* norms, int* n_out,
                     double* t_submit, double* t_done);
 int tnr_server_stats(tnr_server* s, size_t* hbm_in_use, int* resident,

@@ -72,13 +72,18 @@ int tnr_kernel_jacobi(int count, int n, const double* G, double* W,
  * by its dispatch's start and end timestamps (the events of
  * hipExtLaunchKernelGGL, no marker packets between launches); a class call
  * of two kernels by events recorded around both.  tnr_profile_enable(1)
- * clears and starts it without any host synchronisation, (2) also reads
- * back after each Jacobi launch the sweeps it took (its flops), (0) stops
- * it.  Classes: 0 FP64 GEMM, 1 split-FP32 GEMM, 2 split-K reduce, 3 Jacobi,
- * 4 permute/copy, 5 column norms, 6 gather, 7 trace/normalise, 8 random
- * fill, 9 other.  flops/bytes are the algorithmic work the launches were
- * asked to do (GEMM: 2MNK summed over the grouped problems; copies: bytes
- * read + written). */
+ * synchronises the device, clears the totals and starts recording (no host
+ * synchronisation while recording), (2) also reads back after each Jacobi
+ * launch the sweeps it took (its flops), (0) stops it.  Classes: 0 FP64
+ * GEMM, 1 split-FP32 GEMM, 2 split-K reduce, 3 Jacobi, 4 permute/copy,
+ * 5 column norms, 6 gather, 7 trace/normalise, 8 random fill, 9 other.
+ * tnr_profile_read: *ms sums the launch durations, *busy_ms is the union of
+ * the launch intervals (launches of one class on different streams
+ * overlap, so busy_ms <= ms and <= the wall time).  *flops / *bytes are the
+ * algorithmic work: GEMM flops 2MNK are counted on the device per output
+ * tile, for the problems that actually ran (a problem the solver has
+ * finished skips its tiles), *flops_planned is what the launches were asked
+ * to do; copies count bytes read + written. */
 int tnr_profile_enable(int on);
 /* GEMM tile-shape autotuner: on = 1 (default; TNR_AUTOTUNE=0 in the
  * environment starts it off) times each tile shape on the first launches of
@@ -86,10 +91,20 @@ int tnr_profile_enable(int on);
  * rule.  Every call forgets what was learned.  Tile shapes never change
  * results: each output element is summed in the same order under all. */
 int tnr_autotune_enable(int on);
+/* Solver limits and thresholds of the truncated SVD / eigensolver: the
+ * defaults, then the TNR_SOLVER_CFG environment variable, then `spec`
+ * ("key=value,..."; NULL or "" = defaults and environment only).  Keys:
+ * max_iters (block iterations, default 80), sweeps / sweeps_iter (Jacobi
+ * sweep caps of final and in-loop solves, 40), tol_iter (in-loop Jacobi
+ * threshold, 1e-10), conv_target (1e-14), block_mult (2), block_extra (2),
+ * cholqr_y (1), cholqr_z (0).  A solve that reaches max_iters or a sweep cap
+ * unconverged fails its step with TNR_ERR_NOCONV.  Not thread-safe: call it
+ * while no other entry point runs. */
+int tnr_solver_config(const char* spec);
 /* hipDeviceSynchronize() on the current device (timing brackets). */
 int tnr_device_sync(void);
 int tnr_profile_read(int kind, int64_t* launches, double* ms, double* flops,
-                     double* bytes);
+                     double* bytes, double* busy_ms, double* flops_planned);
 
 #ifdef __cplusplus
 }

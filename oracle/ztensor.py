@@ -167,14 +167,15 @@ def block_row_rank(t: ZT, ncod: int, rows):
 class Trunc:
     """Diagnostics of one truncated factorization (the sigma gap log)."""
 
-    __slots__ = ("kept", "first_dropped", "err", "sectors", "n")
+    __slots__ = ("kept", "first_dropped", "err", "sectors", "n", "top")
 
-    def __init__(self, kept, first_dropped, err, sectors, n):
+    def __init__(self, kept, first_dropped, err, sectors, n, top=0.0):
         self.kept = kept            # smallest kept value
         self.first_dropped = first_dropped  # largest dropped value (0 if none)
         self.err = err              # 2-norm of the dropped values
         self.sectors = sectors      # kept count per coupled charge
         self.n = n                  # matrix dimension (rows)
+        self.top = top              # largest value over all blocks
 
     def rel_gap(self):
         if self.first_dropped == 0.0 or self.kept == 0.0:
@@ -307,7 +308,7 @@ def svd_trunc(t: ZT, ncod: int, chi: int, gauge: str = "lapack"):
     info = Trunc(float(flat[keep].min()) if len(keep) else 0.0,
                  float(flat[drop].max()) if len(drop) else 0.0,
                  float(np.linalg.norm(flat[drop])) if len(drop) else 0.0,
-                 sectors, M.shape[0])
+                 sectors, M.shape[0], float(flat.max()) if len(flat) else 0.0)
     Ut = ZT(U.reshape(*rshape, newdim), [t.q[i] for i in range(ncod)] + [(-qnew) % t.N], t.N)
     Vtt = ZT(Vt.reshape(newdim, *cshape), [qnew] + [t.q[i] for i in range(ncod, nl)], t.N)
     return Ut, s_out, Vtt, info
@@ -359,7 +360,7 @@ def eigh_trunc(t: ZT, ncod: int, chi: int):
     eps = float(np.linalg.norm(flat[drop])) if len(drop) else 0.0
     info = Trunc(float(flat[keep].min()) if len(keep) else 0.0,
                  float(flat[drop].max()) if len(drop) else 0.0,
-                 eps, sectors, n)
+                 eps, sectors, n, float(np.abs(flat).max()) if len(flat) else 0.0)
     Ut = ZT(U.reshape(*rshape, newdim), [t.q[i] for i in range(ncod)] + [(-qnew) % t.N], t.N)
     return lam, Ut, eps, info
 

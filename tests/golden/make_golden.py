@@ -14,6 +14,15 @@ python tests/golden/make_golden.py).
   SVD gauge) and ImpurityHOTRG rows, and cft_data / ground_state_degeneracy /
   gu_wen_ratio / central_charge after fixed runs
   (python tests/golden/make_golden.py extensions writes only this file).
+* oracle_configs.json - norm vectors at the benchmark configurations' sizes
+  (BTRG chi=32 x 30 at three betas of the seeded beta scan; TRG and HOTRG
+  on the Trivial Ising tensor at beta_c, chi 32 and 48, 30 steps; BTRG Z2
+  Ising and Z3 Potts at chi 64, 30 steps; the first 16 runs of the seeded
+  512-run scan with chi in {16, 24, 32, 48}), each with its truncation log:
+  per factorization the matrix size, the largest value, the smallest kept and
+  the largest dropped value and the 2-norm of the dropped ones
+  (python tests/golden/make_golden.py configs [name ...] writes only this
+  file; names select cases and merge into the existing file).
 """
 from __future__ import annotations
 
@@ -131,8 +140,80 @@ def extensions():
     print("wrote", len(out), "cases")
 
 
+# seeds of the benchmark workloads: the headline beta scan and the 512-run
+# mixed-chi scan draw from numpy default_rng(SCAN_SEED) and (SCAN_SEED + 4)
+SCAN_SEED = 20260512
+
+
+def config_cases():
+    """(name, scheme, model spec, chi, maxiter) at the benchmark sizes."""
+    import numpy as np
+    cases = []
+    betas = np.random.default_rng(SCAN_SEED).uniform(0.35, 0.55, 128)
+    for i in range(3):
+        cases.append((f"scan_btrg_trivial_ising_run{i}_chi32_30", "BTRG",
+                      ("ising_trivial", float(betas[i])), 32, 30))
+    for kind in ("TRG", "HOTRG"):
+        for chi in (32, 48):
+            cases.append((f"c2_{kind.lower()}_trivial_ising_betac_chi{chi}_30", kind,
+                          ("ising_trivial_betac", None), chi, 30))
+    cases.append(("c3_btrg_z2_ising_betac_chi64_30", "BTRG", ("ising_z2", None), 64, 30))
+    cases.append(("c3_btrg_z3_potts_betac_chi64_30", "BTRG", ("potts_z3", None), 64, 30))
+    rng = np.random.default_rng(SCAN_SEED + 4)
+    b4 = rng.uniform(0.35, 0.55, 512)
+    c4 = rng.choice([16, 24, 32, 48], 512)
+    for i in range(16):
+        cases.append((f"c4_btrg_trivial_ising_run{i:02d}_chi{int(c4[i])}_30", "BTRG",
+                      ("ising_trivial", float(b4[i])), int(c4[i]), 30))
+    return cases
+
+
+def config_model(spec):
+    name, beta = spec
+    if name == "ising_trivial_betac":
+        return om.classical_ising_trivial(om.ising_betac_mp)
+    return model(spec)
+
+
+def _merge_into(path, name, entry):
+    """Read-merge-write of one case under an exclusive lock (several
+    generator processes may run side by side on disjoint cases)."""
+    import fcntl
+    with open(path + ".lock", "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        out = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                out = json.load(f)
+        out[name] = entry
+        with open(path + ".tmp", "w") as f:
+            json.dump(dict(sorted(out.items())), f, indent=1)
+        os.replace(path + ".tmp", path)
+
+
+def configs(names=()):
+    import time
+    path = os.path.join(HERE, "oracle_configs.json")
+    for name, kind, spec, chi, n in config_cases():
+        if names and name not in names:
+            continue
+        t0 = time.time()
+        sc = {"TRG": osch.TRG, "BTRG": osch.BTRG, "HOTRG": osch.HOTRG}[kind](config_model(spec))
+        d = osch.run(sc, chi, n)
+        beta = float(om.ising_betac if spec[1] is None and spec[0] != "potts_z3"
+                     else om.potts_betac(3) if spec[1] is None else spec[1])
+        _merge_into(path, name, {
+            "scheme": kind, "model": spec[0], "beta": spec[1], "chi": chi, "maxiter": n,
+            "norms": [float(x) for x in d],
+            "f": float(osch.free_energy(d, beta, sc.scalefactor)),
+            "cuts": [[int(c.n), c.top, c.kept, c.first_dropped, c.err] for c in sc.log]})
+        print(name, f"{time.time() - t0:.1f} s", flush=True)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["extensions"]:
         extensions()
+    elif sys.argv[1:2] == ["configs"]:
+        configs(tuple(sys.argv[2:]))
     else:
         main()

@@ -96,3 +96,97 @@ def test_chi_beyond_the_solver_limit_is_an_error_and_the_engine_recovers(tk):
     data = tk.run(b, tk.truncrank(16), tk.maxiter(25), verbosity=0)
     f = tk.free_energy(data, tk.ising_betac)
     assert abs(f / -2.1096504926141826902647832 - 1) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# Failures the reference raises from its LAPACK-backed factorizations
+# (src/schemes/btrg.jl:61,72; src/schemes/hotrg.jl:103,110), forced from the
+# host through the solver's own caps: every kernel still exits normally.
+
+@pytest.fixture
+def solver_caps(tk):
+    yield tk.solver_config
+    tk.solver_config("")
+
+
+def test_block_iteration_cap_is_reported(tk, solver_caps):
+    """max_iters=2: the chi=32 SVDs of 256 x 256 and larger matrices (block
+    iteration) stop unconverged; the step fails with TNR_ERR_NOCONV, the
+    handle keeps failing until its tensor is replaced, and with the defaults
+    restored the same handle runs to the oracle's norms."""
+    from oracle import models as om, schemes as osch
+    T = tk.classical_ising(0.44, symmetry="Trivial")
+    s = tk.BTRG(T)
+    solver_caps("max_iters=2")
+    with pytest.raises(tk.TNRError) as e:
+        tk.run(s, tk.truncrank(32), tk.maxiter(4), verbosity=0)
+    assert e.value.status == -5 and "converge" in str(e.value)
+    solver_caps("")
+    with pytest.raises(tk.TNRError) as e:          # sticky until set_tensor
+        tk.run(s, tk.truncrank(32), tk.maxiter(4), verbosity=0)
+    assert e.value.status == -5
+    dims = np.ascontiguousarray(T.leg_dims, dtype=np.int32).ravel()
+    assert tk.lib().tnr_set_tensor(s._h, _ptr(dims, I32P), _ptr(T.data, DP), T.data.size) == 0
+    d = np.array(tk.run(s, tk.truncrank(32), tk.maxiter(4), verbosity=0))
+    ref = np.array(osch.run(osch.BTRG(om.classical_ising_trivial(0.44)), 32, 4))
+    assert np.max(np.abs(d / ref - 1)) < 1e-10
+    s.close()
+
+
+def test_jacobi_sweep_cap_is_reported(tk, solver_caps):
+    """sweeps=1: the final Jacobi solves of the direct (small-block) SVDs
+    stop after one sweep with rotations still due -> TNR_ERR_NOCONV."""
+    solver_caps("sweeps=1")
+    s = tk.TRG(tk.classical_ising(symmetry="Z2"))
+    with pytest.raises(tk.TNRError) as e:
+        tk.run(s, tk.truncrank(8), tk.maxiter(3), verbosity=0)
+    assert e.value.status == -5
+    s.close()
+
+
+def test_failed_run_in_a_batch_leaves_the_others_untouched(tk, solver_caps):
+    """In one server a chi=32 Trivial run (block iteration) fails under
+    max_iters=2 while a chi=8 Z2 run (direct solves only) in the same grouped
+    launches finishes with the same norms as alone; tnr_run_batch returns the
+    failed run's status and names it."""
+    small = tk.Request("BTRG", tk.classical_ising(0.41, symmetry="Z2"), 8, 6)
+    big = tk.Request("BTRG", tk.classical_ising(0.41, symmetry="Trivial"), 32, 6)
+    alone = tk.run_batch([small])[0]
+    solver_caps("max_iters=2")
+    srv = tk.Server(max_slots=4)
+    i_small, i_big = srv.submit(small), srv.submit(big)
+    while srv.pump() > 0:
+        pass
+    res = srv.poll(i_small)
+    assert res is not None and np.array_equal(res[0], alone)
+    with pytest.raises(tk.TNRError) as e:
+        srv.poll(i_big)
+    assert e.value.status == -5
+    srv.close()
+    with pytest.raises(tk.TNRError) as e:
+        tk.run_batch([small, big])
+    assert e.value.status == -5 and "run 1" in str(e.value)
+
+
+def test_nonfinite_tensors_are_refused(tk):
+    """NaN / Inf entries: tnr_create and tnr_server_submit return
+    TNR_ERR_NONFINITE; a server keeps serving its other requests."""
+    T = tk.classical_ising(symmetry="Trivial")
+    bad = T.data.copy()
+    bad[3] = np.nan
+    st, _ = _create(tk, tk.BTRG.kind, 1, np.asarray(T.leg_dims), bad)
+    assert st == -4 and "finite" in _last_error(tk)
+    bad[3] = np.inf
+    st, _ = _create(tk, tk.TRG.kind, 1, np.asarray(T.leg_dims), bad)
+    assert st == -4
+    srv = tk.Server(max_slots=2)
+    good = tk.Request("BTRG", T, 8, 4)
+    Tb = tk.trivial(bad.reshape(2, 2, 2, 2, order="F"))
+    with pytest.raises(tk.TNRError) as e:
+        srv.submit(tk.Request("BTRG", Tb, 8, 4))
+    assert e.value.status == -4
+    i = srv.submit(good)
+    while srv.pump() > 0:
+        pass
+    assert np.array_equal(srv.poll(i)[0], tk.run_batch([good])[0])
+    srv.close()

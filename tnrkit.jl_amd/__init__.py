@@ -18,7 +18,7 @@ from .models import (classical_clock, classical_ising, classical_ising_3D,  # no
                      ising_betac, ising_cft_exact, potts_betac)
 from .schemes import (ATRG, BTRG, FP64, HOTRG, HOTRG_3D, SPLIT32, TRG, Finalizer, ImpurityHOTRG,  # noqa: F401
                       ImpurityTRG, MultipleCrit,
-                      convcrit, default_Finalizer, finalize, free_energy, maxiter, run, step,
+                      convcrit, default_Finalizer, finalize, free_energy, maxiter, run, solver_config, step,
                       trivial_convcrit, truncrank)
 from .observables import (GSDegeneracy_Finalizer, central_charge, cft_data,  # noqa: F401
                           finalize_central_charge, finalize_groundstatedegeneracy,

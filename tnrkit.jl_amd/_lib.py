@@ -96,8 +96,9 @@ SIGNATURES = {
     "tnr_kernel_jacobi": (C.c_int, [C.c_int, C.c_int, _DP, _DP, _DP, _IP, _DP]),
     "tnr_profile_enable": (C.c_int, [C.c_int]),
     "tnr_autotune_enable": (C.c_int, [C.c_int]),
+    "tnr_solver_config": (C.c_int, [C.c_char_p]),
     "tnr_device_sync": (C.c_int, []),
-    "tnr_profile_read": (C.c_int, [C.c_int, _I64P, _DP, _DP, _DP]),
+    "tnr_profile_read": (C.c_int, [C.c_int, _I64P, _DP, _DP, _DP, _DP, _DP]),
 }
 
 _lib = None

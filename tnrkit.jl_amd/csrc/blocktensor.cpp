@@ -918,6 +918,11 @@ BT upload(Ctx& ctx, int N, const int32_t* leg_dims, const double* data, size_t n
                 "ndata = " + std::to_string(n) + " does not match the sector dimensions (" +
                     std::to_string(L.size) + ")");
     TNR_REQUIRE(L.size > 0, TNR_ERR_ARG, "empty tensor");
+    // NaN / Inf entries would only surface as non-finite norms later (and,
+    // in a grouped launch, inside other runs' solves): refused here
+    for (size_t i = 0; i < n; ++i)
+        TNR_REQUIRE(std::isfinite(data[i]), TNR_ERR_NONFINITE,
+                    "tensor entry " + std::to_string(i) + " is not finite");
     BT t(ctx, L);
     TNR_HIP(hipMemcpyAsync(t.d(), data, n * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
     ctx.sync();

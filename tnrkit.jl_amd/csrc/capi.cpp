@@ -61,6 +61,24 @@ void check_norms(const double* v, int n) {
             throw Error(TNR_ERR_NONFINITE, "norm " + std::to_string(i) + " is " + std::to_string(v[i]));
 }
 
+// A scheme whose step failed (sticky until its tensor is replaced): the
+// reference's step! raises there, e.g. from a LAPACK-backed svd_trunc.
+void check_status(const Scheme& s) {
+    if (s.status == TNR_ERR_NOCONV)
+        throw Error(s.status, "a truncated SVD or eigensolve did not converge; the tensor is "
+                              "undefined until tnr_set_tensor replaces it");
+    if (s.status != TNR_OK)
+        throw Error(s.status, "a truncated SVD or eigensolve met non-finite values; the tensor "
+                              "is undefined until tnr_set_tensor replaces it");
+}
+
+// one step! of a single scheme, failing like the reference's step!
+void step_one(Ctx& ctx, Scheme& s, int chi) {
+    check_status(s);
+    step_many(ctx, {&s}, {chi});
+    check_status(s);
+}
+
 double now_s(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -227,6 +245,7 @@ int tnr_ncomponents(int kind) {
 int tnr_finalize_multi(tnr_scheme* s, double* out) {
     return guarded([&] {
         if (!s || !out) return fail(TNR_ERR_ARG, "null pointer");
+        check_status(s->s);
         finalize_host(*s->ctx, s->s, out);
         check_norms(out, 1);
         return TNR_OK;
@@ -319,7 +338,7 @@ int tnr_step(tnr_scheme* s, int chi) {
     return guarded([&] {
         if (!s) return fail(TNR_ERR_ARG, "null handle");
         if (chi < 1) return fail(TNR_ERR_ARG, "chi must be >= 1");
-        step_many(*s->ctx, {&s->s}, {chi});
+        step_one(*s->ctx, s->s, chi);
         s->ctx->sync();
         return TNR_OK;
     });
@@ -330,6 +349,7 @@ int tnr_finalize(tnr_scheme* s, double* norm) {
         if (!s || !norm) return fail(TNR_ERR_ARG, "null pointer");
         if (scheme_components(s->s.kind) != 1)
             return fail(TNR_ERR_STATE, "impurity schemes return several values: tnr_finalize_multi");
+        check_status(s->s);
         double* slot = s->slots(1);
         finalize_many(*s->ctx, {&s->s}, {slot});
         s->ctx->fetch(norm, slot, sizeof(double));
@@ -350,8 +370,9 @@ int tnr_run(tnr_scheme* s, int chi, int maxiter, int finalize_beginning, double*
         if (nc > 1) {   // impurity schemes: each finalize! fetches its values
             int w = 0;
             if (finalize_beginning) finalize_host(ctx, s->s, norms + nc * w++);
+            check_status(s->s);
             for (int i = 0; i < steps; ++i) {
-                step_many(ctx, {&s->s}, {chi});
+                step_one(ctx, s->s, chi);
                 finalize_host(ctx, s->s, norms + nc * w++);
             }
             if (n_out) *n_out = n;
@@ -360,9 +381,10 @@ int tnr_run(tnr_scheme* s, int chi, int maxiter, int finalize_beginning, double*
         }
         double* slot = s->slots(n);
         int w = 0;
+        check_status(s->s);
         if (finalize_beginning) finalize_many(ctx, {&s->s}, {slot + w++});
         for (int i = 0; i < steps; ++i) {
-            step_many(ctx, {&s->s}, {chi});
+            step_one(ctx, s->s, chi);
             finalize_many(ctx, {&s->s}, {slot + w++});
         }
         ctx.fetch(norms, slot, n * sizeof(double));
@@ -601,6 +623,8 @@ struct tnr_server {
         for (auto& sl : L.active) {
             slots.push_back(sl->norms.d() + sl->written++);
             sl->done++;
+            // a failed step ends the run; the others in the group go on
+            if (sl->s.status != TNR_OK) sl->done = sl->nsteps;
         }
         finalize_many(L.ctx, g, slots);
     }
@@ -648,9 +672,10 @@ struct tnr_server {
                 Req* r = sl->r;
                 r->norms.resize(sl->written);
                 L.ctx.fetch(r->norms.data(), sl->norms.d(), sl->written * sizeof(double));
-                r->status = TNR_OK;
-                for (double v : r->norms)
-                    if (!std::isfinite(v) || v == 0.0) r->status = TNR_ERR_NONFINITE;
+                r->status = sl->s.status;
+                if (r->status == TNR_OK)
+                    for (double v : r->norms)
+                        if (!std::isfinite(v) || v == 0.0) r->status = TNR_ERR_NONFINITE;
                 r->t_done = now_s(t0);
                 in_use -= r->est;
                 L.load -= work_estimate(*r);
@@ -762,6 +787,9 @@ int tnr_server_submit(tnr_server* s, const tnr_run_desc* req, int64_t* id) {
         Layout L = abi_layout(req->zn, req->leg_dims);
         if ((int64_t)req->ndata != L.size || L.size == 0)
             return fail(TNR_ERR_ARG, "ndata does not match the sector dimensions");
+        for (size_t i = 0; i < req->ndata; ++i)
+            if (!std::isfinite(req->data[i]))
+                return fail(TNR_ERR_NONFINITE, "the tensor has non-finite entries");
         std::unique_ptr<Req> r(new Req());
         r->id = s->next_id++;
         r->kind = req->kind; r->zn = req->zn; r->chi = req->chi;

@@ -82,6 +82,13 @@ int tnr_autotune_enable(int on) {
     });
 }
 
+int tnr_solver_config(const char* spec) {
+    return kguard([&] {
+        set_solver_cfg(spec);
+        return TNR_OK;
+    });
+}
+
 int tnr_kernel_svd(int m, int n, const double* M, int k, double* U, double* s, double* V,
                    int* iters) {
     return kguard([&] {
@@ -101,6 +108,7 @@ int tnr_kernel_svd(int m, int n, const double* M, int k, double* U, double* s, d
         }
         std::memcpy(s, out[0].s_host.data(), k * 8);
         if (iters) *iters = out[0].iters;
+        TNR_REQUIRE(out[0].status == TNR_OK, out[0].status, "the truncated SVD failed");
         return TNR_OK;
     });
 }
@@ -163,6 +171,7 @@ int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam, int* 
         std::memcpy(U, hu.data(), (size_t)n * k * 8);
         std::memcpy(lam, out[0].lam_host.data(), k * 8);
         if (iters) *iters = out[0].iters;
+        TNR_REQUIRE(out[0].status == TNR_OK, out[0].status, "the truncated eigensolve failed");
         return TNR_OK;
     });
 }
@@ -217,7 +226,8 @@ int tnr_device_sync(void) {
     });
 }
 
-int tnr_profile_read(int kind, int64_t* launches, double* ms, double* flops, double* bytes) {
+int tnr_profile_read(int kind, int64_t* launches, double* ms, double* flops, double* bytes,
+                     double* busy_ms, double* flops_planned) {
     return kguard([&] {
         TNR_REQUIRE(kind >= 0 && kind < PK_COUNT, TNR_ERR_ARG, "unknown kernel class");
         ProfTotals t = prof_read(kind);
@@ -225,6 +235,8 @@ int tnr_profile_read(int kind, int64_t* launches, double* ms, double* flops, dou
         if (ms) *ms = t.ms;
         if (flops) *flops = t.flops;
         if (bytes) *bytes = t.bytes;
+        if (busy_ms) *busy_ms = t.busy_ms;
+        if (flops_planned) *flops_planned = t.flops_planned;
         return TNR_OK;
     });
 }

@@ -99,6 +99,8 @@ struct JacobiDesc {
     int* sweeps;
     int32_t n, ldg, ldw;
     const int* skip;              // optional
+    double* fail = nullptr;       // optional: set to 1 when the sweep cap is
+                                  // reached with a rotation still due
 };
 
 // Column-norm / normalise job (also the Frobenius-norm job).
@@ -262,7 +264,16 @@ struct ProfScope {
             hipLaunchKernelGGL(kernel, grid, block, shm, st, __VA_ARGS__);                  \
         }                                                                                   \
     } while (0)
-struct ProfTotals { int64_t launches = 0; double ms = 0, flops = 0, bytes = 0; };
+// ms: summed launch durations; busy_ms: the union of the launch intervals
+// (launches on different streams overlap); flops: GEMM flops of the problems
+// that ran (device-counted), flops_planned: what the launches were asked to do
+struct ProfTotals {
+    int64_t launches = 0;
+    double ms = 0, busy_ms = 0, flops = 0, flops_planned = 0, bytes = 0;
+};
+// device counter of the GEMM flops that run while profiling is on (FP64 or
+// split), nullptr when it is off
+unsigned long long* prof_flop_counter(bool split32);
 ProfTotals prof_read(int kind);   // synchronises the device
 
 // GEMM k-tile depth shared with the host planner.

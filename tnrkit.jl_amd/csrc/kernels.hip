@@ -11,6 +11,7 @@
 
 #include <hip/hip_ext.h>
 
+#include <map>
 #include <mutex>
 
 #include <algorithm>
@@ -189,6 +190,19 @@ struct TileCoord {
     int m0, n0, kbeg, kend, kz;
 };
 
+// Profiling: the algorithmic flops of one output tile (its rows and columns
+// inside M x N, its K range) added to a device counter, so only problems
+// that actually ran are counted (a finished problem's tiles return before
+// this).  The counts are integers, so the sum does not depend on order.
+template <int BM, int BN>
+__device__ __forceinline__ void count_tile_flops(unsigned long long* ctr, const GemmDesc& g,
+                                                 const TileCoord& tc) {
+    if (!ctr || threadIdx.x != 0) return;
+    const long long rows = min(BM, g.M - tc.m0), cols = min(BN, g.N - tc.n0);
+    const long long kk = tc.kend - tc.kbeg;
+    if (rows > 0 && cols > 0 && kk > 0) atomicAdd(ctr, (unsigned long long)(2 * rows * cols * kk));
+}
+
 template <int BM, int BN>
 __device__ __forceinline__ TileCoord tile_coord(const GemmDesc& g, int64_t local) {
     TileCoord c;
@@ -328,7 +342,7 @@ __device__ __forceinline__ void write_tile(const GemmDesc& g, const TileCoord& t
 // of KB.  Every shape runs the MFMAs over k in the same 4-wide steps.
 template <int BM, int BN, int WM, int WN, int KB>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __restrict__ descs,
-                                                           int nd) {
+                                                           int nd, unsigned long long* flop_ctr) {
     constexpr int NT = 64 * WM * WN;
     constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
     // two LDS stages: the next k-tile is stored while the current one is
@@ -340,6 +354,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
     const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
     if (g.skip && *g.skip) return;
     const TileCoord tc = tile_coord<BM, BN>(g, (int64_t)blockIdx.x - g.tile_start);
+    count_tile_flops<BM, BN>(flop_ctr, g, tc);
     dbl4 acc[MI][NI], tot[MI][NI];
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi)
@@ -415,13 +430,15 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
 // significant products hi*hi, hi*lo, lo*hi are accumulated on FP32 MFMA
 // (v_mfma_f32_16x16x4_f32) in separate accumulators, then flushed into FP64
 // registers, so FP32 rounding only acts within one 16-deep k-tile.
-__global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict__ descs, int nd) {
+__global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict__ descs, int nd,
+                                                      unsigned long long* flop_ctr) {
     __shared__ float Ah[BK][65], Al[BK][65];
     __shared__ float Bh[BK][65], Bl[BK][65];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w & 1, wn = w >> 1;
     const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
     if (g.skip && *g.skip) return;
     const TileCoord tc = tile_coord<64, 64>(g, (int64_t)blockIdx.x - g.tile_start);
+    count_tile_flops<64, 64>(flop_ctr, g, tc);
     dbl4 acc[2][2], tot[2][2];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
@@ -661,9 +678,8 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
         }
     }
     const int wj0 = t / np, wr0 = t - (t / np) * np, wdj = JT / np, wdr = JT - (JT / np) * np;
-    int sweeps = 0;
-    for (int sw = 0; sw < max_sweeps; ++sw) {
-        // convergence scan over all pairs p < q
+    // any off-diagonal pair p < q that still violates the criterion
+    auto violated = [&]() {
         int viol = 0;
         for (int e = t; e < np * np; e += JT) {
             const int q = (int)(((float)e + 0.5f) * inv_np), p = e - q * np;
@@ -673,7 +689,15 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                         (fabs(gpq) > tol * sqrt(fabs(G[p + p * ld] * G[q + q * ld])));
             }
         }
-        if (!__syncthreads_or(viol)) break;
+        return __syncthreads_or(viol);
+    };
+    int sweeps = 0;
+    bool converged = false;
+    for (int sw = 0; sw < max_sweeps; ++sw) {
+        if (!violated()) {
+            converged = true;
+            break;
+        }
         ++sweeps;
         int a = t, b = np - 1 - t;      // tournament positions of pair t, round 0
         for (int rnd = 0; rnd < m; ++rnd) {
@@ -765,6 +789,8 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
             __syncthreads();
         }
     }
+    // sweep cap reached: converged only if the last sweep left nothing to rotate
+    if (!converged && violated() && t == 0 && D.fail) *D.fail = 1.0;
     // order by |lambda| descending (stable by index) and write out
     if (t < n) {
         const double li = G[t + t * ld], ai = fabs(li);
@@ -1528,6 +1554,22 @@ struct ProfState {
     std::vector<Rec> recs;
     std::vector<hipEvent_t> spare;
     ProfTotals tot[PK_COUNT];
+    // start / end of every filed launch (ms since `ref`), per kind: their
+    // union is the kind's device-busy time (launches on different streams
+    // overlap)
+    hipEvent_t ref = nullptr;
+    std::vector<std::pair<float, float>> spans[PK_COUNT];
+    // device counters of the GEMM flops that ran (FP64, split), per device
+    std::map<int, unsigned long long*> flop_ctr;
+    unsigned long long* counters(int dev) {   // with mu held
+        auto it = flop_ctr.find(dev);
+        if (it != flop_ctr.end()) return it->second;
+        unsigned long long* p = nullptr;
+        TNR_HIP(hipMalloc(&p, 2 * sizeof(unsigned long long)));
+        TNR_HIP(hipMemset(p, 0, 2 * sizeof(unsigned long long)));
+        flop_ctr[dev] = p;
+        return p;
+    }
     hipEvent_t ev() {   // with mu held
         if (!spare.empty()) { hipEvent_t e = spare.back(); spare.pop_back(); return e; }
         hipEvent_t e;
@@ -1535,12 +1577,15 @@ struct ProfState {
         return e;
     }
     void file(const Rec& r) {   // with mu held; r.b has completed
-        float ms = 0;
+        float ms = 0, t0 = 0, t1 = 0;
         TNR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        TNR_HIP(hipEventElapsedTime(&t0, ref, r.a));
+        TNR_HIP(hipEventElapsedTime(&t1, ref, r.b));
         tot[r.kind].launches++;
         tot[r.kind].ms += ms;
         tot[r.kind].flops += r.flops;
         tot[r.kind].bytes += r.bytes;
+        spans[r.kind].emplace_back(t0, t1);
         spare.push_back(r.a);
         spare.push_back(r.b);
     }
@@ -1572,6 +1617,31 @@ void prof_enable(int level) {
     p.drain();
     p.level = level;
     for (auto& t : p.tot) t = ProfTotals();
+    for (auto& v : p.spans) v.clear();
+    if (level) {
+        // the reference point of the launch spans, and zeroed flop counters
+        // (outside any timed region: this synchronises the device)
+        TNR_HIP(hipDeviceSynchronize());
+        if (!p.ref) TNR_HIP(hipEventCreate(&p.ref));
+        TNR_HIP(hipEventRecord(p.ref, 0));
+        TNR_HIP(hipEventSynchronize(p.ref));
+        for (auto& kv : p.flop_ctr) {
+            int dev = 0;
+            TNR_HIP(hipGetDevice(&dev));
+            TNR_HIP(hipSetDevice(kv.first));
+            TNR_HIP(hipMemset(kv.second, 0, 2 * sizeof(unsigned long long)));
+            TNR_HIP(hipSetDevice(dev));
+        }
+    }
+}
+
+unsigned long long* prof_flop_counter(bool split32) {
+    ProfState& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    if (!p.level) return nullptr;
+    int dev = 0;
+    TNR_HIP(hipGetDevice(&dev));
+    return p.counters(dev) + (split32 ? 1 : 0);
 }
 
 bool prof_enabled() { return prof_level() > 0; }
@@ -1631,7 +1701,41 @@ ProfTotals prof_read(int kind) {
     ProfState& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
     p.drain();
-    return (kind >= 0 && kind < PK_COUNT) ? p.tot[kind] : ProfTotals();
+    if (kind < 0 || kind >= PK_COUNT) return ProfTotals();
+    ProfTotals t = p.tot[kind];
+    // union of the launch spans
+    auto v = p.spans[kind];
+    std::sort(v.begin(), v.end());
+    double busy = 0;
+    float a = 0, b = -1;
+    for (auto& s : v) {
+        if (s.first > b) {
+            if (b > a) busy += b - a;
+            a = s.first;
+            b = s.second;
+        } else if (s.second > b) {
+            b = s.second;
+        }
+    }
+    if (b > a) busy += b - a;
+    t.busy_ms = busy;
+    if (kind == PK_GEMM_F64 || kind == PK_GEMM_SPLIT32) {
+        // flops of the problems that ran, from the device counters
+        TNR_HIP(hipDeviceSynchronize());
+        double ran = 0;
+        for (auto& kv : p.flop_ctr) {
+            unsigned long long c[2] = {0, 0};
+            int dev = 0;
+            TNR_HIP(hipGetDevice(&dev));
+            TNR_HIP(hipSetDevice(kv.first));
+            TNR_HIP(hipMemcpy(c, kv.second, sizeof(c), hipMemcpyDeviceToHost));
+            TNR_HIP(hipSetDevice(dev));
+            ran += (double)c[kind == PK_GEMM_SPLIT32 ? 1 : 0];
+        }
+        t.flops_planned = t.flops;
+        t.flops = ran;
+    }
+    return t;
 }
 
 // =================================================================== launch
@@ -1664,20 +1768,21 @@ void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles,
                  double flops, int shape) {
     if (tiles <= 0 || nd <= 0) return;
     ProfScope ps(st, split32 ? PK_GEMM_SPLIT32 : PK_GEMM_F64, flops, 0.0);
+    unsigned long long* fc = ps.active ? prof_flop_counter(split32) : nullptr;
     if (split32)
-        TNR_LAUNCH(ps, k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd);
+        TNR_LAUNCH(ps, k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd, fc);
     else if (shape == GEMM_128x128)
         TNR_LAUNCH(ps, (k_gemm_f64<128, 128, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
-                           st, d_descs, nd);
+                           st, d_descs, nd, fc);
     else if (shape == GEMM_128x64)
         TNR_LAUNCH(ps, (k_gemm_f64<128, 64, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
-                           st, d_descs, nd);
+                           st, d_descs, nd, fc);
     else if (shape == GEMM_64x64_K32)
         TNR_LAUNCH(ps, (k_gemm_f64<64, 64, 2, 2, 32>), dim3((unsigned)tiles), dim3(256), 0,
-                           st, d_descs, nd);
+                           st, d_descs, nd, fc);
     else
         TNR_LAUNCH(ps, (k_gemm_f64<64, 64, 2, 2, 16>), dim3((unsigned)tiles), dim3(256), 0,
-                           st, d_descs, nd);
+                           st, d_descs, nd, fc);
     TNR_HIP(hipGetLastError());
 }
 

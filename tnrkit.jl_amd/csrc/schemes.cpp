@@ -73,6 +73,7 @@ struct Factor {
     double kept_min = 0, dropped_max = 0;
     int iters = 0;
     double conv = 0;
+    int status = 0;          // worst block status of the solver (TNR_OK if none failed)
 };
 
 // weights (wu on U's new leg, wv on V's new leg) of tensor t from its kept
@@ -156,6 +157,7 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
             f.dim[owner[p].second] = kc[p];
             f.iters = std::max(f.iters, out[p].iters);
             f.conv = std::max(f.conv, out[p].conv);
+            if (out[p].status != TNR_OK) f.status = out[p].status;
         }
         int off[MAXN], o = 0;
         for (int c = 0; c < N; ++c) { off[c] = o; o += f.dim[c]; }
@@ -250,6 +252,7 @@ struct EFactor {
     double kept_min = 0, dropped_max = 0;
     int iters = 0;
     double conv = 0;
+    int status = 0;          // worst block status of the solver (TNR_OK if none failed)
 };
 
 // eigh_trunc!(project_hermitian!(MM); trunc = truncrank(chi)): the chi
@@ -300,6 +303,7 @@ std::vector<EFactor> eigh_trunc_many(Ctx& ctx, const std::vector<BT*>& Ms,
         for (size_t p = p0; p < p1; ++p) {
             F[t].iters = std::max(F[t].iters, out[p].iters);
             F[t].conv = std::max(F[t].conv, out[p].conv);
+            if (out[p].status != TNR_OK) F[t].status = out[p].status;
             if (out[p].direct)
                 for (int i = kc[p]; i < in[p].n; ++i) eps2[t] += out[p].lam_host[i] * out[p].lam_host[i];
         }
@@ -364,8 +368,12 @@ std::vector<EFactor> eigh_trunc_many(Ctx& ctx, const std::vector<BT*>& Ms,
     return F;
 }
 
-void log_factor(Scheme& s, double kept, double dropped, int iters, double conv) {
+// a factorization whose solver failed (unconverged, non-finite values)
+// marks the scheme failed: the reference's LAPACK-backed factorizations
+// raise there
+void log_factor(Scheme& s, double kept, double dropped, int iters, double conv, int status) {
     s.tlog.push_back({kept, dropped, (double)iters, conv});
+    if (status != TNR_OK && s.status == TNR_OK) s.status = status;
 }
 
 // ------------------------------------------------------------------- TRG
@@ -422,8 +430,10 @@ void step_trg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& c
     auto Tn = trg_networks(ctx, nets);
     for (size_t i = 0; i < n; ++i) {
         G[i]->tlog.clear();
-        log_factor(*G[i], F1[i].kept_min, F1[i].dropped_max, F1[i].iters, F1[i].conv);
-        log_factor(*G[i], F2[i].kept_min, F2[i].dropped_max, F2[i].iters, F2[i].conv);
+        log_factor(*G[i], F1[i].kept_min, F1[i].dropped_max, F1[i].iters, F1[i].conv,
+                   F1[i].status);
+        log_factor(*G[i], F2[i].kept_min, F2[i].dropped_max, F2[i].iters, F2[i].conv,
+                   F2[i].status);
         G[i]->T = std::move(Tn[i]);
     }
 }
@@ -488,8 +498,10 @@ void step_btrg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& 
     for (size_t i = 0; i < n; ++i) {
         Scheme& s = *G[i];
         s.tlog.clear();
-        log_factor(s, F1[i].kept_min, F1[i].dropped_max, F1[i].iters, F1[i].conv);
-        log_factor(s, F2[i].kept_min, F2[i].dropped_max, F2[i].iters, F2[i].conv);
+        log_factor(s, F1[i].kept_min, F1[i].dropped_max, F1[i].iters, F1[i].conv,
+                   F1[i].status);
+        log_factor(s, F2[i].kept_min, F2[i].dropped_max, F2[i].iters, F2[i].conv,
+                   F2[i].status);
         s.T = std::move(Tn[i]);
     }
     std::vector<int> Ns;
@@ -557,8 +569,10 @@ std::vector<BT> hotrg_proj(Ctx& ctx, const std::vector<Scheme*>& G, const std::v
     for (size_t i = 0; i < n; ++i) {
         EFactor& a = F[2 * i];
         EFactor& b = F[2 * i + 1];
-        log_factor(*G[i], a.kept_min, a.dropped_max, a.iters, a.conv);
-        log_factor(*G[i], b.kept_min, b.dropped_max, b.iters, b.conv);
+        log_factor(*G[i], a.kept_min, a.dropped_max, a.iters, a.conv,
+                   a.status);
+        log_factor(*G[i], b.kept_min, b.dropped_max, b.iters, b.conv,
+                   b.status);
         U.push_back(a.eps > b.eps ? std::move(b.U) : std::move(a.U));
     }
     return U;
@@ -644,7 +658,8 @@ void step_impurity_trg(Ctx& ctx, const std::vector<Scheme*>& G, const std::vecto
         G[i]->tlog.clear();
         for (int f = 0; f < 6; ++f) {
             const Factor& x = F[6 * i + f];
-            log_factor(*G[i], x.kept_min, x.dropped_max, x.iters, x.conv);
+            log_factor(*G[i], x.kept_min, x.dropped_max, x.iters, x.conv,
+                   x.status);
         }
         G[i]->T = std::move(out[5 * i]);
         for (int k = 0; k < 4; ++k) G[i]->imp[k] = std::move(out[5 * i + 1 + k]);
@@ -723,7 +738,8 @@ void atrg_half(Ctx& ctx, const std::vector<Scheme*>& G, const std::vector<int>& 
     auto Tn = contract_many(ctx, cj);
     for (size_t i = 0; i < n; ++i) {
         for (const Factor* f : {&F1[i], &F2[i], &F3[i]})
-            log_factor(*G[i], f->kept_min, f->dropped_max, f->iters, f->conv);
+            log_factor(*G[i], f->kept_min, f->dropped_max, f->iters, f->conv,
+                   f->status);
         G[i]->T = std::move(Tn[i]);
     }
 }
@@ -790,8 +806,10 @@ std::vector<BT> hotrg3d_proj(Ctx& ctx, const std::vector<const BT*>& As, const s
     for (size_t i = 0; i < n; ++i) {
         EFactor& a = F[2 * i];
         EFactor& b = F[2 * i + 1];
-        log_factor(*G[i], a.kept_min, a.dropped_max, a.iters, a.conv);
-        log_factor(*G[i], b.kept_min, b.dropped_max, b.iters, b.conv);
+        log_factor(*G[i], a.kept_min, a.dropped_max, a.iters, a.conv,
+                   a.status);
+        log_factor(*G[i], b.kept_min, b.dropped_max, b.iters, b.conv,
+                   b.status);
         U.push_back(a.eps > b.eps ? std::move(b.U) : std::move(a.U));
     }
     return U;
@@ -935,6 +953,7 @@ int gauge_clusters(const double* sv, int k, std::vector<int>& cl, int& knn) {
 
 // ------------------------------------------------------------------ public
 void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T) {
+    s.status = TNR_OK;   // a new tensor clears a failed step
     if (s.kind == TNR_HOTRG_3D) {
         const int N = T.L.N;
         TNR_REQUIRE(T.L.nleg == 6 && T.L.ncod == 2, TNR_ERR_ARG, "3D tensors have 2+4 legs");

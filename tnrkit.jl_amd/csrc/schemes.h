@@ -21,6 +21,10 @@ struct Scheme {
     BondVec S1, S2;            // BTRG bond weights (vertical, horizontal)
     std::vector<BT> imp;       // ImpurityTRG: T_imp1..4; ImpurityHOTRG: T_imp_order1_1, _1_2, _2
     int steps = 0;
+    // TNR_OK, or the status of a failed step (TNR_ERR_NOCONV: a truncated
+    // SVD / eigensolve stopped unconverged; TNR_ERR_NONFINITE: it met NaN or
+    // infinite values); sticky until the tensor is replaced
+    int status = TNR_OK;
     // per factorization of the last step: smallest kept value, largest
     // dropped value, solver iterations, convergence bound
     std::vector<std::array<double, 4>> tlog;

@@ -14,12 +14,12 @@
 
 namespace tnr {
 
-// TNR_SOLVER_CFG="key=value,..." overrides the defaults (keys: tol_iter,
-// sweeps_iter, block_mult, block_extra, conv_target, max_iters)
-static SolverCfg make_cfg() {
-    SolverCfg c;
-    const char* e = std::getenv("TNR_SOLVER_CFG");
-    if (!e) return c;
+// "key=value,..." overrides of the defaults (keys: tol_iter, sweeps_iter,
+// sweeps, block_mult, block_extra, conv_target, max_iters, cholqr_y,
+// cholqr_z); unknown keys are an argument error.  TNR_SOLVER_CFG in the
+// environment is applied first, then tnr_solver_config's string.
+static void apply_cfg(SolverCfg& c, const char* e) {
+    if (!e) return;
     std::string s(e);
     size_t i = 0;
     while (i < s.size()) {
@@ -27,26 +27,48 @@ static SolverCfg make_cfg() {
         if (j == std::string::npos) j = s.size();
         const std::string kv = s.substr(i, j - i);
         const size_t eq = kv.find('=');
+        TNR_REQUIRE(eq != std::string::npos || kv.empty(), TNR_ERR_ARG,
+                    "solver option '" + kv + "' is not key=value");
         if (eq != std::string::npos) {
             const std::string k = kv.substr(0, eq);
             const double v = std::atof(kv.c_str() + eq + 1);
             if (k == "tol_iter") c.jacobi_tol_iter = v;
             else if (k == "sweeps_iter") c.jacobi_sweeps_iter = (int)v;
+            else if (k == "sweeps") c.jacobi_sweeps = (int)v;
             else if (k == "block_mult") c.block_mult = v;
             else if (k == "block_extra") c.block_extra = (int)v;
             else if (k == "conv_target") c.conv_target = v;
             else if (k == "max_iters") c.max_iters = (int)v;
             else if (k == "cholqr_y") c.cholqr_y = v != 0.0;
             else if (k == "cholqr_z") c.cholqr_z = v != 0.0;
+            else TNR_REQUIRE(false, TNR_ERR_ARG, "unknown solver option '" + k + "'");
         }
         i = j + 1;
+    }
+}
+
+static SolverCfg env_cfg() {
+    SolverCfg c;
+    try {
+        apply_cfg(c, std::getenv("TNR_SOLVER_CFG"));
+    } catch (const Error&) {
+        c = SolverCfg();   // a malformed environment leaves the defaults
     }
     return c;
 }
 
 SolverCfg& solver_cfg() {
-    static SolverCfg c = make_cfg();
+    static SolverCfg c = env_cfg();
     return c;
+}
+
+void set_solver_cfg(const char* spec) {
+    SolverCfg c = env_cfg();
+    apply_cfg(c, spec);
+    TNR_REQUIRE(c.max_iters >= 1 && c.jacobi_sweeps >= 0 && c.jacobi_sweeps_iter >= 0 &&
+                    c.block_mult >= 1.0 && c.block_extra >= 0,
+                TNR_ERR_ARG, "solver option out of range");
+    solver_cfg() = c;
 }
 
 namespace {
@@ -68,6 +90,7 @@ struct RR {
     int ldu;
     double* sigma;
     const int* skip = nullptr;
+    double* fail = nullptr;     // the problem's non-convergence flag
 };
 
 // `passes` = 1 inside the block iteration (the next iteration corrects
@@ -112,7 +135,7 @@ RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool chol = fa
                 rj.push_back(w);
             }
             P.jac[pass].push_back(JacobiDesc{j.G, pass == 0 ? j.W : j.W2, j.lam, j.sweeps, j.l, j.l,
-                                             j.l, j.skip});
+                                             j.l, j.skip, j.fail});
         }
         P.gram[pass] = gemm_plan(ctx, gj);
         P.rot[pass] = gemm_plan(ctx, rj);
@@ -181,6 +204,31 @@ void randfill(Ctx& ctx, const std::vector<FillDesc>& fills) {
     launch_randfill(ctx.stream, d, (int)f.size(), start);
 }
 
+// One double per problem, zeroed on the stream: the Jacobi kernels set a
+// problem's entry to 1 when they stop at the sweep cap unconverged.
+Buf fail_flags(Ctx& ctx, size_t P) {
+    Buf f(&ctx.pool, std::max<size_t>(1, P) * sizeof(double));
+    std::vector<double> z(std::max<size_t>(1, P), 0.0);
+    void* a = ctx.stage->put(ctx.stream, z.data(), z.size() * sizeof(double));
+    TNR_HIP(hipMemcpyAsync(f.d(), a, z.size() * sizeof(double), hipMemcpyDeviceToDevice, ctx.stream));
+    return f;
+}
+
+// Status of one problem from its read-back values and Jacobi flag; a
+// non-finite value is zeroed (a NaN would break the global truncation's
+// ordering) and marks the problem TNR_ERR_NONFINITE.
+int problem_status(std::vector<double>& v, double fail_flag, int status) {
+    bool finite = true;
+    for (double& x : v)
+        if (!std::isfinite(x)) {
+            finite = false;
+            x = 0.0;
+        }
+    if (!finite) return TNR_ERR_NONFINITE;
+    if (fail_flag != 0.0) return TNR_ERR_NOCONV;
+    return status;
+}
+
 // block width of the subspace iteration: max(block_mult * k, k + block_extra),
 // at most LMAX
 int block_size(int kmax) {
@@ -232,6 +280,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
     out.clear();
     out.resize(in.size());
     const SolverCfg& cfg = solver_cfg();
+    Buf failv = fail_flags(ctx, in.size());
     std::vector<int> direct, iter;
     for (size_t p = 0; p < in.size(); ++p) {
         const SvdIn& s = in[p];
@@ -270,6 +319,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             gd.push_back(g);
             rr.push_back(RR{x.Y.d(), rows, l, rows, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
                             x.lam.d(), x.sw.as<int>(), x.Wt.d(), x.Y.d(), rows, x.sig.d()});
+            rr.back().fail = failv.d() + direct[i];
         }
         gather_many(ctx, gd);
         rr_grouped(ctx, rr, 2);
@@ -320,26 +370,27 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             x.sw = Buf(&ctx.pool, 64);
             const int* skip = cs.done(i);
             double* sv = sigV.d() + (size_t)i * LMAX;
+            double* fl = failv.d() + iter[i];
             fills.push_back(FillDesc{x.V.d(), s.n, l, s.n, s.seed, 0});
             GemmJob mv{s.M, x.V.d(), x.Y.d(), s.m, l, s.n, s.ld, s.n, s.m, 0, 0};   // Y = M V
             mv.skip = skip;
             jMV.push_back(mv);
             rY.push_back(RR{x.Y.d(), s.m, l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
-                            x.sw.as<int>(), nullptr, x.U.d(), s.m, x.sigU.d(), skip});
+                            x.sw.as<int>(), nullptr, x.U.d(), s.m, x.sigU.d(), skip, fl});
             GemmJob mtu{s.M, x.U.d(), x.Z.d(), s.n, l, s.m, s.ld, s.m, s.n, 1, 0};  // Z = M^T U
             mtu.skip = skip;
             jMtU.push_back(mtu);
             rZ.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
-                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip});
+                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip, fl});
             // Ritz values of the Z side, or with Cholesky QR the pivots R_jj
             conv.push_back(ConvDesc{cfg.cholqr_z ? x.lam.d() : sv, l, s.kmax, 1, cs.it(i),
                                     cs.prod(i), cs.done(i)});
             rV.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
-                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
+                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr, fl});
             // Ritz extraction from Y = M V (two passes), V <- V Wtot
             jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, l, s.n, s.ld, s.n, s.m, 0, 0});
             rF.push_back(RR{x.Y.d(), s.m, l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
-                            x.sw.as<int>(), x.Wt.d(), x.U.d(), s.m, x.sigU.d()});
+                            x.sw.as<int>(), x.Wt.d(), x.U.d(), s.m, x.sigU.d(), nullptr, fl});
             jV.push_back(GemmJob{x.V.d(), x.Wt.d(), x.Z.d(), s.n, l, l, s.n, l, s.n, 0, 0});
         }
         randfill(ctx, fills);
@@ -389,18 +440,26 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             o.s = std::move(x.sigU);
             o.iters = its[i];
             o.conv = prods[i];
+            if (!(prods[i] < cfg.conv_target)) o.status = TNR_ERR_NOCONV;   // stopped at max_iters
         }
     }
-    // singular values to the host: packed on the device, one copy
-    std::vector<const double*> src(in.size());
-    std::vector<int> cnt(in.size());
+    // singular values and the non-convergence flags to the host: packed on
+    // the device, one copy
+    std::vector<const double*> src(in.size() + 1);
+    std::vector<int> cnt(in.size() + 1);
+    std::vector<std::vector<double>*> dst(in.size() + 1);
     for (size_t p = 0; p < in.size(); ++p) {
         src[p] = out[p].s.d();
         cnt[p] = in[p].kmax;
+        dst[p] = &out[p].s_host;
     }
-    std::vector<std::vector<double>*> dst(in.size());
-    for (size_t p = 0; p < in.size(); ++p) dst[p] = &out[p].s_host;
+    std::vector<double> failh;
+    src[in.size()] = failv.d();
+    cnt[in.size()] = (int)in.size();
+    dst[in.size()] = &failh;
     readback_vectors(ctx, src, cnt, dst);
+    for (size_t p = 0; p < in.size(); ++p)
+        out[p].status = problem_status(out[p].s_host, failh[p], out[p].status);
 }
 
 // ---------------------------------------------------------------- eigh
@@ -408,6 +467,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
     out.clear();
     out.resize(in.size());
     const SolverCfg& cfg = solver_cfg();
+    Buf failv = fail_flags(ctx, in.size());
     std::vector<int> direct, iter;
     for (size_t p = 0; p < in.size(); ++p) {
         const EighIn& s = in[p];
@@ -431,7 +491,8 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             o.ncols = s.n;
             o.direct = true;
             sw[i] = Buf(&ctx.pool, 64);
-            jd.push_back(JacobiDesc{s.M, o.U.d(), o.lam.d(), sw[i].as<int>(), s.n, s.ld, s.n});
+            jd.push_back(JacobiDesc{s.M, o.U.d(), o.lam.d(), sw[i].as<int>(), s.n, s.ld, s.n, nullptr,
+                                    failv.d() + direct[i]});
             nmax = std::max(nmax, s.n);
         }
         JacobiDesc* djd = ctx.stage->put(ctx.stream, jd);
@@ -470,10 +531,10 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             mv.skip = skip;
             jMV.push_back(mv);
             rY.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
-                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip});
+                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip, failv.d() + iter[i]});
             conv.push_back(ConvDesc{sv, l, s.kmax, 0, cs.it(i), cs.prod(i), cs.done(i)});
             rV.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
-                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv});
+                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr, failv.d() + iter[i]});
             jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, l, s.n, s.ld, s.n, s.n, 0, 0});
             jH.push_back(GemmJob{x.V.d(), x.Y.d(), x.G.d(), l, l, s.n, s.n, s.n, l, 1, 0});
             nmax = std::max(nmax, s.n);
@@ -510,7 +571,8 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             IW& x = w[i];
             EighOut& o = out[iter[i]];
             o.lam = Buf(&ctx.pool, (size_t)x.l * 8);
-            jd.push_back(JacobiDesc{x.G.d(), x.W.d(), o.lam.d(), x.sw.as<int>(), x.l, x.l, x.l});
+            jd.push_back(JacobiDesc{x.G.d(), x.W.d(), o.lam.d(), x.sw.as<int>(), x.l, x.l, x.l, nullptr,
+                                    failv.d() + iter[i]});
             lmax = std::max(lmax, x.l);
         }
         JacobiDesc* djd = ctx.stage->put(ctx.stream, jd);
@@ -533,17 +595,24 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         for (int i = 0; i < P; ++i) {
             out[iter[i]].iters = its[i];
             out[iter[i]].conv = prods[i];
+            if (!(prods[i] < cfg.conv_target)) out[iter[i]].status = TNR_ERR_NOCONV;   // at max_iters
         }
     }
-    std::vector<const double*> lsrc(in.size());
-    std::vector<int> lcnt(in.size());
-    std::vector<std::vector<double>*> ldst(in.size());
+    std::vector<const double*> lsrc(in.size() + 1);
+    std::vector<int> lcnt(in.size() + 1);
+    std::vector<std::vector<double>*> ldst(in.size() + 1);
     for (size_t p = 0; p < in.size(); ++p) {
         lsrc[p] = out[p].lam.d();
         lcnt[p] = out[p].ncols;
         ldst[p] = &out[p].lam_host;
     }
+    std::vector<double> failh;
+    lsrc[in.size()] = failv.d();
+    lcnt[in.size()] = (int)in.size();
+    ldst[in.size()] = &failh;
     readback_vectors(ctx, lsrc, lcnt, ldst);
+    for (size_t p = 0; p < in.size(); ++p)
+        out[p].status = problem_status(out[p].lam_host, failh[p], out[p].status);
 }
 
 }  // namespace tnr

@@ -29,6 +29,7 @@ struct SvdOut {
     std::vector<double> s_host;
     int iters = 0;       // subspace iterations (0 = direct)
     double conv = 0.0;   // accumulated contraction bound of the top-k subspace
+    int status = 0;      // TNR_OK, TNR_ERR_NOCONV or TNR_ERR_NONFINITE (see below)
 };
 
 void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& out);
@@ -47,10 +48,20 @@ struct EighOut {
     bool direct = false; // all n eigenvalues available
     int iters = 0;
     double conv = 0.0;
+    int status = 0;      // TNR_OK, TNR_ERR_NOCONV or TNR_ERR_NONFINITE (see below)
 };
 
 void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>& out);
 
+// Per-problem status: TNR_ERR_NOCONV for a problem whose block iteration
+// reaches max_iters with the accumulated contraction bound still above
+// conv_target, or any of whose Jacobi solves reaches its sweep cap with a
+// rotation still due (the reference's LAPACK-backed svd_trunc and
+// eigh_trunc! raise there); TNR_ERR_NONFINITE for a problem with a NaN or
+// infinite singular value / eigenvalue, whose host values are then zeroed so
+// the global truncation stays well defined.  Other problems of the group are
+// unaffected.
+//
 // Solver knobs (the autotuner may override them).
 struct SolverCfg {
     double jacobi_tol = 1e-15;   // relative off-diagonal threshold (final Ritz extraction, direct)
@@ -72,5 +83,8 @@ struct SolverCfg {
     int max_iters = 80;
 };
 SolverCfg& solver_cfg();
+// Defaults, then TNR_SOLVER_CFG, then `spec` ("key=value,...", see
+// solver.cpp); not while other calls are running.
+void set_solver_cfg(const char* spec);
 
 }  // namespace tnr

@@ -351,6 +351,15 @@ def run(scheme: TNRScheme, trscheme: truncrank, criterion: stopcrit,
     return data
 
 
+def solver_config(spec: str = ""):
+    """Limits and thresholds of the device's truncated SVD / eigensolver
+    ("key=value,..."; "" restores the defaults and TNR_SOLVER_CFG).  A solve
+    that reaches ``max_iters`` or a Jacobi sweep cap (``sweeps``,
+    ``sweeps_iter``) unconverged makes its step raise TNRError with status
+    TNR_ERR_NOCONV, as the reference's LAPACK-backed factorizations raise."""
+    check(lib().tnr_solver_config(spec.encode()))
+
+
 def free_energy(data, beta, scalefactor=2.0, initial_size=1.0):
     """src/utility/free_energy.jl:13-20"""
     d = np.ascontiguousarray(np.asarray(data, dtype=np.float64))
