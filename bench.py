@@ -580,6 +580,29 @@ def workload_3d(tk, maxiter, cpu_seconds):
     return {"workload": "HOTRG_3D: Z2 3D Ising beta_c", "cases": cases}
 
 
+def workload_copy():
+    """Calibration of the HBM-traffic counters: three device-to-device
+    copies of 256 MiB (the HIP runtime's copy kernel), so rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE of a known byte count give the counters' scale."""
+    h = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    nbytes = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if h.hipMalloc(C.byref(a), nbytes) or h.hipMalloc(C.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        for _ in range(3):
+            if h.hipMemcpy(b, a, nbytes, 3):          # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        h.hipDeviceSynchronize()
+    finally:
+        h.hipFree(a)
+        h.hipFree(b)
+    return {"workload": "copy calibration: 3 device copies of 256 MiB", "bytes": nbytes}
+
+
 def workload_c4(tk, n, maxiter, check):
     """512 BTRG runs, beta ~ U[0.35, 0.55], chi ~ U{16, 24, 32, 48}."""
     rng = np.random.default_rng(SEED + 4)
@@ -676,7 +699,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrivals per second")
     ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs", "atrg", "3d",
-                                           "plain"],
+                                           "plain", "copy"],
                     default="scan",
                     help="scan = the headline beta scan; c2-c4, imp, obs, atrg, 3d and plain print a "
                          "report line of their own")
@@ -685,6 +708,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent runs would shard by request across GPUs)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "copy":
+        print(json.dumps(workload_copy()))
+        return
     if a.workload != "scan":
         import __graft_entry__
         tk = __graft_entry__.package()

@@ -54,7 +54,9 @@ extern "C" {
 
 #define TNR_PREC_FP64     0  /* all contractions FP64 MFMA */
 #define TNR_PREC_SPLIT32  1  /* contractions on FP32 MFMA, hi/lo split
-                                products; truncation SVD/eigh, log-norms
+                                products, from a run's 9th step on (the
+                                first 8, which dominate the free energy,
+                                stay FP64); truncation SVD/eigh, log-norms
                                 and finalize stay FP64 */
 
 typedef struct tnr_scheme tnr_scheme;

@@ -22,7 +22,10 @@ python tests/golden/make_golden.py).
   per factorization the matrix size, the largest value, the smallest kept and
   the largest dropped value and the 2-norm of the dropped ones
   (python tests/golden/make_golden.py configs [name ...] writes only this
-  file; names select cases and merge into the existing file).
+  file; names select cases and merge into the existing file), and the
+  oracle's own roundoff sensitivity per norm ("sens": the same run with a
+  seeded 1e-15 perturbation of every factorized matrix;
+  python tests/golden/make_golden.py configs-sens [name ...]).
 """
 from __future__ import annotations
 
@@ -210,10 +213,68 @@ def configs(names=()):
         print(name, f"{time.time() - t0:.1f} s", flush=True)
 
 
+def _perturbed_run(kind, spec, chi, n, seed=7, rel=1e-15):
+    """The same oracle run with every matrix the truncated factorizations
+    see perturbed by seeded noise of `rel` times its largest entry (the size
+    of the roundoff two correct implementations differ by)."""
+    import numpy as np
+    from oracle import ztensor as zt
+    rng = np.random.default_rng(seed)
+    svd0, eigh0 = osch.svd_trunc, osch.eigh_trunc
+
+    def noisy(t):
+        d = t.data + rng.standard_normal(t.data.shape) * rel * float(np.abs(t.data).max())
+        # keep the symmetry: entries that violate charge conservation stay 0
+        tot = np.zeros(t.data.shape, dtype=np.int64)
+        for leg, x in enumerate(t.q):
+            shp = [1] * t.data.ndim
+            shp[leg] = len(x)
+            tot = tot + x.reshape(shp)
+        d[(tot % t.N) != 0] = 0.0
+        return zt.ZT(d, t.q, t.N)
+
+    def svd_p(t, ncod, c, gauge="lapack"):
+        return svd0(noisy(t), ncod, c, gauge)
+
+    def eigh_p(t, ncod, c):
+        return eigh0(zt.project_hermitian(noisy(t), ncod), ncod, c)
+
+    osch.svd_trunc, osch.eigh_trunc = svd_p, eigh_p
+    try:
+        sc = {"TRG": osch.TRG, "BTRG": osch.BTRG, "HOTRG": osch.HOTRG}[kind](config_model(spec))
+        return osch.run(sc, chi, n)
+    finally:
+        osch.svd_trunc, osch.eigh_trunc = svd0, eigh0
+
+
+def configs_sensitivity(names=()):
+    """Adds to each case of oracle_configs.json "sens": per norm, the
+    relative change under a 1e-15 perturbation of every factorized matrix
+    (_perturbed_run).  Steps where it exceeds ~1e-12 are not determined to
+    1e-10 by the tensor: the critical RG flow amplifies roundoff there, or
+    a truncation splits a near-degenerate multiplet."""
+    import time
+    path = os.path.join(HERE, "oracle_configs.json")
+    for name, kind, spec, chi, n in config_cases():
+        if names and name not in names:
+            continue
+        with open(path) as f:
+            cur = json.load(f).get(name)
+        if cur is None:
+            continue
+        t0 = time.time()
+        d = _perturbed_run(kind, spec, chi, n)
+        cur["sens"] = [abs(x / y - 1) for x, y in zip(d, cur["norms"])]
+        _merge_into(path, name, cur)
+        print(name, "sens", f"{max(cur['sens']):.2e}", f"{time.time() - t0:.1f} s", flush=True)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["extensions"]:
         extensions()
     elif sys.argv[1:2] == ["configs"]:
         configs(tuple(sys.argv[2:]))
+    elif sys.argv[1:2] == ["configs-sens"]:
+        configs_sensitivity(tuple(sys.argv[2:]))
     else:
         main()

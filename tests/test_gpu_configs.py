@@ -66,13 +66,29 @@ def cut_determined(n, top, kept, dropped, err):
     return (kept - dropped) / kept >= GAP_REL
 
 
+# The oracle's own roundoff sensitivity ("sens" in the fixtures: the same run
+# with every factorized matrix perturbed by 1e-15 of its largest entry): a
+# step counts as determined only while the norms move by at most SENS_MAX
+# under that perturbation.  At beta_c the RG flow is unstable (HOTRG doubles
+# a perturbation per step), and past the determined steps the norms are
+# compared within AMP times the oracle's sensitivity instead.
+SENS_MAX = 1e-12
+AMP = 1e3
+
+
 def determined_steps(case):
     """Number of leading steps whose truncations (and, for HOTRG, projector
-    choices) are all determined by the tensor."""
+    choices) are all determined by the tensor and whose norms the oracle's
+    roundoff sensitivity leaves within SENS_MAX."""
     cuts = case["cuts"]
     per = 4 if case["scheme"] == "HOTRG" else 2
     steps = len(cuts) // per
+    sens = case.get("sens")
+    if sens is not None and sens[0] > SENS_MAX:
+        return 0
     for s in range(steps):
+        if sens is not None and sens[s + 1] > SENS_MAX:
+            return s
         c = cuts[s * per:(s + 1) * per]
         if not all(cut_determined(*x) for x in c):
             return s
@@ -115,7 +131,16 @@ def _compare(tk, case, d, label):
     df = abs(f / case["f"] - 1)
     print(f"{label}: determined steps {k}/{len(ref) - 1}, max rel norm err {head:.2e} "
           f"(all steps {float(np.max(err)):.2e}), |df/f| {df:.2e}")
-    assert head < TOL, (label, k, head)
+    assert head < TOL, label + f" (determined steps {k}): step, rel err: " + "; ".join(
+        f"{i} {err[i]:.2e}" for i in range(len(err)))
+    sens = case.get("sens")
+    if sens is not None:
+        # past the determined steps: within AMP times the oracle's own
+        # roundoff sensitivity
+        bound = np.maximum(TOL, AMP * np.asarray(sens))
+        bad = np.nonzero(err > bound)[0]
+        assert bad.size == 0, label + ": step, rel err, oracle sensitivity: " + "; ".join(
+            f"{i} {err[i]:.2e} {sens[i]:.2e}" for i in range(len(err)))
     return k, df
 
 
@@ -140,8 +165,9 @@ def test_config_size_parity(tk, name):
 
 C4 = sorted(n for n in _fixtures() if n.startswith("c4_"))
 # Split precision (TNR_PREC_SPLIT32): FP32-MFMA hi/lo products accumulated
-# in FP32 per k-tile; its free energies against the oracle's, per run.
-SPLIT_DF = 1e-7
+# in FP32 per k-tile from a run's 9th step on; its free energies against the
+# oracle's, per run, at the 1e-9 the engine promises.
+SPLIT_DF = 1e-9
 
 
 @pytest.mark.skipif(not C4, reason="no 512-run scan fixtures")

@@ -110,16 +110,18 @@ def solver_caps(tk):
 
 
 def test_block_iteration_cap_is_reported(tk, solver_caps):
-    """max_iters=2: the chi=32 SVDs of 256 x 256 and larger matrices (block
-    iteration) stop unconverged; the step fails with TNR_ERR_NOCONV, the
-    handle keeps failing until its tensor is replaced, and with the defaults
-    restored the same handle runs to the oracle's norms."""
+    """max_iters=2: the chi=32 SVDs of 1024 x 1024 matrices (block
+    iteration) stop unconverged once the spectrum has full rank (from step 5
+    on; before, the rank is below 32 and two iterations converge); the step
+    fails with TNR_ERR_NOCONV, the handle keeps failing until its tensor is
+    replaced, and with the defaults restored the same handle runs to the
+    oracle's norms."""
     from oracle import models as om, schemes as osch
     T = tk.classical_ising(0.44, symmetry="Trivial")
     s = tk.BTRG(T)
     solver_caps("max_iters=2")
     with pytest.raises(tk.TNRError) as e:
-        tk.run(s, tk.truncrank(32), tk.maxiter(4), verbosity=0)
+        tk.run(s, tk.truncrank(32), tk.maxiter(8), verbosity=0)
     assert e.value.status == -5 and "converge" in str(e.value)
     solver_caps("")
     with pytest.raises(tk.TNRError) as e:          # sticky until set_tensor

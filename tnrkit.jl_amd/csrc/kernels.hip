@@ -148,20 +148,36 @@ __global__ __launch_bounds__(256) void k_transpose(const TransDesc* __restrict__
         __syncthreads();
         gconst_double* src = as_global(D.src);
         gmut_double* dst = as_global(D.dst);
-        for (int e = t; e < 4096; e += 256) {       // x along n0 (source-contiguous)
+        // all 16 loads of this thread are issued before the first LDS store
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {              // x along n0 (source-contiguous)
+            const int e = t + 256 * u;
             const int s = e >> la, rem = e & ((1 << la) - 1), j = rem >> lx, x = rem & (tx - 1);
             const int a = i0 + x, c = i1 + j;
             const int64_t so = soff[s];
-            if (so >= 0 && a < D.n0 && c < D.n1)
-                buf[s * ty * pitch + j * pitch + x] = src[so + a + (int64_t)c * D.ss1];
+            v[u] = (so >= 0 && a < D.n0 && c < D.n1) ? src[so + a + (int64_t)c * D.ss1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = t + 256 * u;
+            const int s = e >> la, rem = e & ((1 << la) - 1), j = rem >> lx, x = rem & (tx - 1);
+            buf[s * ty * pitch + j * pitch + x] = v[u];
         }
         __syncthreads();
-        for (int e = t; e < 4096; e += 256) {       // x along n1 (destination-contiguous)
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {              // x along n1 (destination-contiguous)
+            const int e = t + 256 * u;
+            const int s = e >> la, rem = e & ((1 << la) - 1), j = rem >> ly, x = rem & (ty - 1);
+            v[u] = buf[s * ty * pitch + x * pitch + j];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = t + 256 * u;
             const int s = e >> la, rem = e & ((1 << la) - 1), j = rem >> ly, x = rem & (ty - 1);
             const int a = i0 + j, c = i1 + x;
             const int64_t dof = doff[s];
-            if (dof >= 0 && a < D.n0 && c < D.n1)
-                dst[dof + c + (int64_t)a * D.ds0] = buf[s * ty * pitch + x * pitch + j];
+            if (dof >= 0 && a < D.n0 && c < D.n1) dst[dof + c + (int64_t)a * D.ds0] = v[u];
         }
     }
 }
@@ -638,6 +654,67 @@ __global__ __launch_bounds__(256) void k_lincomb(const LinDesc* __restrict__ des
 // descending, eigenvectors as columns of W in the same order.
 constexpr int JT = 1024;
 
+// Pair (p < q) that tournament slot `slot` (0 <= slot < np / 2) rotates in
+// round r of a sweep: slot 0 keeps position 0, every other position moves
+// one step round the circle 1..np-1 per round.
+__device__ __forceinline__ void jacobi_pair(int slot, int r, int np, int& p, int& q) {
+    const int m = np - 1;
+    int a = 0;
+    if (slot > 0) {
+        a = slot - 1 + r;
+        if (a >= m) a -= m;
+        a += 1;
+    }
+    int b = np - 2 - slot + r;
+    if (b >= m) b -= m;
+    b += 1;
+    p = min(a, b);
+    q = max(a, b);
+}
+
+// A pair is rotated while g_pq^2 > tol^2 |g_pp g_qq| (the relative
+// criterion keeps the small eigenvalues of graded Gram matrices relatively
+// accurate); the sweep scan and the rounds use this same predicate.
+__device__ __forceinline__ int jacobi_violates(double gpp, double gqq, double gpq, double tol2) {
+    return gpq != 0.0 && gpq * gpq > tol2 * fabs(gpp * gqq);
+}
+
+// Rotation zeroing g_pq: with a = g_qq - g_pp, b = 2 g_pq and
+// h = sqrt(a^2 + b^2), t = sign(a) b / (|a| + h) is the tangent of the
+// smaller angle, c = 1 / sqrt(1 + t^2), s = t c.  The square roots and the
+// reciprocal come from the hardware estimates (v_rsq_f64, v_rcp_f64)
+// refined by two Newton steps each, a short dependent chain instead of
+// IEEE division and square-root sequences; c^2 + s^2 = 1 to a few ulp.
+__device__ __forceinline__ double rsqrt_refined(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-x * y, y, 1.0);       // 1 - x y^2
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
+__device__ __forceinline__ double rcp_refined(double d) {
+    double y = __builtin_amdgcn_rcp(d);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-d, y, 1.0);
+        y = fma(y, e, y);
+    }
+    return y;
+}
+
+__device__ __forceinline__ void jacobi_rotation(double gpp, double gqq, double gpq, double& t,
+                                                double& c, double& s) {
+    const double a = gqq - gpp, b = 2.0 * gpq;
+    const double h2 = fma(a, a, b * b);
+    const double h = h2 * rsqrt_refined(h2);
+    t = (a >= 0.0 ? b : -b) * rcp_refined(fabs(a) + h);
+    c = rsqrt_refined(fma(t, t, 1.0));
+    s = t * c;
+}
+
 __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ descs, int nd,
                                                double tol, int max_sweeps) {
     extern __shared__ double lds[];
@@ -648,10 +725,8 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
     const int ld = np + 1;              // padded column stride (bank spread)
     double* G = lds;                    // np x np, G[i + j*ld]
     double* W = G + np * ld;
-    double2* rcs = (double2*)(W + np * ld);      // (c, s) per pair
-    double* rt = (double*)(rcs + LMAX / 2);      // t per pair
-    int4* pq = (int4*)(rt + LMAX / 2);           // (p, q, active, -) per pair
-    int* pos = (int*)(pq + LMAX / 2);            // output order (LMAX ints)
+    double4* rot = (double4*)(W + np * ld);      // (c, s, t, active) per pair
+    int* pos = (int*)(rot + LMAX / 2);           // output order (LMAX ints)
     const int t = threadIdx.x;
     const float inv_np = 1.0f / (float)np;
     for (int e = t; e < np * np; e += JT) {
@@ -663,6 +738,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
     }
     const int half = np / 2, m = np - 1;
     const int ntri = half * (half + 1) / 2;      // pair blocks (i <= j), <= 2 * JT
+    const double tol2 = tol * tol;
     // this thread's fixed tasks: pair blocks e = t, t + JT of the upper
     // triangle, and (pair, row) entries e = t + u JT of W
     int gi[2] = {-1, -1}, gj[2] = {-1, -1};
@@ -677,17 +753,27 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
             gj[u] = j;
         }
     }
-    const int wj0 = t / np, wr0 = t - (t / np) * np, wdj = JT / np, wdr = JT - (JT / np) * np;
+    constexpr int WT = (LMAX / 2) * LMAX / JT + 1;   // W tasks per thread, max
+    int wslot[WT], wrow[WT];
+    {
+        int j = t / np, r = t - (t / np) * np;
+        const int wdj = JT / np, wdr = JT - (JT / np) * np;
+#pragma unroll
+        for (int u = 0; u < WT; ++u) {
+            wslot[u] = j < half ? j : -1;
+            wrow[u] = r;
+            r += wdr;
+            j += wdj;
+            if (r >= np) { r -= np; ++j; }
+        }
+    }
     // any off-diagonal pair p < q that still violates the criterion
     auto violated = [&]() {
         int viol = 0;
         for (int e = t; e < np * np; e += JT) {
             const int q = (int)(((float)e + 0.5f) * inv_np), p = e - q * np;
-            if (p < q) {
-                const double gpq = G[p + q * ld];
-                viol |= (gpq != 0.0) &&
-                        (fabs(gpq) > tol * sqrt(fabs(G[p + p * ld] * G[q + q * ld])));
-            }
+            if (p < q)
+                viol |= jacobi_violates(G[p + p * ld], G[q + q * ld], G[p + q * ld], tol2);
         }
         return __syncthreads_or(viol);
     };
@@ -699,51 +785,67 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
             break;
         }
         ++sweeps;
-        int a = t, b = np - 1 - t;      // tournament positions of pair t, round 0
         for (int rnd = 0; rnd < m; ++rnd) {
-            int on = 0;
-            if (t < half) {
-                const int p = min(a, b), q = max(a, b);
-                const double gpp = G[p + p * ld], gqq = G[q + q * ld], gpq = G[p + q * ld];
-                on = (gpq != 0.0) && (fabs(gpq) > tol * sqrt(fabs(gpp * gqq)));
-                double c = 1.0, s = 0.0, tt = 0.0;
-                if (on) {
-                    const double tau = (gqq - gpp) / (2.0 * gpq);
-                    tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    c = 1.0 / sqrt(1.0 + tt * tt);
-                    s = tt * c;
-                }
-                pq[t] = make_int4(p, q, on, 0);
-                rcs[t] = make_double2(c, s);
-                rt[t] = tt;
-                if (t > 0) a = (a == m) ? 1 : a + 1;
-                b = (b == m) ? 1 : b + 1;
+            // The pairs of a round are fixed by the tournament schedule, so
+            // this thread's G block and W entries are loaded before the
+            // rotations are known (nothing writes G or W until the barrier
+            // after them).
+            int bp[2] = {0, 0}, bq[2] = {0, 0}, cp[2] = {0, 0}, cq[2] = {0, 0};
+            double b00[2], b01[2], b10[2], b11[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (gi[u] < 0) continue;
+                jacobi_pair(gi[u], rnd, np, bp[u], bq[u]);
+                jacobi_pair(gj[u], rnd, np, cp[u], cq[u]);
+                b00[u] = G[bp[u] + cp[u] * ld];
+                b01[u] = G[bp[u] + cq[u] * ld];
+                b10[u] = G[bq[u] + cp[u] * ld];
+                b11[u] = G[bq[u] + cq[u] * ld];
             }
-            if (!__syncthreads_or(on)) continue;
+            int wa[WT], wb[WT];
+            double wp[WT], wq[WT];
+#pragma unroll
+            for (int u = 0; u < WT; ++u) {
+                if (wslot[u] < 0) continue;
+                int p, q;
+                jacobi_pair(wslot[u], rnd, np, p, q);
+                wa[u] = wrow[u] + p * ld;
+                wb[u] = wrow[u] + q * ld;
+                wp[u] = W[wa[u]];
+                wq[u] = W[wb[u]];
+            }
+            if (t < half) {
+                int p, q;
+                jacobi_pair(t, rnd, np, p, q);
+                const double gpp = G[p + p * ld], gqq = G[q + q * ld], gpq = G[p + q * ld];
+                const int on = jacobi_violates(gpp, gqq, gpq, tol2);
+                double c = 1.0, sn = 0.0, tt = 0.0;
+                if (on) jacobi_rotation(gpp, gqq, gpq, tt, c, sn);
+                rot[t] = make_double4(c, sn, tt, on ? 1.0 : 0.0);
+            }
+            // every task checks its pairs' flags below, so a plain barrier
+            // suffices (a workgroup OR-reduction costs two)
+            __syncthreads();
             // G <- J^T G J on the upper pair blocks; the lower ones are their
             // transposes, so G stays exactly symmetric
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int i = gi[u], j = gj[u];
                 if (i < 0) continue;
-                const int4 Pi = pq[i], Pj = pq[j];
-                if (!Pi.z && !Pj.z) continue;
-                const int pi = Pi.x, qi = Pi.y;
+                const double4 ri = rot[i], rj = rot[j];
+                if (ri.w == 0.0 && rj.w == 0.0) continue;
+                const int pi = bp[u], qi = bq[u], pj = cp[u], qj = cq[u];
                 if (i == j) {
-                    const double tt = rt[i], gpq = G[pi + qi * ld];
-                    G[pi + pi * ld] -= tt * gpq;
-                    G[qi + qi * ld] += tt * gpq;
+                    const double tt = ri.z, gpq = b01[u];
+                    G[pi + pi * ld] = b00[u] - tt * gpq;
+                    G[qi + qi * ld] = b11[u] + tt * gpq;
                     G[pi + qi * ld] = 0.0;
                     G[qi + pi * ld] = 0.0;
                     continue;
                 }
-                const int pj = Pj.x, qj = Pj.y;
-                const double2 ri = rcs[i], rj = rcs[j];
                 const double ci = ri.x, si = ri.y, cj = rj.x, sj = rj.y;
-                const double b00 = G[pi + pj * ld], b01 = G[pi + qj * ld];
-                const double b10 = G[qi + pj * ld], b11 = G[qi + qj * ld];
-                const double x00 = ci * b00 - si * b10, x01 = ci * b01 - si * b11;   // J_i^T B
-                const double x10 = si * b00 + ci * b10, x11 = si * b01 + ci * b11;
+                const double x00 = ci * b00[u] - si * b10[u], x01 = ci * b01[u] - si * b11[u];
+                const double x10 = si * b00[u] + ci * b10[u], x11 = si * b01[u] + ci * b11[u];
                 const double y00 = cj * x00 - sj * x01, y01 = sj * x00 + cj * x01;   // (.) J_j
                 const double y10 = cj * x10 - sj * x11, y11 = sj * x10 + cj * x11;
                 G[pi + pj * ld] = y00; G[pj + pi * ld] = y00;
@@ -751,40 +853,14 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                 G[qi + pj * ld] = y10; G[pj + qi * ld] = y10;
                 G[qi + qj * ld] = y11; G[qj + qi * ld] = y11;
             }
-            // W <- W J: this thread's (pair, row) tasks touch distinct
-            // columns, so all their loads are issued before any store
-            {
-                constexpr int WT = (LMAX / 2) * LMAX / JT + 1;   // tasks per thread, max
-                int wa[WT], wb[WT];
-                double wp[WT], wq[WT], wc[WT], ws[WT];
-                bool wl[WT];
-                int j = wj0, r = wr0;
+            // W <- W J: this thread's (pair, row) tasks touch distinct columns
 #pragma unroll
-                for (int u = 0; u < WT; ++u) {
-                    wl[u] = false;
-                    if (j < half) {
-                        const int4 Pj = pq[j];
-                        if (Pj.z) {
-                            const double2 rj = rcs[j];
-                            wa[u] = r + Pj.x * ld;
-                            wb[u] = r + Pj.y * ld;
-                            wc[u] = rj.x;
-                            ws[u] = rj.y;
-                            wp[u] = W[wa[u]];
-                            wq[u] = W[wb[u]];
-                            wl[u] = true;
-                        }
-                    }
-                    r += wdr;
-                    j += wdj;
-                    if (r >= np) { r -= np; ++j; }
-                }
-#pragma unroll
-                for (int u = 0; u < WT; ++u)
-                    if (wl[u]) {
-                        W[wa[u]] = wc[u] * wp[u] - ws[u] * wq[u];
-                        W[wb[u]] = ws[u] * wp[u] + wc[u] * wq[u];
-                    }
+            for (int u = 0; u < WT; ++u) {
+                if (wslot[u] < 0) continue;
+                const double4 rj = rot[wslot[u]];
+                if (rj.w == 0.0) continue;
+                W[wa[u]] = rj.x * wp[u] - rj.y * wq[u];
+                W[wb[u]] = rj.y * wp[u] + rj.x * wq[u];
             }
             __syncthreads();
         }
@@ -812,7 +888,9 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
 }
 
 // ------------------------------------------------ pivoted Cholesky QR
-// One 256-thread workgroup per n x n Gram matrix G = Y^T Y (n <= LMAX):
+// One 256-thread workgroup per n x n Gram matrix G = Y^T Y (n <= LMAX;
+// measured at 64 x 64: 178 us with 256 threads, 220 us with 128, 333 us with
+// one wavefront, 236 us with 1024):
 // diagonal-pivoted Cholesky G[p, p] = R^T R, stopped where the largest
 // remaining diagonal falls to null_rel2 * max diag(G) (numerical rank r),
 // then X = R^{-1} (r x r).  Output W = P X (W[perm[i], j] = X[i, j], zero
@@ -975,8 +1053,10 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
 // ------------------------------------------------- column norms / normalise
 // sigma_j = ||Y[:, j]||; U[:, j] = Y[:, j] / sigma_j, or 0 when
 // sigma_j <= rtol * max_j sigma_j (numerical null direction).  One
-// workgroup per problem; wave w owns columns j = w (mod 16) and reduces with
-// a fixed butterfly, so the result does not depend on scheduling.
+// workgroup per problem; wave w owns columns j = w (mod 16).  A lane keeps
+// four partial sums over rows 256 apart (four loads in flight), adds them
+// in a fixed order and the wave reduces with a fixed butterfly, so the
+// result does not depend on scheduling.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -991,9 +1071,14 @@ __global__ __launch_bounds__(1024) void k_colnorm(const ColDesc* __restrict__ de
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int j = w; j < D.ncol; j += 16) {
         const double* y = D.Y + (int64_t)j * D.ldy;
-        double acc = 0.0;
-        for (int i = lane; i < D.nrow; i += 64) acc += y[i] * y[i];
-        acc = wave_sum(acc);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int i = lane;
+        for (; i + 192 < D.nrow; i += 256) {
+            const double y0 = y[i], y1 = y[i + 64], y2 = y[i + 128], y3 = y[i + 192];
+            a0 = fma(y0, y0, a0); a1 = fma(y1, y1, a1); a2 = fma(y2, y2, a2); a3 = fma(y3, y3, a3);
+        }
+        for (; i < D.nrow; i += 64) a0 = fma(y[i], y[i], a0);
+        double acc = wave_sum((a0 + a1) + (a2 + a3));
         if (lane == 0) sig[j] = sqrt(acc);
     }
     __syncthreads();
@@ -1005,7 +1090,12 @@ __global__ __launch_bounds__(1024) void k_colnorm(const ColDesc* __restrict__ de
         const double inv = live ? 1.0 / sj : 0.0;
         const double* y = D.Y + (int64_t)j * D.ldy;
         double* u = D.U + (int64_t)j * D.ldu;
-        for (int i = lane; i < D.nrow; i += 64) u[i] = y[i] * inv;
+        int i = lane;
+        for (; i + 192 < D.nrow; i += 256) {
+            const double y0 = y[i], y1 = y[i + 64], y2 = y[i + 128], y3 = y[i + 192];
+            u[i] = y0 * inv; u[i + 64] = y1 * inv; u[i + 128] = y2 * inv; u[i + 192] = y3 * inv;
+        }
+        for (; i < D.nrow; i += 64) u[i] = y[i] * inv;
         if (lane == 0) D.sigma[j] = live ? sj : 0.0;
     }
 }
@@ -1810,8 +1900,8 @@ void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const Trac
 
 size_t jacobi_lds_bytes(int n) {
     const size_t np = n + (n & 1);
-    return 2 * np * (np + 1) * sizeof(double) + (LMAX / 2) * (sizeof(double) * 3) +
-           (LMAX / 2) * sizeof(int) * 4 + LMAX * sizeof(int);
+    return 2 * np * (np + 1) * sizeof(double) + (LMAX / 2) * sizeof(double) * 4 +
+           LMAX * sizeof(int);
 }
 
 void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, double tol,

@@ -1025,25 +1025,47 @@ void scheme_init(Ctx& ctx, Scheme& s, int kind, BT&& T, double k) {
     scheme_set_tensor(ctx, s, std::move(T));
 }
 
+namespace {
+// Contraction precision of a context for the duration of a scope.
+struct PrecisionScope {
+    Ctx& ctx;
+    int saved;
+    PrecisionScope(Ctx& c, int p) : ctx(c), saved(c.precision) { c.precision = p; }
+    ~PrecisionScope() { ctx.precision = saved; }
+};
+
+void step_kind(Ctx& ctx, int kind, const std::vector<Scheme*>& g, const std::vector<int>& c) {
+    if (g.empty()) return;
+    if (kind == TNR_TRG) step_trg(ctx, g, c);
+    else if (kind == TNR_BTRG) step_btrg(ctx, g, c);
+    else if (kind == TNR_HOTRG) step_hotrg(ctx, g, c);
+    else if (kind == TNR_ATRG) step_atrg(ctx, g, c);
+    else if (kind == TNR_HOTRG_3D) step_hotrg3d(ctx, g, c);
+    else if (kind == TNR_IMPURITY_TRG) step_impurity_trg(ctx, g, c);
+    else step_impurity_hotrg(ctx, g, c);
+}
+}  // namespace
+
 void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<int>& chis) {
     for (int kind : {TNR_TRG, TNR_BTRG, TNR_HOTRG, TNR_ATRG, TNR_HOTRG_3D, TNR_IMPURITY_TRG,
                      TNR_IMPURITY_HOTRG}) {
-        std::vector<Scheme*> g;
-        std::vector<int> c;
+        // split precision: a scheme's first kSplitFromStep steps stay FP64
+        std::vector<Scheme*> g[2];
+        std::vector<int> c[2];
         for (size_t i = 0; i < group.size(); ++i)
             if (group[i]->kind == kind) {
-                g.push_back(group[i]);
-                c.push_back(chis[i]);
+                const int late =
+                    ctx.precision == TNR_PREC_SPLIT32 && group[i]->steps >= kSplitFromStep;
+                g[late].push_back(group[i]);
+                c[late].push_back(chis[i]);
             }
-        if (g.empty()) continue;
-        if (kind == TNR_TRG) step_trg(ctx, g, c);
-        else if (kind == TNR_BTRG) step_btrg(ctx, g, c);
-        else if (kind == TNR_HOTRG) step_hotrg(ctx, g, c);
-        else if (kind == TNR_ATRG) step_atrg(ctx, g, c);
-        else if (kind == TNR_HOTRG_3D) step_hotrg3d(ctx, g, c);
-        else if (kind == TNR_IMPURITY_TRG) step_impurity_trg(ctx, g, c);
-        else step_impurity_hotrg(ctx, g, c);
-        for (auto* s : g) s->steps++;
+        if (!g[0].empty()) {
+            PrecisionScope fp64(ctx, TNR_PREC_FP64);
+            step_kind(ctx, kind, g[0], c[0]);
+        }
+        step_kind(ctx, kind, g[1], c[1]);
+        for (auto& gg : g)
+            for (auto* s : gg) s->steps++;
     }
 }
 

@@ -41,6 +41,12 @@ int scheme_components(int kind);   // values one finalize! returns
 void finalize_host(Ctx& ctx, Scheme& s, double* out);
 void scheme_set_tensor(Ctx& ctx, Scheme& s, BT&& T);
 
+// Split precision (TNR_PREC_SPLIT32) starts at this step of a run: the
+// free energy weighs step i's log-norm by scalefactor^-i, so the first steps
+// dominate it; run in FP64 they keep |df/f| within 1e-9 of the FP64 path
+// (the FP32 hi/lo products are good to ~1e-8 per step).
+constexpr int kSplitFromStep = 8;
+
 // step!(scheme, truncrank(chi)) for every scheme of the group
 void step_many(Ctx& ctx, const std::vector<Scheme*>& group, const std::vector<int>& chis);
 

@@ -6,8 +6,8 @@
       --command CMD > profiles/traffic_X.json
 
 traffic: FETCH_SIZE / WRITE_SIZE are KiB per dispatch.  Each is scaled by the
-ratio of the known byte count of a plain device copy (copy_calibration.py:
-256 MiB read, 256 MiB written) to what the same counter reads for it.
+ratio of the known byte count of a plain device copy (bench.py --workload
+copy: 256 MiB read, 256 MiB written) to what the same counter reads for it.
 """
 from __future__ import annotations
 
@@ -41,20 +41,41 @@ def stats(path, command):
     return "\n".join(out)
 
 
+def _union_ms(spans):
+    """Length of the union of [start, end) intervals (ns in, ms out)."""
+    busy, a, b = 0, 0, -1
+    for s, e in sorted(spans):
+        if s > b:
+            if b > a:
+                busy += b - a
+            a, b = s, e
+        elif e > b:
+            b = e
+    if b > a:
+        busy += b - a
+    return busy / 1e6
+
+
 def phases(trace, kernel, spec):
-    """Average duration of `kernel` over consecutive groups of its dispatches
-    in start order (e.g. warmup, timed steps, a last pass), from the trace."""
+    """Per consecutive group of `kernel`'s dispatches in start order (e.g.
+    warmup, timed steps, a last pass), from the trace: the average duration,
+    the summed durations and the union of the dispatch intervals (dispatches
+    on different streams overlap; the union is the time the kernel was on
+    the device)."""
     rows = [r for r in csv.DictReader(open(trace)) if short(r["Kernel_Name"]).split("<")[0] == kernel]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
+    spans = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows]
+    dur = [(e - s) / 1e3 for s, e in spans]
     out = ["", "Per phase (`%s`, dispatches in start order, from the kernel trace):" % kernel, "",
-           "| phase | dispatches | avg us |", "|---|---|---|"]
+           "| phase | dispatches | avg us | summed ms | union ms |", "|---|---|---|---|---|"]
     i = 0
     for item in spec.split(","):
         name, cnt = item.split(":")
         seg = dur[i:i + int(cnt)]
+        un = _union_ms(spans[i:i + int(cnt)])
         i += int(cnt)
-        out.append("| %s | %d | %.1f |" % (name, len(seg), sum(seg) / max(1, len(seg))))
+        out.append("| %s | %d | %.1f | %.1f | %.1f |" % (name, len(seg), sum(seg) / max(1, len(seg)),
+                                                        sum(seg) / 1e3, un))
     if i != len(dur):
         out.append("")
         out.append("(%d dispatches in the trace, %d assigned to phases)" % (len(dur), i))
@@ -91,7 +112,7 @@ def traffic(fetch, write, cal_fetch, cal_write, command):
         kernels[key] = {"dispatches": nf, "fetch_bytes_per_launch": fb,
                         "write_bytes_per_launch": wb, "bytes_per_launch": fb + wb}
     return json.dumps({"command": command, "fetch_scale": sf, "write_scale": sw,
-                       "calibration": "256 MiB device copy (copy_calibration.py)",
+                       "calibration": "256 MiB device copy (bench.py --workload copy)",
                        "kernels": kernels}, indent=1, sort_keys=True)
 
 
