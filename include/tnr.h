@@ -185,20 +185,27 @@ const char* tnr_last_error(void);
  * together: every coarse-graining step of every run in the batch is issued
  * as grouped launches (one grouped GEMM / permute / solver launch per phase
  * covering all runs and all symmetry blocks).  Results equal the one-run
- * path within the documented tolerance. */
+ * path within the documented tolerance.  Every kind runs here: TRG, BTRG,
+ * HOTRG, ATRG, HOTRG_3D (6-leg tensors, as tnr_create_3d) and the impurity
+ * methods (several tensors on one set of spaces, as tnr_create_multi). */
 typedef struct {
-    int kind;                 /* TNR_TRG / TNR_BTRG / TNR_HOTRG / TNR_ATRG */
+    int kind;                 /* any TNR_* scheme kind           */
     int zn;                   /* 1 = Trivial, N = Z_N            */
-    const int32_t* leg_dims;  /* 4*zn                            */
-    const double* data;       /* initial tensor                  */
+    const int32_t* leg_dims;  /* 4*zn (6*zn for TNR_HOTRG_3D)    */
+    const double* data;       /* initial tensor T                */
     size_t ndata;
     double k;                 /* BTRG exponent                   */
     int chi;                  /* truncrank(chi)                  */
     int maxiter;              /* maxiter(n)                      */
     int finalize_beginning;
+    /* impurity kinds: the tensors after T (4 for TNR_IMPURITY_TRG, 3 for
+     * TNR_IMPURITY_HOTRG, in constructor order), each of ndata doubles on
+     * T's spaces; ignored (may be NULL) for the other kinds */
+    const double* const* impurity;
 } tnr_run_desc;
 
-/* norms: concatenation over runs of (maxiter + finalize_beginning) doubles.
+/* norms: concatenation over runs of (maxiter + finalize_beginning) rows of
+ * tnr_ncomponents(kind) doubles.
  * If a run fails (see tnr_server_poll) the call returns its status and
  * tnr_last_error names the run.
  * The batch engine behind this call (its streams, device memory pools and
@@ -225,7 +232,8 @@ int tnr_server_submit(tnr_server* s, const tnr_run_desc* req, int64_t* id);
  * queued requests first.  *active receives the number still in flight. */
 int tnr_server_pump(tnr_server* s, int* active);
 /* Fetch a finished result: TNR_ERR_BUSY if not finished.  norms must hold
- * maxiter + finalize_beginning doubles; t_admit/t_done are seconds since
+ * maxiter + finalize_beginning rows of tnr_ncomponents(kind) doubles and
+ * *n_out receives the rows written; t_admit/t_done are seconds since
  * server creation.  A run whose step failed (TNR_ERR_NOCONV,
  * TNR_ERR_NONFINITE, see tnr_step) ends there and returns that status with
  * the norms written so far; the other resident runs go on unaffected. */

@@ -76,21 +76,25 @@ def test_eigensolver_rejects_blocks_above_its_limit(tk):
     assert tk.lib().tnr_kernel_eig(1, n, _ptr(A, DP), _ptr(wr, DP), _ptr(wi, DP)) == -1
 
 
-def test_chi_beyond_the_solver_limit_is_an_error_and_the_engine_recovers(tk):
-    """A Trivial 10^4 tensor gives 100 x 100 SVD blocks; above 96 rows the
-    block iteration keeps at most 88 values, so chi = 90 is refused (the
-    check precedes any launch for that solve).  The same process then runs
-    the README case to the published free energy."""
+def test_chi_above_the_rayleigh_ritz_width_runs_the_wide_solver(tk):
+    """A Trivial 10^4 tensor gives 100 x 100 SVD blocks; keeping chi = 90 of
+    them needs a Rayleigh-Ritz step wider than 96, which the wide-block
+    solver (block one-sided Jacobi) takes: TRG and HOTRG first steps match
+    the oracle.  The same process then runs the README case to the published
+    free energy."""
+    from oracle import schemes as osch
+    from oracle.ztensor import ZT
     rng = np.random.default_rng(5)
     D = 10
     dense = rng.standard_normal((D, D, D, D))
     dense = dense + dense.transpose(3, 2, 1, 0)
     T = tk.SymTensor.from_dense(dense, [np.zeros(D, dtype=int)] * 4, 1)
-    s = tk.TRG(T)
-    with pytest.raises(tk.TNRError) as e:
-        tk.run(s, tk.truncrank(90), tk.maxiter(1), verbosity=0)
-    assert e.value.status == -1 and "88" in str(e.value)
-    s.close()
+    for name in ("TRG", "HOTRG"):
+        s = getattr(tk, name)(T)
+        d = np.array(tk.run(s, tk.truncrank(90), tk.maxiter(1), verbosity=0))
+        s.close()
+        ref = np.array(osch.run(getattr(osch, name)(ZT(dense)), 90, 1))
+        assert np.max(np.abs(d / ref - 1)) < 1e-10, (name, d, ref)
     # recovery: README.md:87 (BTRG, Z2 Ising at beta_c, chi 16, 25 steps)
     b = tk.BTRG(tk.classical_ising(tk.ising_betac, symmetry="Z2"))
     data = tk.run(b, tk.truncrank(16), tk.maxiter(25), verbosity=0)

@@ -141,9 +141,12 @@ def decaying(rng, m, n, rate):
     return (U * s) @ V.T
 
 
+# k above 88 on blocks above 96 takes the wide-block solver (block
+# one-sided Jacobi as the Rayleigh-Ritz step)
 @pytest.mark.parametrize("m,n,k,rate", [
     (4, 4, 4, 0.5), (16, 16, 16, 0.3), (50, 200, 20, 0.1), (200, 50, 20, 0.1),
-    (256, 256, 16, 0.05), (1024, 1024, 32, 0.02), (300, 500, 48, 0.05)])
+    (256, 256, 16, 0.05), (1024, 1024, 32, 0.02), (300, 500, 48, 0.05),
+    (300, 400, 120, 0.02), (1024, 1024, 160, 0.01)])
 def test_svd_truncated(tk, m, n, k, rate):
     rng = np.random.default_rng(m + n + k)
     M = decaying(rng, m, n, rate)
@@ -158,7 +161,7 @@ def test_svd_truncated(tk, m, n, k, rate):
     assert np.max(np.abs(V.T @ V - np.eye(k))) < 1e-12
 
 
-@pytest.mark.parametrize("n,k", [(64, 16), (512, 32)])
+@pytest.mark.parametrize("n,k", [(64, 16), (512, 32), (400, 120)])
 def test_eigh_truncated(tk, n, k):
     rng = np.random.default_rng(n)
     X = decaying(rng, n, n, 0.04)

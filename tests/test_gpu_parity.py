@@ -325,7 +325,8 @@ def test_reused_engine_and_contexts_repeat_bitwise(tk):
     other sizes and symmetries, then the first batch again gives the first
     norms bit for bit; the same holds for single handles created after
     handles of other sizes were destroyed.  The last check is a batch that
-    fails (chi above the solver's limit) followed by the first batch again."""
+    fails (a solver capped to non-convergence) followed by the first batch
+    again."""
     a = [tk.Request("BTRG", tk.classical_ising(0.41 + 0.02 * i, symmetry="Trivial"), 24, 10)
          for i in range(6)]
     b = [tk.Request("HOTRG", tk.classical_ising(0.44, symmetry="Z2"), 12, 6),
@@ -345,13 +346,50 @@ def test_reused_engine_and_contexts_repeat_bitwise(tk):
     single(tk.classical_potts(3, 0.9), 24, 5)
     assert np.array_equal(single(T1, 16, 8), s1)
 
-    rng = np.random.default_rng(3)
-    D = 10
-    dense = rng.standard_normal((D, D, D, D))
-    big = tk.SymTensor.from_dense(dense + dense.transpose(3, 2, 1, 0),
-                                  [np.zeros(D, dtype=int)] * 4, 1)
-    with pytest.raises(tk.TNRError):
-        tk.run_batch([tk.Request("TRG", big, 90, 1)] + a[:2])
+    # a batch in which a run fails mid-way (its block iteration capped at
+    # two iterations: TNR_ERR_NOCONV), then the first batch again
+    fail = tk.Request("BTRG", tk.classical_ising(0.44, symmetry="Trivial"), 32, 8)
+    tk.solver_config("max_iters=2")
+    try:
+        with pytest.raises(tk.TNRError):
+            tk.run_batch([fail] + a[:2])
+    finally:
+        tk.solver_config("")
     after = tk.run_batch(a)
     for x, y in zip(first, after):
         assert np.array_equal(x, y)
+
+
+def test_batch_runs_every_scheme_kind(tk):
+    """ImpurityTRG, ImpurityHOTRG and HOTRG_3D runs go through the batch
+    engine (tnr_run_batch) and the server beside TRG / BTRG / HOTRG / ATRG
+    runs, each giving the same rows as the run alone (impuritytrg.jl:103,
+    hotrg3d.jl:128-136: the reference runs them through the same run!)."""
+    T, Ti = tk.classical_ising(0.44, "Trivial", 0.1), tk.classical_ising_impurity(0.44, 0.1)
+    T3 = tk.classical_ising_3D()
+    reqs = [tk.Request("ImpurityTRG", [T, Ti, T, T, T], 8, 5),
+            tk.Request("BTRG", tk.classical_ising(0.42, symmetry="Z2"), 16, 6),
+            tk.Request("ImpurityHOTRG", [T, Ti, Ti, T], 8, 4),
+            tk.Request("HOTRG_3D", T3, 6, 3),
+            tk.Request("ATRG", tk.classical_ising(0.44, symmetry="Z2"), 12, 4),
+            tk.Request("HOTRG", tk.classical_ising(0.44, symmetry="Trivial"), 8, 4)]
+    batch = tk.run_batch(reqs)
+    alone = []
+    for cls, args, chi, n in ((tk.ImpurityTRG, (T, Ti, T, T, T), 8, 5),
+                              (tk.BTRG, (reqs[1].T,), 16, 6),
+                              (tk.ImpurityHOTRG, (T, Ti, Ti, T), 8, 4),
+                              (tk.HOTRG_3D, (T3,), 6, 3),
+                              (tk.ATRG, (reqs[4].T,), 12, 4),
+                              (tk.HOTRG, (reqs[5].T,), 8, 4)):
+        sc = cls(*args)
+        alone.append(np.array(tk.run(sc, tk.truncrank(chi), tk.maxiter(n), verbosity=0)))
+        sc.close()
+    for r, b, a in zip(reqs, batch, alone):
+        assert b.shape == a.shape and np.array_equal(b, a), r.kind
+    srv = tk.Server(max_slots=8)
+    ids = [srv.submit(r) for r in reqs]
+    while srv.pump() > 0:
+        pass
+    for i, a in zip(ids, alone):
+        assert np.array_equal(srv.poll(i)[0], a)
+    srv.close()

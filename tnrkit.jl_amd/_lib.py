@@ -37,6 +37,7 @@ class RunDesc(C.Structure):
         ("chi", C.c_int),
         ("maxiter", C.c_int),
         ("finalize_beginning", C.c_int),
+        ("impurity", C.POINTER(C.POINTER(C.c_double))),
     ]
 
 

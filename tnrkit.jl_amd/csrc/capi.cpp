@@ -490,9 +490,11 @@ namespace {
 struct Req {
     int64_t id = 0;
     int kind = 0, zn = 1, chi = 1, maxiter = 1, fb = 1;
+    int nleg = 4, ncomp = 1;           // legs of T, values per finalize!
     double k = -0.5;
     std::vector<int32_t> dims;
     std::vector<double> data;
+    std::vector<std::vector<double>> imp;   // impurity tensors after T
     size_t est = 0;
     double t_submit = 0, t_done = 0;
     std::vector<double> norms;
@@ -502,8 +504,10 @@ struct Req {
 struct Slot {
     Req* r = nullptr;
     Scheme s;
-    Buf norms;
+    Buf norms;                  // single-tensor kinds: one norm per finalize!, on the device
+    std::vector<double> hnorms; // impurity kinds: the finalize! rows, on the host
     int nsteps = 0, done = 0, written = 0;
+    bool host_finalize() const { return r->ncomp > 1; }
 };
 
 }  // namespace
@@ -582,9 +586,20 @@ struct tnr_server {
             sl->nsteps = steps_of(r->maxiter);
             const int n = sl->nsteps + (r->fb ? 1 : 0);
             try {
-                BT T = upload(L.ctx, r->zn, r->dims.data(), r->data.data(), r->data.size());
-                scheme_init(L.ctx, sl->s, r->kind, std::move(T), r->k);
-                sl->norms = Buf(&L.ctx.pool, (size_t)n * sizeof(double));
+                if (r->ncomp > 1) {
+                    std::vector<BT> Ts;
+                    Ts.push_back(upload(L.ctx, r->zn, r->dims.data(), r->data.data(),
+                                        r->data.size()));
+                    for (auto& t : r->imp)
+                        Ts.push_back(upload(L.ctx, r->zn, r->dims.data(), t.data(), t.size()));
+                    scheme_init_multi(L.ctx, sl->s, r->kind, std::move(Ts));
+                    sl->hnorms.reserve((size_t)n * r->ncomp);
+                } else {
+                    BT T = upload(L.ctx, r->zn, r->dims.data(), r->data.data(), r->data.size(),
+                                  r->nleg, 2);
+                    scheme_init(L.ctx, sl->s, r->kind, std::move(T), r->k);
+                    sl->norms = Buf(&L.ctx.pool, (size_t)n * sizeof(double));
+                }
             } catch (const Error& e) {
                 r->status = e.code;
                 r->t_done = now_s(t0);
@@ -598,15 +613,30 @@ struct tnr_server {
         }
         // finalize at the beginning, grouped over the newly admitted of a lane
         for (size_t li = 0; li < lanes.size(); ++li) {
-            std::vector<Scheme*> g;
-            std::vector<double*> slots;
+            std::vector<Slot*> fb;
             for (Slot* sl : fresh[li])
-                if (sl->r->fb) {
-                    g.push_back(&sl->s);
-                    slots.push_back(sl->norms.d() + sl->written++);
-                }
-            if (!g.empty()) finalize_many(lanes[li]->ctx, g, slots);
+                if (sl->r->fb) fb.push_back(sl);
+            finalize_slots(lanes[li]->ctx, fb);
         }
+    }
+
+    // finalize! of the given slots: single-tensor kinds grouped on the
+    // device, impurity kinds one by one with their rows on the host
+    static void finalize_slots(Ctx& ctx, const std::vector<Slot*>& sls) {
+        std::vector<Scheme*> g;
+        std::vector<double*> slots;
+        for (Slot* sl : sls) {
+            if (sl->host_finalize()) {
+                const size_t o = sl->hnorms.size();
+                sl->hnorms.resize(o + sl->r->ncomp);
+                finalize_host(ctx, sl->s, sl->hnorms.data() + o);
+                sl->written++;
+            } else {
+                g.push_back(&sl->s);
+                slots.push_back(sl->norms.d() + sl->written++);
+            }
+        }
+        if (!g.empty()) finalize_many(ctx, g, slots);
     }
 
     // one coarse-graining step of every resident scheme of a lane
@@ -614,19 +644,19 @@ struct tnr_server {
         if (L.active.empty()) return;
         std::vector<Scheme*> g;
         std::vector<int> chis;
-        std::vector<double*> slots;
         for (auto& sl : L.active) {
             g.push_back(&sl->s);
             chis.push_back(sl->r->chi);
         }
         step_many(L.ctx, g, chis);
+        std::vector<Slot*> fin;
         for (auto& sl : L.active) {
-            slots.push_back(sl->norms.d() + sl->written++);
             sl->done++;
             // a failed step ends the run; the others in the group go on
             if (sl->s.status != TNR_OK) sl->done = sl->nsteps;
+            fin.push_back(sl.get());
         }
-        finalize_many(L.ctx, g, slots);
+        finalize_slots(L.ctx, fin);
     }
 
     void step_all() {
@@ -670,12 +700,20 @@ struct tnr_server {
                     continue;
                 }
                 Req* r = sl->r;
-                r->norms.resize(sl->written);
-                L.ctx.fetch(r->norms.data(), sl->norms.d(), sl->written * sizeof(double));
+                if (sl->host_finalize()) {
+                    r->norms = sl->hnorms;
+                } else {
+                    r->norms.resize(sl->written);
+                    L.ctx.fetch(r->norms.data(), sl->norms.d(), sl->written * sizeof(double));
+                }
                 r->status = sl->s.status;
+                // the first value of each row is the norm (impurity rows also
+                // carry impurity traces, which may vanish)
                 if (r->status == TNR_OK)
-                    for (double v : r->norms)
+                    for (size_t i = 0; i < r->norms.size(); i += r->ncomp) {
+                        const double v = r->norms[i];
                         if (!std::isfinite(v) || v == 0.0) r->status = TNR_ERR_NONFINITE;
+                    }
                 r->t_done = now_s(t0);
                 in_use -= r->est;
                 L.load -= work_estimate(*r);
@@ -779,23 +817,34 @@ int tnr_server_create(tnr_server** out, size_t hbm_budget_bytes, int max_slots, 
 int tnr_server_submit(tnr_server* s, const tnr_run_desc* req, int64_t* id) {
     return guarded([&] {
         if (!s || !req || !id) return fail(TNR_ERR_ARG, "null pointer");
-        if (!single_tensor_kind(req->kind) || req->kind == TNR_HOTRG_3D)
-            return fail(TNR_ERR_ARG, "unknown scheme kind (HOTRG_3D runs through tnr_create_3d)");
+        const int ntens = scheme_tensors(req->kind);
+        if (!ntens) return fail(TNR_ERR_ARG, "unknown scheme kind");
         if (req->zn < 1 || req->zn > MAXN) return fail(TNR_ERR_ARG, "zn must be in 1..8");
         if (req->chi < 1) return fail(TNR_ERR_ARG, "chi must be >= 1");
         if (!req->leg_dims || !req->data) return fail(TNR_ERR_ARG, "null tensor");
-        Layout L = abi_layout(req->zn, req->leg_dims);
+        if (ntens > 1 && !req->impurity) return fail(TNR_ERR_ARG, "null impurity tensor list");
+        const int nleg = req->kind == TNR_HOTRG_3D ? 6 : 4;
+        Layout L = abi_layout(req->zn, req->leg_dims, nleg, 2);
         if ((int64_t)req->ndata != L.size || L.size == 0)
             return fail(TNR_ERR_ARG, "ndata does not match the sector dimensions");
-        for (size_t i = 0; i < req->ndata; ++i)
-            if (!std::isfinite(req->data[i]))
-                return fail(TNR_ERR_NONFINITE, "the tensor has non-finite entries");
+        std::vector<const double*> tens{req->data};
+        for (int i = 1; i < ntens; ++i) {
+            if (!req->impurity[i - 1]) return fail(TNR_ERR_ARG, "null impurity tensor");
+            tens.push_back(req->impurity[i - 1]);
+        }
+        for (const double* t : tens)
+            for (size_t i = 0; i < req->ndata; ++i)
+                if (!std::isfinite(t[i]))
+                    return fail(TNR_ERR_NONFINITE, "the tensor has non-finite entries");
         std::unique_ptr<Req> r(new Req());
         r->id = s->next_id++;
         r->kind = req->kind; r->zn = req->zn; r->chi = req->chi;
         r->maxiter = req->maxiter; r->fb = req->finalize_beginning ? 1 : 0; r->k = req->k;
-        r->dims.assign(req->leg_dims, req->leg_dims + 4 * req->zn);
+        r->nleg = nleg;
+        r->ncomp = scheme_components(req->kind);
+        r->dims.assign(req->leg_dims, req->leg_dims + nleg * req->zn);
         r->data.assign(req->data, req->data + req->ndata);
+        for (int i = 1; i < ntens; ++i) r->imp.emplace_back(tens[i], tens[i] + req->ndata);
         r->est = scheme_hbm_estimate(r->kind, r->zn, r->chi);
         if (r->est > s->budget) return fail(TNR_ERR_BUDGET, "request exceeds the HBM budget");
         r->t_submit = now_s(s->t0);
@@ -825,7 +874,7 @@ int tnr_server_poll(tnr_server* s, int64_t id, double* norms, int* n_out, double
         if (r.status == TNR_ERR_BUSY) return TNR_ERR_BUSY;
         if (t_submit) *t_submit = r.t_submit;
         if (t_done) *t_done = r.t_done;
-        if (n_out) *n_out = (int)r.norms.size();
+        if (n_out) *n_out = (int)(r.norms.size() / r.ncomp);
         if (norms && !r.norms.empty()) std::memcpy(norms, r.norms.data(), r.norms.size() * sizeof(double));
         return r.status;
     });

@@ -101,6 +101,10 @@ struct JacobiDesc {
     const int* skip;              // optional
     double* fail = nullptr;       // optional: set to 1 when the sweep cap is
                                   // reached with a rotation still due
+    int32_t natural = 0;          // 1: leave eigenpair i in position i (the
+                                  // accumulated rotation, close to the
+                                  // identity for a nearly diagonal G) instead
+                                  // of ordering by |lambda| descending
 };
 
 // Column-norm / normalise job (also the Frobenius-norm job).

@@ -867,13 +867,17 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
     }
     // sweep cap reached: converged only if the last sweep left nothing to rotate
     if (!converged && violated() && t == 0 && D.fail) *D.fail = 1.0;
-    // order by |lambda| descending (stable by index) and write out
+    // order by |lambda| descending (stable by index), or keep the diagonal
+    // positions, and write out
     if (t < n) {
         const double li = G[t + t * ld], ai = fabs(li);
-        int k = 0;
-        for (int j = 0; j < n; ++j) {
-            const double aj = fabs(G[j + j * ld]);
-            k += (aj > ai) || (aj == ai && j < t);
+        int k = t;
+        if (!D.natural) {
+            k = 0;
+            for (int j = 0; j < n; ++j) {
+                const double aj = fabs(G[j + j * ld]);
+                k += (aj > ai) || (aj == ai && j < t);
+            }
         }
         pos[t] = k;
         D.lam[k] = li;
@@ -1065,7 +1069,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __global__ __launch_bounds__(1024) void k_colnorm(const ColDesc* __restrict__ descs, int nd,
                                                   double rtol) {
-    __shared__ double sig[LMAX];
+    __shared__ double wmax[16];
     const ColDesc D = descs[blockIdx.x];
     if (D.skip && *D.skip) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -1079,13 +1083,19 @@ __global__ __launch_bounds__(1024) void k_colnorm(const ColDesc* __restrict__ de
         }
         for (; i < D.nrow; i += 64) a0 = fma(y[i], y[i], a0);
         double acc = wave_sum((a0 + a1) + (a2 + a3));
-        if (lane == 0) sig[j] = sqrt(acc);
+        if (lane == 0) D.sigma[j] = sqrt(acc);   // raw norms first (any column count)
     }
     __syncthreads();
     double smax = 0.0;
-    for (int j = 0; j < D.ncol; ++j) smax = fmax(smax, sig[j]);
+    for (int j = t; j < D.ncol; j += 1024) smax = fmax(smax, D.sigma[j]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) smax = fmax(smax, __shfl_xor(smax, o, 64));
+    if (lane == 0) wmax[w] = smax;
+    __syncthreads();
+    smax = wmax[0];
+    for (int q = 1; q < 16; ++q) smax = fmax(smax, wmax[q]);
     for (int j = w; j < D.ncol; j += 16) {
-        const double sj = sig[j];
+        const double sj = D.sigma[j];
         const bool live = sj > rtol * smax && sj > 0.0;
         const double inv = live ? 1.0 / sj : 0.0;
         const double* y = D.Y + (int64_t)j * D.ldy;

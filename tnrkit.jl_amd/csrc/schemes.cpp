@@ -196,6 +196,9 @@ std::vector<Factor> svd_trunc_many(Ctx& ctx, const std::vector<const BT*>& Ts,
             const double* vsrc = out[p].V.d();
             int ldus = out[p].ldu, ldvs = out[p].ldv;
             if (canonical) {
+                TNR_REQUIRE(kc[p] <= LMAX, TNR_ERR_ARG,
+                            "the pivoted-row SVD gauge of ImpurityTRG keeps at most 96 values "
+                            "per symmetry block");
                 const int k = kc[p], m = in[p].m, n = in[p].n;
                 int knn = 0;
                 cl_off.push_back(allcl.size());
@@ -1143,7 +1146,10 @@ size_t scheme_hbm_estimate(int kind, int N, int chi) {
     const double c4 = std::pow((double)chi, 4) / std::max(1, N) * 8.0;
     const double c5 = std::pow((double)chi, 5) / std::max(1, N) * 8.0;
     double b = 8.0 * c4 + 4.0 * (double)chi * chi * LMAX * 8.0 * 6.0;
-    if (kind == TNR_HOTRG) b += 3.0 * c5;
+    // impurity methods carry their impurity tensors through every step
+    if (kind == TNR_IMPURITY_TRG || kind == TNR_IMPURITY_HOTRG)
+        b += 2.0 * c4 * (scheme_tensors(kind) - 1);
+    if (kind == TNR_HOTRG || kind == TNR_IMPURITY_HOTRG) b += 3.0 * c5;
     if (kind == TNR_HOTRG_3D) b += 4.0 * std::pow((double)chi, 8) / std::max(1, N) * 8.0;
     return (size_t)b + (size_t(8) << 20);
 }

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <numeric>
 #include <string>
 
 namespace tnr {
@@ -273,6 +274,482 @@ void iterate(Ctx& ctx, const ConvState& cs, F&& issue_one) {
     }
 }
 
+// ------------------------------------------------------------ wide blocks
+// Blocks whose Rayleigh-Ritz step is wider than LMAX (chi above LMAX - 8 on
+// a block larger than LMAX) run the same block subspace iteration with a
+// block one-sided Jacobi as the Rayleigh-Ritz step: the l columns of Y are
+// cut into blocks of at most LMAX / 2; in each round of a tournament over
+// the blocks every disjoint block pair [Y_I Y_J] is gathered, its Gram
+// matrix (MFMA GEMM) diagonalised by k_jacobi and the pair rotated by a GEMM
+// (and the accumulated rotation W with it); sweeps repeat until no pair
+// rotates.  Columns are then normalised and put in order of decreasing
+// norm.  The host steers the loop (one readback per sweep and per
+// iteration): wide blocks only occur at large chi, where the arithmetic
+// dwarfs the synchronisation.
+constexpr int WIDE_B = LMAX / 2;
+
+struct WideRR {
+    double* Y;           // rows x l, ld ldy: rotated in place
+    int rows, l, ldy;
+    double* U;           // rows x l (ld rows): normalised columns, by decreasing norm
+    double* sigma;       // l norms, decreasing (0 for null columns)
+    double* Wtot;        // optional l x l (ld l): Y_in Wtot = U diag(sigma)
+};
+
+int block_count(int l) {
+    const int nb = (l + WIDE_B - 1) / WIDE_B;
+    return nb + (nb & 1);   // even, for the tournament (the last block may be empty)
+}
+
+// columns [b0, b1) of block b of an l-wide job
+void block_cols(int l, int b, int& b0, int& b1) {
+    b0 = std::min(l, b * WIDE_B);
+    b1 = std::min(l, (b + 1) * WIDE_B);
+}
+
+// A job is converged after a sweep in which no pair needed a rotation, i.e.
+// every pair's Gram matrix had relative off-diagonals below the threshold.
+// The Gram matrices are formed anew from the rotated columns each round, so
+// their roundoff (~1e-16 sqrt(rows) relative) sets a floor: the pair solves
+// use at least WIDE_TOL.
+constexpr double WIDE_TOL = 1e-13;
+
+// One-sided block Jacobi on every job; returns TNR_OK or TNR_ERR_NOCONV per job.
+std::vector<int> wide_rr(Ctx& ctx, const std::vector<WideRR>& jobs, double tol, int max_sweeps) {
+    const SolverCfg& cfg = solver_cfg();
+    tol = std::max(tol, WIDE_TOL);
+    const size_t J = jobs.size();
+    std::vector<int> status(J, TNR_OK);
+    // accumulated rotations start at the identity
+    for (size_t j = 0; j < J; ++j) {
+        if (!jobs[j].Wtot) continue;
+        const int l = jobs[j].l;
+        std::vector<double> id((size_t)l * l, 0.0);
+        for (int i = 0; i < l; ++i) id[(size_t)i * l + i] = 1.0;
+        void* t = ctx.stage->put(ctx.stream, id.data(), id.size() * sizeof(double));
+        TNR_HIP(hipMemcpyAsync(jobs[j].Wtot, t, id.size() * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx.stream));
+    }
+    std::vector<char> live(J, 1);
+    for (int sw = 0;; ++sw) {
+        int nlive = 0;
+        for (size_t j = 0; j < J; ++j) nlive += live[j];
+        if (!nlive) break;
+        if (sw == max_sweeps) {
+            for (size_t j = 0; j < J; ++j)
+                if (live[j]) status[j] = TNR_ERR_NOCONV;
+            break;
+        }
+        int maxnb = 0;
+        for (size_t j = 0; j < J; ++j)
+            if (live[j]) maxnb = std::max(maxnb, block_count(jobs[j].l));
+        // sweep counts of every pair solve of this sweep, and its job
+        std::vector<int> owner;
+        Buf counts(&ctx.pool, std::max<size_t>(1, J * maxnb * maxnb) * sizeof(int));
+        int npairs = 0;
+        for (int rnd = 0; rnd + 1 < maxnb; ++rnd) {
+            struct Pair { size_t j; int c0[2], c1[2]; int c; Buf Z, Z2, G, Q, lam, Wz, Wz2; };
+            std::vector<Pair> P;
+            for (size_t j = 0; j < J; ++j) {
+                if (!live[j]) continue;
+                const int nb = block_count(jobs[j].l);
+                if (rnd + 1 >= nb) continue;
+                const int m = nb - 1;
+                for (int slot = 0; slot < nb / 2; ++slot) {
+                    // circle method: slot 0 fixed at position 0, the rest rotate
+                    int a = slot == 0 ? 0 : 1 + (slot - 1 + rnd) % m;
+                    int b = 1 + (nb - 2 - slot + rnd) % m;
+                    Pair q;
+                    q.j = j;
+                    block_cols(jobs[j].l, a, q.c0[0], q.c1[0]);
+                    block_cols(jobs[j].l, b, q.c0[1], q.c1[1]);
+                    const int wa = q.c1[0] - q.c0[0], wb = q.c1[1] - q.c0[1];
+                    if (wa <= 0 || wb <= 0) continue;
+                    q.c = wa + wb;
+                    P.push_back(std::move(q));
+                }
+            }
+            if (P.empty()) continue;
+            std::vector<GatherDesc> gin, gout;
+            std::vector<GemmJob> gram, rot;
+            std::vector<JacobiDesc> jd;
+            int cmax = 0;
+            for (auto& q : P) {
+                const WideRR& R = jobs[q.j];
+                q.Z = Buf(&ctx.pool, (size_t)R.rows * q.c * 8);
+                q.Z2 = Buf(&ctx.pool, (size_t)R.rows * q.c * 8);
+                q.G = Buf(&ctx.pool, (size_t)q.c * q.c * 8);
+                q.Q = Buf(&ctx.pool, (size_t)q.c * q.c * 8);
+                q.lam = Buf(&ctx.pool, (size_t)q.c * 8);
+                int o = 0;
+                for (int h = 0; h < 2; ++h) {
+                    const int w = q.c1[h] - q.c0[h];
+                    GatherDesc g{};
+                    g.in = R.Y + (int64_t)q.c0[h] * R.ldy; g.out = q.Z.d() + (int64_t)o * R.rows;
+                    g.rows = R.rows; g.cols = w; g.ldi = R.ldy; g.ldo = R.rows;
+                    gin.push_back(g);
+                    GatherDesc b{};
+                    b.in = q.Z2.d() + (int64_t)o * R.rows; b.out = R.Y + (int64_t)q.c0[h] * R.ldy;
+                    b.rows = R.rows; b.cols = w; b.ldi = R.rows; b.ldo = R.ldy;
+                    gout.push_back(b);
+                    o += w;
+                }
+                gram.push_back(GemmJob{q.Z.d(), q.Z.d(), q.G.d(), q.c, q.c, R.rows, R.rows, R.rows,
+                                       q.c, 1, 0});
+                // eigenvectors left in place: a pair's rotation stays close to
+                // the identity, so columns do not migrate between blocks (with
+                // the eigenpairs sorted, the sweeps hardly converge)
+                JacobiDesc d{q.G.d(), q.Q.d(), q.lam.d(), counts.as<int>() + npairs, q.c, q.c, q.c};
+                d.natural = 1;
+                jd.push_back(d);
+                owner.push_back((int)q.j);
+                ++npairs;
+                rot.push_back(GemmJob{q.Z.d(), q.Q.d(), q.Z2.d(), R.rows, q.c, q.c, R.rows, q.c,
+                                      R.rows, 0, 0});
+                if (R.Wtot) {
+                    q.Wz = Buf(&ctx.pool, (size_t)R.l * q.c * 8);
+                    q.Wz2 = Buf(&ctx.pool, (size_t)R.l * q.c * 8);
+                    int o2 = 0;
+                    for (int h = 0; h < 2; ++h) {
+                        const int w = q.c1[h] - q.c0[h];
+                        GatherDesc g{};
+                        g.in = R.Wtot + (int64_t)q.c0[h] * R.l; g.out = q.Wz.d() + (int64_t)o2 * R.l;
+                        g.rows = R.l; g.cols = w; g.ldi = R.l; g.ldo = R.l;
+                        gin.push_back(g);
+                        GatherDesc b{};
+                        b.in = q.Wz2.d() + (int64_t)o2 * R.l; b.out = R.Wtot + (int64_t)q.c0[h] * R.l;
+                        b.rows = R.l; b.cols = w; b.ldi = R.l; b.ldo = R.l;
+                        gout.push_back(b);
+                        o2 += w;
+                    }
+                    rot.push_back(GemmJob{q.Wz.d(), q.Q.d(), q.Wz2.d(), R.l, q.c, q.c, R.l, q.c, R.l,
+                                          0, 0});
+                }
+                cmax = std::max(cmax, q.c);
+            }
+            gather_many(ctx, gin);
+            gemm_grouped(ctx, gram);
+            JacobiDesc* djd = ctx.stage->put(ctx.stream, jd);
+            launch_jacobi(ctx.stream, djd, (int)jd.size(), cmax, tol, cfg.jacobi_sweeps);
+            gemm_grouped(ctx, rot);
+            gather_many(ctx, gout);
+        }
+        // a job whose pairs needed no rotation in this sweep is done
+        std::vector<int> h(std::max(1, npairs));
+        if (npairs) ctx.fetch(h.data(), counts.d(), (size_t)npairs * sizeof(int));
+        std::vector<char> rotated(J, 0);
+        for (int i = 0; i < npairs; ++i)
+            if (h[i] > 0) rotated[owner[i]] = 1;
+        for (size_t j = 0; j < J; ++j)
+            if (live[j] && !rotated[j]) live[j] = 0;
+    }
+    // norms, then columns in order of decreasing norm
+    std::vector<Buf> Yn(J), sg(J);
+    std::vector<ColDesc> cd;
+    for (size_t j = 0; j < J; ++j) {
+        const WideRR& R = jobs[j];
+        Yn[j] = Buf(&ctx.pool, (size_t)R.rows * R.l * 8);
+        sg[j] = Buf(&ctx.pool, (size_t)R.l * 8);
+        cd.push_back(ColDesc{R.Y, Yn[j].d(), sg[j].d(), R.rows, R.l, R.ldy, R.rows});
+    }
+    ColDesc* dcd = ctx.stage->put(ctx.stream, cd);
+    launch_colnorm(ctx.stream, dcd, (int)cd.size(), cfg.null_rtol);
+    std::vector<const double*> src;
+    std::vector<int> cnt;
+    std::vector<std::vector<double>> hs(J);
+    std::vector<std::vector<double>*> dst;
+    for (size_t j = 0; j < J; ++j) {
+        src.push_back(sg[j].d());
+        cnt.push_back(jobs[j].l);
+        dst.push_back(&hs[j]);
+    }
+    readback_vectors(ctx, src, cnt, dst);
+    std::vector<GatherDesc> perm;
+    std::vector<double> sorted_all;
+    std::vector<size_t> soff(J);
+    std::vector<Buf> Wsrc(J);
+    for (size_t j = 0; j < J; ++j) {
+        const WideRR& R = jobs[j];
+        std::vector<int> ord(R.l);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int a, int b) { return hs[j][a] > hs[j][b]; });
+        soff[j] = sorted_all.size();
+        if (R.Wtot) {   // permute a copy of the accumulated rotation
+            Wsrc[j] = Buf(&ctx.pool, (size_t)R.l * R.l * 8);
+            GatherDesc g{};
+            g.in = R.Wtot; g.out = Wsrc[j].d(); g.rows = R.l; g.cols = R.l; g.ldi = R.l; g.ldo = R.l;
+            perm.push_back(g);
+        }
+        for (int c = 0; c < R.l; ++c) sorted_all.push_back(hs[j][ord[c]]);
+        hs[j].assign(ord.begin(), ord.end());   // keep the order
+    }
+    gather_many(ctx, perm);
+    perm.clear();
+    for (size_t j = 0; j < J; ++j) {
+        const WideRR& R = jobs[j];
+        for (int c = 0; c < R.l; ++c) {
+            const int from = (int)hs[j][c];
+            GatherDesc g{};
+            g.in = Yn[j].d() + (int64_t)from * R.rows; g.out = R.U + (int64_t)c * R.rows;
+            g.rows = R.rows; g.cols = 1; g.ldi = R.rows; g.ldo = R.rows;
+            perm.push_back(g);
+            if (R.Wtot) {
+                GatherDesc w{};
+                w.in = Wsrc[j].d() + (int64_t)from * R.l; w.out = R.Wtot + (int64_t)c * R.l;
+                w.rows = R.l; w.cols = 1; w.ldi = R.l; w.ldo = R.l;
+                perm.push_back(w);
+            }
+        }
+        void* t = ctx.stage->put(ctx.stream, sorted_all.data() + soff[j], (size_t)R.l * 8);
+        TNR_HIP(hipMemcpyAsync(R.sigma, t, (size_t)R.l * 8, hipMemcpyDeviceToDevice, ctx.stream));
+    }
+    gather_many(ctx, perm);
+    return status;
+}
+
+// block width of a wide problem
+int wide_block_size(int kmax, int cap) {
+    const SolverCfg& cfg = solver_cfg();
+    const int l = std::max((int)std::ceil(cfg.block_mult * kmax), kmax + cfg.block_extra);
+    return std::min(l, cap);
+}
+
+// Host-side stopping rule of the wide iteration (the device rule of k_conv):
+// prod *= min(1, sigma_l / sigma_kk)^(squared ? 2 : 1) from min_iters on; a
+// block as wide as the matrix (`whole`) spans its whole range and is exact.
+bool wide_converged(const std::vector<double>& sv, int kmax, bool squared, bool whole, int it,
+                    double& prod) {
+    const SolverCfg& cfg = solver_cfg();
+    if (whole) {
+        prod = 0.0;
+        return it >= cfg.min_iters;
+    }
+    int kk = 0;
+    for (int i = 0; i < kmax && i < (int)sv.size(); ++i)
+        if (sv[i] > cfg.null_rtol * sv[0]) kk = i + 1;
+    double r = 0.0;
+    if (kk > 0 && sv[kk - 1] > 0.0) {
+        r = std::min(1.0, sv.back() / sv[kk - 1]);
+        if (squared) r *= r;
+    }
+    if (it >= cfg.min_iters) prod *= r;
+    return it >= cfg.min_iters && prod < cfg.conv_target;
+}
+
+void svd_wide(Ctx& ctx, const std::vector<SvdIn>& in, const std::vector<int>& idx,
+              std::vector<SvdOut>& out) {
+    const SolverCfg& cfg = solver_cfg();
+    struct WS { Buf V, Y, Z, U, sY, sZ, Wt; int l; double prod = 1.0; bool done = false; };
+    std::vector<WS> w(idx.size());
+    std::vector<FillDesc> fills;
+    for (size_t i = 0; i < idx.size(); ++i) {
+        const SvdIn& s = in[idx[i]];
+        WS& x = w[i];
+        x.l = wide_block_size(s.kmax, std::min(s.m, s.n));
+        x.V = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
+        x.Z = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
+        x.Y = Buf(&ctx.pool, (size_t)s.m * x.l * 8);
+        x.U = Buf(&ctx.pool, (size_t)s.m * x.l * 8);
+        x.sY = Buf(&ctx.pool, (size_t)x.l * 8);
+        x.sZ = Buf(&ctx.pool, (size_t)x.l * 8);
+        fills.push_back(FillDesc{x.V.d(), s.n, x.l, s.n, s.seed, 0});
+    }
+    randfill(ctx, fills);
+    for (int it = 1; it <= cfg.max_iters; ++it) {
+        std::vector<size_t> act;
+        for (size_t i = 0; i < idx.size(); ++i)
+            if (!w[i].done) act.push_back(i);
+        if (act.empty()) break;
+        std::vector<GemmJob> mv, mtu;
+        std::vector<WideRR> ry, rz;
+        for (size_t i : act) {
+            const SvdIn& s = in[idx[i]];
+            WS& x = w[i];
+            mv.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, x.l, s.n, s.ld, s.n, s.m, 0, 0});
+            ry.push_back(WideRR{x.Y.d(), s.m, x.l, s.m, x.U.d(), x.sY.d(), nullptr});
+            mtu.push_back(GemmJob{s.M, x.U.d(), x.Z.d(), s.n, x.l, s.m, s.ld, s.m, s.n, 1, 0});
+            rz.push_back(WideRR{x.Z.d(), s.n, x.l, s.n, x.V.d(), x.sZ.d(), nullptr});
+        }
+        gemm_grouped(ctx, mv);
+        auto sty = wide_rr(ctx, ry, cfg.jacobi_tol_iter, cfg.jacobi_sweeps_iter);
+        gemm_grouped(ctx, mtu);
+        auto stz = wide_rr(ctx, rz, cfg.jacobi_tol_iter, cfg.jacobi_sweeps_iter);
+        std::vector<const double*> src;
+        std::vector<int> cnt;
+        std::vector<std::vector<double>> hv(act.size());
+        std::vector<std::vector<double>*> dst;
+        for (size_t a = 0; a < act.size(); ++a) {
+            src.push_back(w[act[a]].sZ.d());
+            cnt.push_back(w[act[a]].l);
+            dst.push_back(&hv[a]);
+        }
+        readback_vectors(ctx, src, cnt, dst);
+        for (size_t a = 0; a < act.size(); ++a) {
+            WS& x = w[act[a]];
+            SvdOut& o = out[idx[act[a]]];
+            if (sty[a] != TNR_OK || stz[a] != TNR_OK) o.status = TNR_ERR_NOCONV;
+            o.iters = it;
+            const SvdIn& sa = in[idx[act[a]]];
+            x.done = wide_converged(hv[a], sa.kmax, true, x.l >= std::min(sa.m, sa.n), it, x.prod);
+            if (!x.done && it == cfg.max_iters) o.status = TNR_ERR_NOCONV;
+            o.conv = x.prod;
+        }
+    }
+    // the loop orthogonalises to its looser threshold: V is rebuilt from the
+    // last Z = M^T U at full accuracy before the extraction relies on
+    // V^T V = I; then the Ritz triplets from Y = M V: U, sigma, V <- V Wtot
+    {
+        std::vector<WideRR> rv;
+        for (size_t i = 0; i < idx.size(); ++i) {
+            const SvdIn& s = in[idx[i]];
+            WS& x = w[i];
+            rv.push_back(WideRR{x.Z.d(), s.n, x.l, s.n, x.V.d(), x.sZ.d(), nullptr});
+        }
+        auto stv = wide_rr(ctx, rv, cfg.jacobi_tol, cfg.jacobi_sweeps);
+        for (size_t i = 0; i < idx.size(); ++i)
+            if (stv[i] != TNR_OK) out[idx[i]].status = TNR_ERR_NOCONV;
+    }
+    std::vector<GemmJob> mv, vw;
+    std::vector<WideRR> rf;
+    for (size_t i = 0; i < idx.size(); ++i) {
+        const SvdIn& s = in[idx[i]];
+        WS& x = w[i];
+        x.Wt = Buf(&ctx.pool, (size_t)x.l * x.l * 8);
+        mv.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, x.l, s.n, s.ld, s.n, s.m, 0, 0});
+        rf.push_back(WideRR{x.Y.d(), s.m, x.l, s.m, x.U.d(), x.sY.d(), x.Wt.d()});
+        vw.push_back(GemmJob{x.V.d(), x.Wt.d(), x.Z.d(), s.n, x.l, x.l, s.n, x.l, s.n, 0, 0});
+    }
+    gemm_grouped(ctx, mv);
+    auto stf = wide_rr(ctx, rf, cfg.jacobi_tol, cfg.jacobi_sweeps);
+    gemm_grouped(ctx, vw);
+    for (size_t i = 0; i < idx.size(); ++i) {
+        SvdOut& o = out[idx[i]];
+        WS& x = w[i];
+        if (stf[i] != TNR_OK) o.status = TNR_ERR_NOCONV;
+        o.U = std::move(x.U); o.ldu = in[idx[i]].m;
+        o.V = std::move(x.Z); o.ldv = in[idx[i]].n;
+        o.s = std::move(x.sY);
+    }
+}
+
+void eigh_wide(Ctx& ctx, const std::vector<EighIn>& in, const std::vector<int>& idx,
+               std::vector<EighOut>& out) {
+    const SolverCfg& cfg = solver_cfg();
+    struct WS { Buf V, Y, sY, H, Uh, sH, Wt; int l; double prod = 1.0; bool done = false; };
+    std::vector<WS> w(idx.size());
+    std::vector<FillDesc> fills;
+    std::vector<ColDesc> sym;
+    int nmax = 0;
+    for (size_t i = 0; i < idx.size(); ++i) {
+        const EighIn& s = in[idx[i]];
+        WS& x = w[i];
+        x.l = wide_block_size(s.kmax, s.n);
+        x.V = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
+        x.Y = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
+        x.sY = Buf(&ctx.pool, (size_t)x.l * 8);
+        fills.push_back(FillDesc{x.V.d(), s.n, x.l, s.n, s.seed, 0});
+        sym.push_back(ColDesc{s.M, nullptr, nullptr, s.n, s.n, s.ld, 0});
+        nmax = std::max(nmax, s.n);
+    }
+    ColDesc* dsym = ctx.stage->put(ctx.stream, sym);
+    launch_symmetrize(ctx.stream, dsym, (int)sym.size(), nmax);
+    randfill(ctx, fills);
+    for (int it = 1; it <= cfg.max_iters; ++it) {
+        std::vector<size_t> act;
+        for (size_t i = 0; i < idx.size(); ++i)
+            if (!w[i].done) act.push_back(i);
+        if (act.empty()) break;
+        std::vector<GemmJob> mv;
+        std::vector<WideRR> ry;
+        for (size_t i : act) {
+            const EighIn& s = in[idx[i]];
+            WS& x = w[i];
+            mv.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, x.l, s.n, s.ld, s.n, s.n, 0, 0});
+            ry.push_back(WideRR{x.Y.d(), s.n, x.l, s.n, x.V.d(), x.sY.d(), nullptr});
+        }
+        gemm_grouped(ctx, mv);
+        auto sty = wide_rr(ctx, ry, cfg.jacobi_tol_iter, cfg.jacobi_sweeps_iter);
+        std::vector<const double*> src;
+        std::vector<int> cnt;
+        std::vector<std::vector<double>> hv(act.size());
+        std::vector<std::vector<double>*> dst;
+        for (size_t a = 0; a < act.size(); ++a) {
+            src.push_back(w[act[a]].sY.d());
+            cnt.push_back(w[act[a]].l);
+            dst.push_back(&hv[a]);
+        }
+        readback_vectors(ctx, src, cnt, dst);
+        for (size_t a = 0; a < act.size(); ++a) {
+            WS& x = w[act[a]];
+            EighOut& o = out[idx[act[a]]];
+            if (sty[a] != TNR_OK) o.status = TNR_ERR_NOCONV;
+            o.iters = it;
+            x.done = wide_converged(hv[a], in[idx[act[a]]].kmax, false, x.l >= in[idx[act[a]]].n, it,
+                                    x.prod);
+            if (!x.done && it == cfg.max_iters) o.status = TNR_ERR_NOCONV;
+            o.conv = x.prod;
+        }
+    }
+    // V at full orthonormality from the last Y = M V (the loop uses a looser
+    // threshold)
+    {
+        std::vector<WideRR> rv;
+        for (size_t i = 0; i < idx.size(); ++i) {
+            const EighIn& s = in[idx[i]];
+            WS& x = w[i];
+            rv.push_back(WideRR{x.Y.d(), s.n, x.l, s.n, x.V.d(), x.sY.d(), nullptr});
+        }
+        auto stv = wide_rr(ctx, rv, cfg.jacobi_tol, cfg.jacobi_sweeps);
+        for (size_t i = 0; i < idx.size(); ++i)
+            if (stv[i] != TNR_OK) out[idx[i]].status = TNR_ERR_NOCONV;
+    }
+    // Ritz pairs: H = V^T M V; one-sided Jacobi H W = Uh diag(sigma) gives the
+    // eigenvectors W (sigma = |lambda|) and lambda_i = sigma_i (w_i . uh_i)
+    std::vector<GemmJob> mv, hh;
+    std::vector<WideRR> rh;
+    for (size_t i = 0; i < idx.size(); ++i) {
+        const EighIn& s = in[idx[i]];
+        WS& x = w[i];
+        x.H = Buf(&ctx.pool, (size_t)x.l * x.l * 8);
+        x.Uh = Buf(&ctx.pool, (size_t)x.l * x.l * 8);
+        x.sH = Buf(&ctx.pool, (size_t)x.l * 8);
+        x.Wt = Buf(&ctx.pool, (size_t)x.l * x.l * 8);
+        mv.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, x.l, s.n, s.ld, s.n, s.n, 0, 0});
+        hh.push_back(GemmJob{x.V.d(), x.Y.d(), x.H.d(), x.l, x.l, s.n, s.n, s.n, x.l, 1, 0});
+        rh.push_back(WideRR{x.H.d(), x.l, x.l, x.l, x.Uh.d(), x.sH.d(), x.Wt.d()});
+    }
+    gemm_grouped(ctx, mv);
+    gemm_grouped(ctx, hh);
+    auto sth = wide_rr(ctx, rh, cfg.jacobi_tol, cfg.jacobi_sweeps);
+    std::vector<GemmJob> vu;
+    for (size_t i = 0; i < idx.size(); ++i) {
+        const EighIn& s = in[idx[i]];
+        WS& x = w[i];
+        EighOut& o = out[idx[i]];
+        if (sth[i] != TNR_OK) o.status = TNR_ERR_NOCONV;
+        // signs of the eigenvalues on the host (l x l, a large-chi path)
+        const size_t ll = (size_t)x.l * x.l;
+        std::vector<double> hW(ll), hU(ll), hs(x.l);
+        ctx.fetch(hW.data(), x.Wt.d(), ll * 8);
+        ctx.fetch(hU.data(), x.Uh.d(), ll * 8);
+        ctx.fetch(hs.data(), x.sH.d(), (size_t)x.l * 8);
+        for (int c = 0; c < x.l; ++c) {
+            double d = 0.0;
+            for (int r = 0; r < x.l; ++r) d += hW[(size_t)c * x.l + r] * hU[(size_t)c * x.l + r];
+            if (d < 0.0) hs[c] = -hs[c];
+        }
+        o.lam = Buf(&ctx.pool, (size_t)x.l * 8);
+        void* t = ctx.stage->put(ctx.stream, hs.data(), (size_t)x.l * 8);
+        TNR_HIP(hipMemcpyAsync(o.lam.d(), t, (size_t)x.l * 8, hipMemcpyDeviceToDevice, ctx.stream));
+        o.U = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
+        o.ncols = x.l;
+        o.direct = false;
+        vu.push_back(GemmJob{x.V.d(), x.Wt.d(), o.U.d(), s.n, x.l, x.l, s.n, x.l, s.n, 0, 0});
+    }
+    gemm_grouped(ctx, vu);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------- SVD
@@ -281,18 +758,16 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
     out.resize(in.size());
     const SolverCfg& cfg = solver_cfg();
     Buf failv = fail_flags(ctx, in.size());
-    std::vector<int> direct, iter;
+    std::vector<int> direct, iter, wide;
     for (size_t p = 0; p < in.size(); ++p) {
         const SvdIn& s = in[p];
         TNR_REQUIRE(s.m > 0 && s.n > 0 && s.kmax > 0 && s.kmax <= std::min(s.m, s.n),
                     TNR_ERR_INTERNAL, "bad SVD problem");
         if (std::min(s.m, s.n) <= LMAX) direct.push_back((int)p);
-        else {
-            TNR_REQUIRE(s.kmax <= LMAX - 8, TNR_ERR_ARG,
-                        "chi above 88 is not supported for blocks larger than 96");
-            iter.push_back((int)p);
-        }
+        else if (s.kmax <= LMAX - 8) iter.push_back((int)p);
+        else wide.push_back((int)p);
     }
+    if (!wide.empty()) svd_wide(ctx, in, wide, out);
     // ---------------- direct problems: Gram/Jacobi twice on the smaller side
     if (!direct.empty()) {
         struct DW { Buf Y, Y2, G, W, W2, Wt, lam, sig, sw; };
@@ -468,17 +943,15 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
     out.resize(in.size());
     const SolverCfg& cfg = solver_cfg();
     Buf failv = fail_flags(ctx, in.size());
-    std::vector<int> direct, iter;
+    std::vector<int> direct, iter, wide;
     for (size_t p = 0; p < in.size(); ++p) {
         const EighIn& s = in[p];
         TNR_REQUIRE(s.n > 0 && s.kmax > 0 && s.kmax <= s.n, TNR_ERR_INTERNAL, "bad eigh problem");
         if (s.n <= LMAX) direct.push_back((int)p);
-        else {
-            TNR_REQUIRE(s.kmax <= LMAX - 8, TNR_ERR_ARG,
-                        "chi above 88 is not supported for blocks larger than 96");
-            iter.push_back((int)p);
-        }
+        else if (s.kmax <= LMAX - 8) iter.push_back((int)p);
+        else wide.push_back((int)p);
     }
+    if (!wide.empty()) eigh_wide(ctx, in, wide, out);
     if (!direct.empty()) {
         std::vector<JacobiDesc> jd;
         std::vector<Buf> sw(direct.size());

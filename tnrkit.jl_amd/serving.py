@@ -12,44 +12,60 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .tensors import SymTensor
 
 KINDS = {"TRG": _lib.TNR_TRG, "BTRG": _lib.TNR_BTRG, "HOTRG": _lib.TNR_HOTRG,
-         "ATRG": _lib.TNR_ATRG}
+         "ATRG": _lib.TNR_ATRG, "HOTRG_3D": _lib.TNR_HOTRG_3D,
+         "ImpurityTRG": _lib.TNR_IMPURITY_TRG, "ImpurityHOTRG": _lib.TNR_IMPURITY_HOTRG}
 
 
 class Request:
-    """One run: scheme kind, initial tensor, chi, maxiter."""
+    """One run: scheme kind, initial tensor(s), chi, maxiter.  ``T`` is a
+    SymTensor (an NTensor with 2 + 4 legs for HOTRG_3D), or for the impurity
+    methods the list of their tensors in constructor order."""
 
-    def __init__(self, kind, T: SymTensor, chi, maxiter, k=-0.5, finalize_beginning=True):
+    def __init__(self, kind, T, chi, maxiter, k=-0.5, finalize_beginning=True):
         self.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
-        self.T = T
+        tensors = list(T) if isinstance(T, (list, tuple)) else [T]
+        self.T = tensors[0]
+        self._extra = [np.ascontiguousarray(t.data, dtype=np.float64) for t in tensors[1:]]
+        self._extra_ptrs = (C.POINTER(C.c_double) * max(1, len(self._extra)))(
+            *[e.ctypes.data_as(C.POINTER(C.c_double)) for e in self._extra])
         self.chi = int(chi)
         self.maxiter = int(maxiter)
         self.k = float(k)
         self.finalize_beginning = bool(finalize_beginning)
-        self._dims = np.ascontiguousarray(T.leg_dims, dtype=np.int32).ravel()
+        self.ncomp = lib().tnr_ncomponents(self.kind)
+        if self.ncomp < 1:
+            check(self.ncomp)
+        self._dims = np.ascontiguousarray(self.T.leg_dims, dtype=np.int32).ravel()
 
     @property
     def n_norms(self):
+        """Rows of the run's norm output (one per finalize!)."""
         return max(self.maxiter, 1) + (1 if self.finalize_beginning else 0)
 
     def desc(self):
+        imp = C.cast(self._extra_ptrs, C.POINTER(C.POINTER(C.c_double))) if self._extra else None
         return _lib.RunDesc(self.kind, self.T.N, self._dims.ctypes.data_as(C.POINTER(C.c_int32)),
                             self.T.data.ctypes.data_as(C.POINTER(C.c_double)), self.T.data.size,
-                            self.k, self.chi, self.maxiter, int(self.finalize_beginning))
+                            self.k, self.chi, self.maxiter, int(self.finalize_beginning), imp)
+
+    def shape(self, flat):
+        """The run's norms: a vector, or rows of ncomp values for the impurity methods."""
+        return flat.copy() if self.ncomp == 1 else flat.reshape(-1, self.ncomp).copy()
 
 
 def run_batch(requests, precision=_lib.TNR_PREC_FP64):
-    """Norm vectors of every request, computed together on the device."""
+    """Norms of every request, computed together on the device."""
     descs = (_lib.RunDesc * len(requests))(*[r.desc() for r in requests])
-    out = np.zeros(sum(r.n_norms for r in requests))
+    out = np.zeros(sum(r.n_norms * r.ncomp for r in requests))
     check(lib().tnr_run_batch(len(requests), descs, precision,
                               out.ctypes.data_as(C.POINTER(C.c_double))))
     res, o = [], 0
     for r in requests:
-        res.append(out[o:o + r.n_norms].copy())
-        o += r.n_norms
+        n = r.n_norms * r.ncomp
+        res.append(r.shape(out[o:o + n]))
+        o += n
     return res
 
 
@@ -75,7 +91,7 @@ class Server:
     def poll(self, rid):
         """(norms, t_submit, t_done) or None while the request is in flight."""
         req = self._reqs[rid]
-        out = np.zeros(req.n_norms)
+        out = np.zeros(req.n_norms * req.ncomp)
         n = C.c_int()
         ts, td = C.c_double(), C.c_double()
         st = lib().tnr_server_poll(self._h, rid, out.ctypes.data_as(C.POINTER(C.c_double)),
@@ -83,7 +99,7 @@ class Server:
         if st == _lib.TNR_ERR_BUSY:
             return None
         check(st)
-        return out[:n.value], ts.value, td.value
+        return req.shape(out[:n.value * req.ncomp]), ts.value, td.value
 
     def stats(self):
         m = C.c_size_t()
