@@ -68,6 +68,19 @@ int tnr_kernel_eigh(int n, const double* M, int k, double* U, double* lam,
 int tnr_kernel_jacobi(int count, int n, const double* G, double* W,
                       double* lam, int* sweeps, double* ms_out);
 
+/* The eigensolver of tnr_kernel_jacobi on device buffers, allocating
+ * nothing: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  G, W (count x n x n, each matrix
+ * column-major, ld = n, one after another) and lam (count x n) are device
+ * pointers; sweeps (count ints, optional) is host memory; stream is a
+ * hipStream_t (NULL = the default stream), synchronised before return.  The
+ * results equal tnr_kernel_jacobi's.  The solver descriptors and sweep
+ * counts live in a 1 MiB array each GPU holds for the life of the process:
+ * count is limited to what it holds (TNR_ERR_ARG beyond, naming the limit:
+ * 13797 problems at 76 bytes each), and calls that overlap on one GPU race. */
+int tnr_kernel_jacobi_noalloc(int count, int n, const double* G, double* W,
+                              double* lam, int* sweeps, void* stream);
 /* Device-time accounting per kernel class.  A launch of one kernel is timed
  * by its dispatch's start and end timestamps (the events of
  * hipExtLaunchKernelGGL, no marker packets between launches); a class call

@@ -98,6 +98,8 @@ SIGNATURES = {
     "tnr_profile_enable": (C.c_int, [C.c_int]),
     "tnr_autotune_enable": (C.c_int, [C.c_int]),
     "tnr_solver_config": (C.c_int, [C.c_char_p]),
+    "tnr_kernel_jacobi_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            _IP, C.c_void_p]),
     "tnr_device_sync": (C.c_int, []),
     "tnr_profile_read": (C.c_int, [C.c_int, _I64P, _DP, _DP, _DP, _DP, _DP]),
 }

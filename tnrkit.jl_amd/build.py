@@ -17,7 +17,7 @@ CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = ["kernels.hip", "runtime.cpp", "blocktensor.cpp", "solver.cpp", "schemes.cpp", "observables.cpp", "capi.cpp",
-           "kapi.cpp"]
+           "kapi.cpp", "noalloc.hip"]
 
 
 def _run(cmd):
