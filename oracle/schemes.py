@@ -17,7 +17,7 @@ Contractions are pairwise in a cost-optimal order (TensorOperations'
 order only changes roundoff).  Each factorization appends its truncation
 diagnostics (smallest kept / largest dropped value) to ``scheme.log``.
 
-Pins (tests/test_oracle_schemes.py): the published BTRG free energy
+Pins (tests/test_oracle_pins.py): the published BTRG free energy
 (README.md:87), the reference's Onsager tolerances (test/schemes.jl:18-23,
 57-62, 94-99), exact small-torus partition functions for the un-truncated
 first steps of all three schemes on random tensors (oracle.torus), Z2 vs

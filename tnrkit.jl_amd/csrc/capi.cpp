@@ -570,6 +570,12 @@ struct tnr_server {
         return n;
     }
 
+    size_t pool_cached() const {
+        size_t n = 0;
+        for (auto& l : lanes) n += l->ctx.pool.cached();
+        return n;
+    }
+
     void admit() {
         std::vector<std::vector<Slot*>> fresh(lanes.size());
         int res = resident();
@@ -924,6 +930,14 @@ int tnr_run_batch(int nruns, const tnr_run_desc* runs, int precision, double* no
             ~Clear() { t_batch_running = nullptr; }
         } clear_running;
         s->reset(nruns > 0 ? nruns : 1);
+        {
+            // the budget follows what is free now: memory that handles, spare
+            // contexts or other users took or gave back since the last batch,
+            // plus the engine's own cached buffers, which it reuses
+            size_t fr = 0, tot = 0;
+            TNR_HIP(hipMemGetInfo(&fr, &tot));
+            s->budget = (size_t)(0.8 * (double)(fr + s->pool_cached()));
+        }
         std::vector<int64_t> ids(nruns);
         for (int i = 0; i < nruns; ++i) {
             int st = tnr_server_submit(s, &runs[i], &ids[i]);
