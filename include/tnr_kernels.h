@@ -109,7 +109,8 @@ int tnr_autotune_enable(int on);
  * ("key=value,..."; NULL or "" = defaults and environment only).  Keys:
  * max_iters (block iterations, default 80), sweeps / sweeps_iter (Jacobi
  * sweep caps of final and in-loop solves, 40), tol_iter (in-loop Jacobi
- * threshold, 1e-10), conv_target (1e-14), block_mult (2), block_extra (2),
+ * threshold, 1e-10), tol_first (the first iteration's, 1e-4),
+ * conv_target (1e-14), block_mult (2), block_extra (2),
  * cholqr_y (1), cholqr_z (0).  A solve that reaches max_iters or a sweep cap
  * unconverged fails its step with TNR_ERR_NOCONV.  Not thread-safe: call it
  * while no other entry point runs. */

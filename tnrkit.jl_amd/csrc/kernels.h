@@ -208,7 +208,7 @@ size_t jacobi_lds_bytes(int n);
 size_t cholqr_lds_bytes(int n);
 // pivoted Cholesky QR of Gram matrices (JacobiDesc: G in, W = P R^{-1} out,
 // lam = R_jj); columns below null_rel2 * max diag are dropped
-void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax,
+void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmin, int nmax,
                    double null_rel2);
 void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, double tol,
                    int max_sweeps);

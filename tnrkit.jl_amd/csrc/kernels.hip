@@ -903,6 +903,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
 // column j of Y after projecting out the earlier pivots).  Ties in the
 // pivot search go to the lowest index, so the result is a function of G.
 constexpr int CQT = 256;
+constexpr int CQ64 = 64;   // Gram matrices up to this size go to k_cholqr64
 
 __device__ __forceinline__ void argmax_wave(double& v, int& i) {
 #pragma unroll
@@ -923,7 +924,7 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
     __shared__ int bi[CQT / 64];
     __shared__ double dmax_s;
     const JacobiDesc D = descs[blockIdx.x];
-    if (D.skip && *D.skip) return;
+    if (D.n <= CQ64 || (D.skip && *D.skip)) return;
     const int n = D.n, ld = n + 1, t = threadIdx.x, lane = t & 63, w = t >> 6;
     double* A = lds;                   // symmetric working copy; R in its upper triangle
     double* X = A + n * ld;            // R^{-1}
@@ -1052,6 +1053,191 @@ __global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ d
     }
     for (int j = t; j < n; j += CQT) D.lam[j] = (j < rank) ? A[j + j * ld] : 0.0;
     if (t == 0 && D.sweeps) *D.sweeps = 0;
+}
+
+// Pivoted Cholesky QR of Gram matrices with at most CQ64 = 64 columns: the
+// algorithm of k_cholqr (same pivot rule and null floor, the same output)
+// in one wavefront that keeps the working matrix in registers, lane i
+// holding row i, so a pivot step has one wave-local barrier and no LDS
+// round trip per element.  Pivoting does not move data: the pivot's row is
+// broadcast through LDS and rows already pivoted drop out of the search.
+// Step j with pivot p, r = sqrt(A_j[p][p]): R[j][i] = A_j[p][i] / r (kept
+// over the original column indices), A_{j+1}[i][k] = A_j[i][k] -
+// (A_j[i][p] / r^2) A_j[p][k].  X = R^{-1} (R in pivot order) is solved
+// column by column, lane c holding column c in registers, by the
+// right-looking elimination of k_cholqr with the reciprocal diagonal.
+// Ties in the pivot search go to the lowest row index.
+constexpr int CQLD = CQ64 + 1;   // padded stride of the LDS matrices
+
+size_t cholqr64_lds_bytes() {
+    return 2 * (size_t)CQ64 * CQLD * sizeof(double)   // R rows / staging, pivot-ordered R
+           + 2 * (size_t)CQ64 * sizeof(double)       // pivot rows (two, by step parity)
+           + 2 * (size_t)CQ64 * sizeof(double)       // R[j][j], 1 / R[j][j]
+           + 2 * (size_t)CQ64 * sizeof(int);         // perm, inverse perm
+}
+
+// One step of a wave argmax over (v, i) through DPP lane exchange CTRL:
+// larger v wins, equal v goes to the lower index.
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(double& v, int& i) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    const int i2 = __builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false);
+    const double v2 = __hiloint2double(hi2, lo2);
+    const bool take = (v2 > v) | ((v2 == v) & (i2 < i));
+    v = take ? v2 : v;
+    i = take ? i2 : i;
+}
+
+// Argmax of (v, i) over the wave, the same result in every lane: within
+// each row of 16 lanes by lane swaps (xor 1, xor 2, mirror within 8, mirror
+// within 16), then over the four rows' results read from lanes 0, 16, 32, 48.
+__device__ __forceinline__ void argmax_wave64(double& v, int& i) {
+    argmax_dpp<0xB1>(v, i);    // quad_perm [1, 0, 3, 2]
+    argmax_dpp<0x4E>(v, i);    // quad_perm [2, 3, 0, 1]
+    argmax_dpp<0x141>(v, i);   // row_half_mirror
+    argmax_dpp<0x140>(v, i);   // row_mirror
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    double bv = __hiloint2double(__builtin_amdgcn_readlane(hi, 0), __builtin_amdgcn_readlane(lo, 0));
+    int bi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+        const double cv = __hiloint2double(__builtin_amdgcn_readlane(hi, 16 * r),
+                                           __builtin_amdgcn_readlane(lo, 16 * r));
+        const int ci = __builtin_amdgcn_readlane(i, 16 * r);
+        const bool take = (cv > bv) | ((cv == bv) & (ci < bi));
+        bv = take ? cv : bv;
+        bi = take ? ci : bi;
+    }
+    v = bv;
+    i = bi;
+}
+
+// (one wave per SIMD: the registers hold the rows and a whole pivot row in
+// flight)
+__global__ __launch_bounds__(CQ64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_cholqr64(
+    const JacobiDesc* __restrict__ descs, int nd, double null_rel2) {
+    extern __shared__ double lds[];
+    const JacobiDesc D = descs[blockIdx.x];
+    if (D.n > CQ64 || (D.skip && *D.skip)) return;
+    const int n = D.n, lane = threadIdx.x;
+    double* Rr = lds;                        // R[j][k] by original column: Rr[j * CQLD + k]
+    double* Rp = Rr + CQ64 * CQLD;           // G staging; later R[i][kk] at Rp[kk * CQ64 + i]
+    double* prow = Rp + CQ64 * CQLD;         // [2][CQ64]
+    double* rdiag = prow + 2 * CQ64;         // R[j][j]
+    double* rinv = rdiag + CQ64;             // 1 / R[j][j]
+    int* perm = (int*)(rinv + CQ64);
+    int* iperm = perm + CQ64;
+    const bool live = lane < n;
+    // G through LDS (coalesced column loads, zero outside n x n), then
+    // symmetrised rows into registers
+    {
+        // every lane loads (clamped, in bounds) and masks the bits, so the
+        // loads are not predicated and stay in flight together
+        const int ii = min(lane, n - 1);
+#pragma unroll
+        for (int c0 = 0; c0 < CQ64; c0 += 16) {
+            long long g[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                g[u] = __double_as_longlong(D.G[ii + (int64_t)min(c0 + u, n - 1) * D.ldg]);
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const long long keep = (live && c0 + u < n) ? -1LL : 0LL;
+                Rp[lane + (c0 + u) * CQLD] = __longlong_as_double(g[u] & keep);
+            }
+        }
+    }
+    __syncthreads();
+    double a[CQ64];
+#pragma unroll
+    for (int k = 0; k < CQ64; ++k) a[k] = 0.5 * (Rp[lane + k * CQLD] + Rp[k + lane * CQLD]);
+    double d = Rp[lane + lane * CQLD];   // = a[lane]
+    double dmax = d;
+    {
+        int ii = lane;
+        argmax_wave64(dmax, ii);
+    }
+    const double nfloor = null_rel2 * dmax;
+    bool done = !live;
+    int rank = n;
+    double v = live ? d : -1.0;
+    int p = live ? lane : n;
+    argmax_wave64(v, p);
+    for (int j = 0; j < n; ++j) {
+        if (!(v > nfloor && dmax > 0.0)) {
+            rank = j;
+            break;
+        }
+        double* pr = prow + (j & 1) * CQ64;
+        if (lane == p) {
+#pragma unroll
+            for (int k = 0; k < CQ64; k += 2) *(double2*)(pr + k) = make_double2(a[k], a[k + 1]);
+            done = true;
+        }
+        const double r = sqrt(v), ri = 1.0 / r;
+        __syncthreads();
+        const double aip = pr[lane];                  // A_j[p][i] = A_j[i][p]
+        Rr[j * CQLD + lane] = lane == p ? r : aip * ri;
+        if (lane == 0) {
+            perm[j] = p;
+            rdiag[j] = r;
+            rinv[j] = ri;
+        }
+        // rows already pivoted keep their values (zero multiplier); the next
+        // pivot search needs only d and overlaps the row updates
+        const double f = done ? 0.0 : (aip * ri) * ri;
+        d = d - f * aip;
+        v = done ? -1.0 : d;
+        p = done ? n : lane;
+        double2 row[CQ64 / 2];
+#pragma unroll
+        for (int k = 0; k < CQ64 / 2; ++k) row[k] = *(const double2*)(pr + 2 * k);
+        argmax_wave64(v, p);
+#pragma unroll
+        for (int k = 0; k < CQ64 / 2; ++k) {
+            a[2 * k] = a[2 * k] - f * row[k].x;
+            a[2 * k + 1] = a[2 * k + 1] - f * row[k].y;
+        }
+    }
+    __syncthreads();
+    // R in pivot order, column kk contiguous: Rp[kk * CQ64 + i] = R[i][kk]
+    iperm[lane] = -1;
+    for (int kk = 0; kk < rank; ++kk) Rp[kk * CQ64 + lane] = Rr[lane * CQLD + perm[kk]];
+    __syncthreads();
+    if (lane < rank) iperm[perm[lane]] = lane;
+    // X = R^{-1}: lane c holds column c; rows kk from the last up, each
+    // scaled by 1 / R[kk][kk] and eliminated from the rows above it
+    double* x = a;
+#pragma unroll
+    for (int i = 0; i < CQ64; ++i) x[i] = (i == lane && lane < rank) ? 1.0 : 0.0;
+#pragma unroll
+    for (int kk = CQ64 - 1; kk >= 0; --kk) {
+        if (kk < rank) {
+            x[kk] *= rinv[kk];
+            const double* col = Rp + kk * CQ64;
+#pragma unroll
+            for (int i = 0; i + 1 < kk; i += 2) {
+                const double2 rr = *(const double2*)(col + i);
+                x[i] = x[i] - rr.x * x[kk];
+                x[i + 1] = x[i + 1] - rr.y * x[kk];
+            }
+            if (kk & 1) x[kk - 1] = x[kk - 1] - col[kk - 1] * x[kk];
+        }
+    }
+    // stage X by columns (X[i][c] at Rr[i + c * CQLD]), then W[perm[i]][c] =
+    // X[i][c] for i <= c < rank and zero elsewhere, one coalesced column at a time
+#pragma unroll
+    for (int i = 0; i < CQ64; ++i) Rr[i + lane * CQLD] = x[i];
+    __syncthreads();
+    const int ip = iperm[lane];
+    if (live)
+        for (int c = 0; c < n; ++c)
+            D.W[lane + (int64_t)c * D.ldw] =
+                (ip >= 0 && ip <= c && c < rank) ? Rr[ip + c * CQLD] : 0.0;
+    if (live) D.lam[lane] = lane < rank ? rdiag[lane] : 0.0;
+    if (lane == 0 && D.sweeps) *D.sweeps = 0;
 }
 
 // ------------------------------------------------- column norms / normalise
@@ -1955,17 +2141,32 @@ size_t cholqr_lds_bytes(int n) {
     return 2 * (size_t)n * (n + 1) * sizeof(double) + (size_t)n * sizeof(int);
 }
 
-void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax,
+// Problems of at most CQ64 columns run on k_cholqr64, larger ones on
+// k_cholqr (each kernel skips the other's), so the kernel a problem gets
+// depends on its own size only, never on the rest of its launch group.
+void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmin, int nmax,
                    double null_rel2) {
     if (nd <= 0) return;
     static const hipError_t attr = hipFuncSetAttribute(
         (const void*)k_cholqr, hipFuncAttributeMaxDynamicSharedMemorySize,
         (int)cholqr_lds_bytes(LMAX));
     TNR_HIP(attr);
-    ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
-    TNR_LAUNCH(ps, k_cholqr, dim3(nd), dim3(CQT), cholqr_lds_bytes(nmax), st, d_descs, nd,
-                       null_rel2);
-    TNR_HIP(hipGetLastError());
+    static const hipError_t attr64 = hipFuncSetAttribute(
+        (const void*)k_cholqr64, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (int)cholqr64_lds_bytes());
+    TNR_HIP(attr64);
+    if (nmin <= CQ64) {
+        ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
+        TNR_LAUNCH(ps, k_cholqr64, dim3(nd), dim3(CQ64), cholqr64_lds_bytes(), st, d_descs, nd,
+                   null_rel2);
+        TNR_HIP(hipGetLastError());
+    }
+    if (nmax > CQ64) {
+        ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
+        TNR_LAUNCH(ps, k_cholqr, dim3(nd), dim3(CQT), cholqr_lds_bytes(nmax), st, d_descs, nd,
+                   null_rel2);
+        TNR_HIP(hipGetLastError());
+    }
 }
 
 void launch_lincomb(hipStream_t st, const LinDesc* d_descs, int nd, int64_t total) {

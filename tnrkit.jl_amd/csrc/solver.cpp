@@ -15,7 +15,7 @@
 
 namespace tnr {
 
-// "key=value,..." overrides of the defaults (keys: tol_iter, sweeps_iter,
+// "key=value,..." overrides of the defaults (keys: tol_iter, tol_first, sweeps_iter,
 // sweeps, block_mult, block_extra, conv_target, max_iters, cholqr_y,
 // cholqr_z); unknown keys are an argument error.  TNR_SOLVER_CFG in the
 // environment is applied first, then tnr_solver_config's string.
@@ -34,6 +34,7 @@ static void apply_cfg(SolverCfg& c, const char* e) {
             const std::string k = kv.substr(0, eq);
             const double v = std::atof(kv.c_str() + eq + 1);
             if (k == "tol_iter") c.jacobi_tol_iter = v;
+            else if (k == "tol_first") c.jacobi_tol_first = v;
             else if (k == "sweeps_iter") c.jacobi_sweeps_iter = (int)v;
             else if (k == "sweeps") c.jacobi_sweeps = (int)v;
             else if (k == "block_mult") c.block_mult = v;
@@ -95,10 +96,13 @@ struct RR {
 };
 
 // `passes` = 1 inside the block iteration (the next iteration corrects
-// what one pass leaves), 2 for the final Ritz extraction and direct solves.
+// what one pass leaves), 2 for the final Ritz extraction and direct solves;
+// `tight` = one pass at the final threshold, for columns that are already
+// orthogonal to the loop's threshold (the second pass of a two-pass step).
 struct RRPlan {
-    int passes = 1, lmax = 0;
+    int passes = 1, lmax = 0, lmin = 0;
     bool chol = false;          // one pass, pivoted Cholesky QR instead of Jacobi
+    bool tight = false;
     GemmPlan gram[2], rot[2];
     std::vector<JacobiDesc> jac[2];
     std::vector<ColDesc> col;
@@ -107,15 +111,20 @@ struct RRPlan {
     const ColDesc* d_col = nullptr;
 };
 
-RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool chol = false) {
+RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool chol = false,
+               bool tight = false) {
     TNR_REQUIRE(passes == 1 || passes == 2, TNR_ERR_INTERNAL, "one or two passes");
     TNR_REQUIRE(!chol || passes == 1, TNR_ERR_INTERNAL, "Cholesky QR is a one-pass step");
+    TNR_REQUIRE(!tight || (passes == 1 && !chol), TNR_ERR_INTERNAL, "tight is a one-pass Jacobi step");
     RRPlan P;
     P.passes = passes;
     P.chol = chol;
+    P.tight = tight;
+    P.lmin = jobs.empty() ? 0 : jobs[0].l;
     for (auto& j : jobs) {
         TNR_REQUIRE(passes == 2 || !j.Wtot, TNR_ERR_INTERNAL, "total rotation needs two passes");
         P.lmax = std::max(P.lmax, j.l);
+        P.lmin = std::min(P.lmin, j.l);
     }
     for (int pass = 0; pass < passes; ++pass) {
         std::vector<GemmJob> gj, rj;
@@ -166,26 +175,29 @@ void rr_bind(RRPlan& P, const DescArena& a) {
     P.d_col = a.at<ColDesc>(P.o_col);
 }
 
-void rr_issue(Ctx& ctx, RRPlan& P) {
+// `first`: the block iteration's first step, whose Rayleigh-Ritz step only
+// has to align a random basis roughly (jacobi_tol_first)
+void rr_issue(Ctx& ctx, RRPlan& P, bool first = false) {
     const SolverCfg& cfg = solver_cfg();
     for (int pass = 0; pass < P.passes; ++pass) {
         gemm_issue(ctx, P.gram[pass]);
-        const bool in_loop = P.passes == 1;
+        const bool in_loop = P.passes == 1 && !P.tight;
         if (P.chol)
-            launch_cholqr(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmax,
+            launch_cholqr(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmin, P.lmax,
                           cfg.null_rtol * cfg.null_rtol);
         else
             launch_jacobi(ctx.stream, P.d_jac[pass], (int)P.jac[pass].size(), P.lmax,
-                          in_loop ? cfg.jacobi_tol_iter : cfg.jacobi_tol,
+                          !in_loop ? cfg.jacobi_tol
+                                   : (first ? cfg.jacobi_tol_first : cfg.jacobi_tol_iter),
                           in_loop ? cfg.jacobi_sweeps_iter : cfg.jacobi_sweeps);
         gemm_issue(ctx, P.rot[pass]);
     }
     launch_colnorm(ctx.stream, P.d_col, (int)P.col.size(), cfg.null_rtol);
 }
 
-void rr_grouped(Ctx& ctx, const std::vector<RR>& jobs, int passes) {
+void rr_grouped(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool tight = false) {
     if (jobs.empty()) return;
-    RRPlan P = rr_plan(ctx, jobs, passes);
+    RRPlan P = rr_plan(ctx, jobs, passes, false, tight);
     DescArena a;
     rr_stage(P, a);
     a.commit(ctx);
@@ -267,7 +279,7 @@ void iterate(Ctx& ctx, const ConvState& cs, F&& issue_one) {
     int total = 0;
     const int chunk = 3;
     while (total < cfg.max_iters) {
-        for (int c = 0; c < chunk; ++c) issue_one();
+        for (int c = 0; c < chunk; ++c) issue_one(total + c);
         total += chunk;
         ctx.fetch(done.data(), cs.done(0), cs.P * sizeof(int));
         if (std::all_of(done.begin(), done.end(), [](int d) { return d != 0; })) break;
@@ -860,8 +872,15 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             // Ritz values of the Z side, or with Cholesky QR the pivots R_jj
             conv.push_back(ConvDesc{cfg.cholqr_z ? x.lam.d() : sv, l, s.kmax, 1, cs.it(i),
                                     cs.prod(i), cs.done(i)});
-            rV.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
-                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr, fl});
+            // the loop's last Rayleigh-Ritz step left Z W in Y2; one pass at the
+            // final threshold (into Z, free until V Wtot) rebuilds V.  With
+            // Cholesky QR on the Z side Y2 is not a Ritz basis: two passes on Z.
+            if (cfg.cholqr_z)
+                rV.push_back(RR{x.Z.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
+                                x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr, fl});
+            else
+                rV.push_back(RR{x.Y2.d(), s.n, l, s.n, x.Z.d(), x.G.d(), x.W.d(), x.W2.d(),
+                                x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr, fl});
             // Ritz extraction from Y = M V (two passes), V <- V Wtot
             jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, l, s.n, s.ld, s.n, s.m, 0, 0});
             rF.push_back(RR{x.Y.d(), s.m, l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
@@ -888,18 +907,21 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
         rr_bind(pY, a);
         rr_bind(pZ, a);
         const ConvDesc* d_conv = a.at<ConvDesc>(o_conv);
-        iterate(ctx, cs, [&] {
+        iterate(ctx, cs, [&](int it) {
             gemm_issue(ctx, pMV);
-            rr_issue(ctx, pY);
+            rr_issue(ctx, pY, it == 0);
             gemm_issue(ctx, pMtU);
-            rr_issue(ctx, pZ);
+            rr_issue(ctx, pZ, it == 0);
             launch_conv(ctx.stream, d_conv, P, cfg.null_rtol, cfg.conv_target, cfg.min_iters,
                         cfg.max_iters);
         });
         // the loop orthonormalises only to its looser Jacobi threshold: V is
-        // rebuilt from the last Z = M^T U at full accuracy before the final
-        // extraction relies on V^T V = I
-        rr_grouped(ctx, rV, 2);
+        // rebuilt at full accuracy before the final extraction relies on
+        // V^T V = I.  Its last rotated Z has columns orthogonal to that
+        // threshold, so one pass at the final threshold does what the second
+        // pass of a two-pass step does.
+        if (cfg.cholqr_z) rr_grouped(ctx, rV, 2);
+        else rr_grouped(ctx, rV, 1, true);
         gemm_grouped(ctx, jF);
         rr_grouped(ctx, rF, 2);
         gemm_grouped(ctx, jV);
@@ -1006,7 +1028,8 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             rY.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip, failv.d() + iter[i]});
             conv.push_back(ConvDesc{sv, l, s.kmax, 0, cs.it(i), cs.prod(i), cs.done(i)});
-            rV.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
+            // one tight pass on the loop's last rotated Y (in Y2), output into Y
+            rV.push_back(RR{x.Y2.d(), s.n, l, s.n, x.Y.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr, failv.d() + iter[i]});
             jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, l, s.n, s.ld, s.n, s.n, 0, 0});
             jH.push_back(GemmJob{x.V.d(), x.Y.d(), x.G.d(), l, l, s.n, s.n, s.n, l, 1, 0});
@@ -1026,15 +1049,16 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         gemm_bind(pMV, a);
         rr_bind(pY, a);
         const ConvDesc* d_conv = a.at<ConvDesc>(o_conv);
-        iterate(ctx, cs, [&] {
+        iterate(ctx, cs, [&](int it) {
             gemm_issue(ctx, pMV);
-            rr_issue(ctx, pY);
+            rr_issue(ctx, pY, it == 0);
             launch_conv(ctx.stream, d_conv, P, cfg.null_rtol, cfg.conv_target, cfg.min_iters,
                         cfg.max_iters);
         });
-        // V at full orthonormality from the last Y = M V (the loop uses a
-        // looser Jacobi threshold)
-        rr_grouped(ctx, rV, 2);
+        // V at full orthonormality from the loop's last rotated Y = M V W
+        // (the loop uses a looser Jacobi threshold; one pass at the final
+        // threshold, as in svd_grouped)
+        rr_grouped(ctx, rV, 1, true);
         // Ritz pairs: H = V^T (M V), eigendecompose, U = V W
         gemm_grouped(ctx, jF);
         gemm_grouped(ctx, jH);

@@ -72,6 +72,10 @@ struct SolverCfg {
     // keeps a tight threshold; V is still rebuilt at full accuracy (1e-15)
     // once the loop ends.
     double jacobi_tol_iter = 1e-10;
+    // The first iteration's Rayleigh-Ritz step only aligns the random start
+    // basis; later steps refine it.  Measured on single BTRG chi = 32 runs
+    // (beta 0.44): 1e-4 takes 3 % less time than 1e-10, 1e-6 .. 1e-2 2 %.
+    double jacobi_tol_first = 1e-4;
     int jacobi_sweeps_iter = 40;
     double null_rtol = 1e-14;    // columns below this x sigma_max are null
     double conv_target = 1e-14;  // accumulated contraction of the top-k subspace angle
