@@ -17,7 +17,7 @@ namespace tnr {
 
 // "key=value,..." overrides of the defaults (keys: tol_iter, tol_first, sweeps_iter,
 // sweeps, block_mult, block_extra, conv_target, max_iters, cholqr_y,
-// cholqr_z); unknown keys are an argument error.  TNR_SOLVER_CFG in the
+// cholqr_z, cholqr_eigh); unknown keys are an argument error.  TNR_SOLVER_CFG in the
 // environment is applied first, then tnr_solver_config's string.
 static void apply_cfg(SolverCfg& c, const char* e) {
     if (!e) return;
@@ -43,6 +43,7 @@ static void apply_cfg(SolverCfg& c, const char* e) {
             else if (k == "max_iters") c.max_iters = (int)v;
             else if (k == "cholqr_y") c.cholqr_y = v != 0.0;
             else if (k == "cholqr_z") c.cholqr_z = v != 0.0;
+            else if (k == "cholqr_eigh") c.cholqr_eigh = v != 0.0;
             else TNR_REQUIRE(false, TNR_ERR_ARG, "unknown solver option '" + k + "'");
         }
         i = j + 1;
@@ -1027,10 +1028,19 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             jMV.push_back(mv);
             rY.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip, failv.d() + iter[i]});
-            conv.push_back(ConvDesc{sv, l, s.kmax, 0, cs.it(i), cs.prod(i), cs.done(i)});
-            // one tight pass on the loop's last rotated Y (in Y2), output into Y
-            rV.push_back(RR{x.Y2.d(), s.n, l, s.n, x.Y.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
-                            x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr, failv.d() + iter[i]});
+            // Ritz values, or with Cholesky QR the pivots R_jj
+            conv.push_back(ConvDesc{cfg.cholqr_eigh ? x.lam.d() : sv, l, s.kmax, 0, cs.it(i),
+                                    cs.prod(i), cs.done(i)});
+            // one tight pass on the loop's last rotated Y (in Y2), output into
+            // Y; after Cholesky QR steps Y2 is no Ritz basis: two passes on Y
+            if (cfg.cholqr_eigh)
+                rV.push_back(RR{x.Y.d(), s.n, l, s.n, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(),
+                                x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr,
+                                failv.d() + iter[i]});
+            else
+                rV.push_back(RR{x.Y2.d(), s.n, l, s.n, x.Y.d(), x.G.d(), x.W.d(), x.W2.d(),
+                                x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr,
+                                failv.d() + iter[i]});
             jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.n, l, s.n, s.ld, s.n, s.n, 0, 0});
             jH.push_back(GemmJob{x.V.d(), x.Y.d(), x.G.d(), l, l, s.n, s.n, s.n, l, 1, 0});
             nmax = std::max(nmax, s.n);
@@ -1040,7 +1050,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         randfill(ctx, fills);
         // (random start block used as is, see svd_grouped)
         GemmPlan pMV = gemm_plan(ctx, jMV);
-        RRPlan pY = rr_plan(ctx, rY, 1);
+        RRPlan pY = rr_plan(ctx, rY, 1, cfg.cholqr_eigh);
         DescArena a;
         gemm_stage(pMV, a);
         rr_stage(pY, a);
@@ -1057,8 +1067,9 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         });
         // V at full orthonormality from the loop's last rotated Y = M V W
         // (the loop uses a looser Jacobi threshold; one pass at the final
-        // threshold, as in svd_grouped)
-        rr_grouped(ctx, rV, 1, true);
+        // threshold, as in svd_grouped), or from the last Y in two passes
+        if (cfg.cholqr_eigh) rr_grouped(ctx, rV, 2);
+        else rr_grouped(ctx, rV, 1, true);
         // Ritz pairs: H = V^T (M V), eigendecompose, U = V W
         gemm_grouped(ctx, jF);
         gemm_grouped(ctx, jH);

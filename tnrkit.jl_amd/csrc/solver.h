@@ -80,8 +80,19 @@ struct SolverCfg {
     double null_rtol = 1e-14;    // columns below this x sigma_max are null
     double conv_target = 1e-14;  // accumulated contraction of the top-k subspace angle
     double block_mult = 2.0;     // block width max(block_mult k, k + block_extra) <= LMAX
-    bool cholqr_y = true;        // block iteration: Y side by pivoted Cholesky QR
-    bool cholqr_z = false;       // ... and Z side (stopping rule on the pivots R_jj)
+    // Block iteration orthonormalisation by pivoted Cholesky QR instead of
+    // Rayleigh-Ritz: the Y side (cholqr_y), the Z side (cholqr_z; the
+    // stopping rule then reads the pivots R_jj) and the eigensolver's loop
+    // (cholqr_eigh).  Without Rayleigh-Ritz in the loop the iteration is a
+    // simultaneous iteration: its basis vectors converge to the singular
+    // vectors one by one, so the Gram matrices of the final extraction are
+    // nearly diagonal and cheap for Jacobi.  Measured on single BTRG
+    // chi = 32 runs: 17 % less time with the Z side on Cholesky QR.  The
+    // eigensolver keeps Rayleigh-Ritz: with Cholesky QR its stopping rule on
+    // the pivots ended HOTRG chi = 48 solves early (1e-7 norm errors).
+    bool cholqr_y = true;
+    bool cholqr_z = true;
+    bool cholqr_eigh = false;
     int block_extra = 2;
     int min_iters = 2;
     int max_iters = 80;
