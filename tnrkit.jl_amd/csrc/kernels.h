@@ -212,7 +212,7 @@ void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmin, 
                    double null_rel2);
 void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, double tol,
                    int max_sweeps);
-void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, double rtol);
+void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, int ncol_max, double rtol);
 void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax);
 void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rtol, double target,
                  int min_iters, int max_iters);

@@ -1242,58 +1242,68 @@ __global__ __launch_bounds__(CQ64) __attribute__((amdgpu_waves_per_eu(1, 1))) vo
 
 // ------------------------------------------------- column norms / normalise
 // sigma_j = ||Y[:, j]||; U[:, j] = Y[:, j] / sigma_j, or 0 when
-// sigma_j <= rtol * max_j sigma_j (numerical null direction).  One
-// workgroup per problem; wave w owns columns j = w (mod 16).  A lane keeps
-// four partial sums over rows 256 apart (four loads in flight), adds them
-// in a fixed order and the wave reduces with a fixed butterfly, so the
-// result does not depend on scheduling.
+// sigma_j <= rtol * max_j sigma_j (numerical null direction).  Two launches
+// over (problem, group of CN_COLS columns) workgroups, one wave per column,
+// so even one problem spreads over many CUs: k_colnorm_part writes the
+// raw norms, k_colnorm_scale takes the largest of them and normalises.  A
+// lane keeps four partial sums over rows 256 apart (four loads in flight),
+// adds them in a fixed order and the wave reduces with a fixed butterfly,
+// so the result does not depend on scheduling.
+constexpr int CN_COLS = 4;
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
-__global__ __launch_bounds__(1024) void k_colnorm(const ColDesc* __restrict__ descs, int nd,
-                                                  double rtol) {
-    __shared__ double wmax[16];
+__global__ __launch_bounds__(64 * CN_COLS) void k_colnorm_part(const ColDesc* __restrict__ descs,
+                                                              int nd) {
     const ColDesc D = descs[blockIdx.x];
-    if (D.skip && *D.skip) return;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    for (int j = w; j < D.ncol; j += 16) {
-        const double* y = D.Y + (int64_t)j * D.ldy;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int i = lane;
-        for (; i + 192 < D.nrow; i += 256) {
-            const double y0 = y[i], y1 = y[i + 64], y2 = y[i + 128], y3 = y[i + 192];
-            a0 = fma(y0, y0, a0); a1 = fma(y1, y1, a1); a2 = fma(y2, y2, a2); a3 = fma(y3, y3, a3);
-        }
-        for (; i < D.nrow; i += 64) a0 = fma(y[i], y[i], a0);
-        double acc = wave_sum((a0 + a1) + (a2 + a3));
-        if (lane == 0) D.sigma[j] = sqrt(acc);   // raw norms first (any column count)
+    const int lane = threadIdx.x & 63, j = blockIdx.y * CN_COLS + (threadIdx.x >> 6);
+    if (j >= D.ncol || (D.skip && *D.skip)) return;
+    const double* y = D.Y + (int64_t)j * D.ldy;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = lane;
+    for (; i + 192 < D.nrow; i += 256) {
+        const double y0 = y[i], y1 = y[i + 64], y2 = y[i + 128], y3 = y[i + 192];
+        a0 = fma(y0, y0, a0); a1 = fma(y1, y1, a1); a2 = fma(y2, y2, a2); a3 = fma(y3, y3, a3);
     }
-    __syncthreads();
+    for (; i < D.nrow; i += 64) a0 = fma(y[i], y[i], a0);
+    const double acc = wave_sum((a0 + a1) + (a2 + a3));
+    if (lane == 0) D.sigma[j] = sqrt(acc);   // raw norm
+}
+
+// Every workgroup of a problem reads all its raw norms for the largest one;
+// a column zeroed as null is never the largest, so the workgroups that
+// zero theirs do not change what the others read.
+__global__ __launch_bounds__(64 * CN_COLS) void k_colnorm_scale(const ColDesc* __restrict__ descs,
+                                                               int nd, double rtol) {
+    __shared__ double wmax[CN_COLS];
+    const ColDesc D = descs[blockIdx.x];
+    if (blockIdx.y * CN_COLS >= D.ncol || (D.skip && *D.skip)) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, j = blockIdx.y * CN_COLS + w;
     double smax = 0.0;
-    for (int j = t; j < D.ncol; j += 1024) smax = fmax(smax, D.sigma[j]);
+    for (int c = t; c < D.ncol; c += 64 * CN_COLS) smax = fmax(smax, D.sigma[c]);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) smax = fmax(smax, __shfl_xor(smax, o, 64));
     if (lane == 0) wmax[w] = smax;
     __syncthreads();
     smax = wmax[0];
-    for (int q = 1; q < 16; ++q) smax = fmax(smax, wmax[q]);
-    for (int j = w; j < D.ncol; j += 16) {
-        const double sj = D.sigma[j];
-        const bool live = sj > rtol * smax && sj > 0.0;
-        const double inv = live ? 1.0 / sj : 0.0;
-        const double* y = D.Y + (int64_t)j * D.ldy;
-        double* u = D.U + (int64_t)j * D.ldu;
-        int i = lane;
-        for (; i + 192 < D.nrow; i += 256) {
-            const double y0 = y[i], y1 = y[i + 64], y2 = y[i + 128], y3 = y[i + 192];
-            u[i] = y0 * inv; u[i + 64] = y1 * inv; u[i + 128] = y2 * inv; u[i + 192] = y3 * inv;
-        }
-        for (; i < D.nrow; i += 64) u[i] = y[i] * inv;
-        if (lane == 0) D.sigma[j] = live ? sj : 0.0;
+    for (int q = 1; q < CN_COLS; ++q) smax = fmax(smax, wmax[q]);
+    if (j >= D.ncol) return;
+    const double sj = D.sigma[j];
+    const bool live = sj > rtol * smax && sj > 0.0;
+    const double inv = live ? 1.0 / sj : 0.0;
+    const double* y = D.Y + (int64_t)j * D.ldy;
+    double* u = D.U + (int64_t)j * D.ldu;
+    int i = lane;
+    for (; i + 192 < D.nrow; i += 256) {
+        const double y0 = y[i], y1 = y[i + 64], y2 = y[i + 128], y3 = y[i + 192];
+        u[i] = y0 * inv; u[i + 64] = y1 * inv; u[i + 128] = y2 * inv; u[i + 192] = y3 * inv;
     }
+    for (; i < D.nrow; i += 64) u[i] = y[i] * inv;
+    if (lane == 0 && !live) D.sigma[j] = 0.0;
 }
 
 // ------------------------------------------------ pivoted-row SVD gauge
@@ -2177,10 +2187,14 @@ void launch_lincomb(hipStream_t st, const LinDesc* d_descs, int nd, int64_t tota
     TNR_HIP(hipGetLastError());
 }
 
-void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, double rtol) {
-    if (nd <= 0) return;
+void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, int ncol_max, double rtol) {
+    if (nd <= 0 || ncol_max <= 0) return;
+    const dim3 grid(nd, (ncol_max + CN_COLS - 1) / CN_COLS);
     ProfScope ps(st, PK_COLNORM, 0.0, 0.0);
-    TNR_LAUNCH(ps, k_colnorm, dim3(nd), dim3(1024), 0, st, d_descs, nd, rtol);
+    ps.bracket();
+    TNR_LAUNCH(ps, k_colnorm_part, grid, dim3(64 * CN_COLS), 0, st, d_descs, nd);
+    TNR_HIP(hipGetLastError());
+    TNR_LAUNCH(ps, k_colnorm_scale, grid, dim3(64 * CN_COLS), 0, st, d_descs, nd, rtol);
     TNR_HIP(hipGetLastError());
 }
 

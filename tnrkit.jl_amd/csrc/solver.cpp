@@ -193,7 +193,7 @@ void rr_issue(Ctx& ctx, RRPlan& P, bool first = false) {
                           in_loop ? cfg.jacobi_sweeps_iter : cfg.jacobi_sweeps);
         gemm_issue(ctx, P.rot[pass]);
     }
-    launch_colnorm(ctx.stream, P.d_col, (int)P.col.size(), cfg.null_rtol);
+    launch_colnorm(ctx.stream, P.d_col, (int)P.col.size(), P.lmax, cfg.null_rtol);
 }
 
 void rr_grouped(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool tight = false) {
@@ -465,8 +465,10 @@ std::vector<int> wide_rr(Ctx& ctx, const std::vector<WideRR>& jobs, double tol, 
         sg[j] = Buf(&ctx.pool, (size_t)R.l * 8);
         cd.push_back(ColDesc{R.Y, Yn[j].d(), sg[j].d(), R.rows, R.l, R.ldy, R.rows});
     }
+    int lcol = 0;
+    for (const ColDesc& c : cd) lcol = std::max(lcol, (int)c.ncol);
     ColDesc* dcd = ctx.stage->put(ctx.stream, cd);
-    launch_colnorm(ctx.stream, dcd, (int)cd.size(), cfg.null_rtol);
+    launch_colnorm(ctx.stream, dcd, (int)cd.size(), lcol, cfg.null_rtol);
     std::vector<const double*> src;
     std::vector<int> cnt;
     std::vector<std::vector<double>> hs(J);
