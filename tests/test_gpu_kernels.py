@@ -178,6 +178,51 @@ def test_eigh_truncated(tk, n, k):
     assert np.max(np.abs(Pk - Pr)) < 1e-9
 
 
+# A spectrum that decays too slowly for the block iteration (the top-k
+# subspace contracts by sigma_l / sigma_k ~ 0.97 per iteration, so 80
+# iterations leave it far from converged): such blocks, up to 384 on their
+# smaller side, are factorised whole instead, and match LAPACK as usual
+# (the reference's LAPACK-backed svd_trunc / eigh_trunc!, btrg.jl:61 and
+# hotrg.jl:103, have no iteration to stall).
+def _flat(rng, m, n, lo):
+    U, _ = np.linalg.qr(rng.standard_normal((m, min(m, n))))
+    V, _ = np.linalg.qr(rng.standard_normal((n, min(m, n))))
+    s = np.linspace(1.0, lo, min(m, n))
+    return (U * s) @ V.T, s
+
+
+@pytest.mark.parametrize("m,n,k", [(200, 150, 10), (300, 384, 16)])
+def test_svd_stalled_iteration_is_factorised_whole(tk, m, n, k):
+    rng = np.random.default_rng(m * n + k)
+    M, _ = _flat(rng, m, n, 0.3)
+    U, s, V, it = svd(tk, M, k)
+    Ur, sr, Vtr = np.linalg.svd(M, full_matrices=False)
+    assert np.max(np.abs(s - sr[:k])) < 1e-13 * sr[0]
+    Mk = (U * s) @ V.T
+    Mr = (Ur[:, :k] * sr[:k]) @ Vtr[:k]
+    assert np.max(np.abs(Mk - Mr)) < 1e-11 * sr[0]
+    assert np.max(np.abs(U.T @ U - np.eye(k))) < 1e-12
+    assert np.max(np.abs(V.T @ V - np.eye(k))) < 1e-12
+
+
+@pytest.mark.parametrize("n,k", [(144, 12), (300, 20)])
+def test_eigh_stalled_iteration_is_factorised_whole(tk, n, k):
+    rng = np.random.default_rng(n + k)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    w0 = np.linspace(1.0, 0.2, n) * np.where(np.arange(n) % 3 == 2, -1.0, 1.0)
+    M = (Q * w0) @ Q.T
+    Mf = np.asfortranarray(M)
+    U = np.zeros((n, k), order="F"); lam = np.zeros(k); it = C.c_int()
+    st = tk.lib().tnr_kernel_eigh(n, p(Mf), k, p(U), p(lam), C.byref(it))
+    assert st == 0, tk.lib().tnr_last_error()
+    w, v = np.linalg.eigh(M)
+    o = np.argsort(-np.abs(w), kind="stable"); w, v = w[o], v[:, o]
+    assert np.max(np.abs(lam - w[:k])) < 1e-13
+    Pk = U @ U.T
+    Pr = v[:, :k] @ v[:, :k].T
+    assert np.max(np.abs(Pk - Pr)) < 1e-9
+
+
 def _z2_factor(rng, m, k, s):
     """Left singular vectors of a matrix whose rows split into even (first
     half) and odd coordinates, every value of s shared by one even and one

@@ -248,7 +248,7 @@ static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t t
         for (int j = 0; j < d.nd; ++j) cnt *= d.dim[j];
         const bool weighted = d.wleg[0] >= 0 || d.wleg[1] >= 0;
         if (!weighted && s0 >= 0 && d0 >= 0 && s0 != d0 && d.dim[s0] >= 8 && d.dim[d0] >= 8 &&
-            d.nd - 2 <= 4) {
+            d.nd - 2 <= TR_BATCH) {
             TransDesc t;
             std::memset(&t, 0, sizeof(t));
             t.src = d.src;
@@ -267,7 +267,7 @@ static void issue_copies(Ctx& ctx, const std::vector<CopyDesc>& descs, int64_t t
                 nb *= d.dim[j];
                 ++k;
             }
-            for (; k < 4; ++k) t.bdim[k] = 1;
+            for (; k < TR_BATCH; ++k) t.bdim[k] = 1;
             t.nb = (int32_t)nb;
             auto pow2 = [](int v) { int p = 1; while (p < v && p < 64) p <<= 1; return p; };
             t.tx = pow2(t.n0);

@@ -22,13 +22,15 @@ struct CopyDesc {
 // Tiled transpose job: a sub-block whose source-contiguous dim (extent n0,
 // source stride 1, destination stride ds0) differs from its
 // destination-contiguous dim (extent n1, destination stride 1, source
-// stride ss1); up to 4 further batch dims.  Handled in 64 x 64 LDS tiles.
+// stride ss1); up to TR_BATCH further batch dims.  Handled in 64 x 64 LDS
+// tiles.
+constexpr int TR_BATCH = MAXLEG - 2;
 struct TransDesc {
     const double* src;
     double* dst;
     int64_t ds0, ss1;
-    int64_t bs[4], bd[4];         // batch strides (src, dst)
-    int32_t bdim[4];
+    int64_t bs[TR_BATCH], bd[TR_BATCH];   // batch strides (src, dst)
+    int32_t bdim[TR_BATCH];
     int32_t n0, n1, nb;           // nb = product of batch extents
     int32_t tx, ty, sub;          // sub-tile extents (powers of two <= 64), sub-tiles per tile
     int32_t t0, t1;               // tiles along n0, n1

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_transpose(const TransDesc* __restrict__
                 int q = bi;
                 so = 0;
                 dof = 0;
-                for (int kk = 0; kk < 4; ++kk) {
+                for (int kk = 0; kk < TR_BATCH; ++kk) {
                     const int ik = q % D.bdim[kk];
                     q /= D.bdim[kk];
                     so += (int64_t)ik * D.bs[kk];

@@ -300,6 +300,8 @@ void iterate(Ctx& ctx, const ConvState& cs, F&& issue_one) {
 // iteration): wide blocks only occur at large chi, where the arithmetic
 // dwarfs the synchronisation.
 constexpr int WIDE_B = LMAX / 2;
+// Largest side factorised whole when a block iteration stalls.
+constexpr int kWholeMax = 4 * LMAX;
 
 struct WideRR {
     double* Y;           // rows x l, ld ldy: rotated in place
@@ -552,8 +554,10 @@ bool wide_converged(const std::vector<double>& sv, int kmax, bool squared, bool 
     return it >= cfg.min_iters && prod < cfg.conv_target;
 }
 
+// `whole`: the block spans the whole smaller side (an exact factorisation;
+// the retry of a block iteration that stalled)
 void svd_wide(Ctx& ctx, const std::vector<SvdIn>& in, const std::vector<int>& idx,
-              std::vector<SvdOut>& out) {
+              std::vector<SvdOut>& out, bool whole = false) {
     const SolverCfg& cfg = solver_cfg();
     struct WS { Buf V, Y, Z, U, sY, sZ, Wt; int l; double prod = 1.0; bool done = false; };
     std::vector<WS> w(idx.size());
@@ -561,7 +565,7 @@ void svd_wide(Ctx& ctx, const std::vector<SvdIn>& in, const std::vector<int>& id
     for (size_t i = 0; i < idx.size(); ++i) {
         const SvdIn& s = in[idx[i]];
         WS& x = w[i];
-        x.l = wide_block_size(s.kmax, std::min(s.m, s.n));
+        x.l = whole ? std::min(s.m, s.n) : wide_block_size(s.kmax, std::min(s.m, s.n));
         x.V = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
         x.Z = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
         x.Y = Buf(&ctx.pool, (size_t)s.m * x.l * 8);
@@ -648,8 +652,10 @@ void svd_wide(Ctx& ctx, const std::vector<SvdIn>& in, const std::vector<int>& id
     }
 }
 
+// `whole`: the block spans the whole matrix (all n eigenpairs, exact; the
+// retry of a block iteration that stalled)
 void eigh_wide(Ctx& ctx, const std::vector<EighIn>& in, const std::vector<int>& idx,
-               std::vector<EighOut>& out) {
+               std::vector<EighOut>& out, bool whole = false) {
     const SolverCfg& cfg = solver_cfg();
     struct WS { Buf V, Y, sY, H, Uh, sH, Wt; int l; double prod = 1.0; bool done = false; };
     std::vector<WS> w(idx.size());
@@ -659,7 +665,7 @@ void eigh_wide(Ctx& ctx, const std::vector<EighIn>& in, const std::vector<int>& 
     for (size_t i = 0; i < idx.size(); ++i) {
         const EighIn& s = in[idx[i]];
         WS& x = w[i];
-        x.l = wide_block_size(s.kmax, s.n);
+        x.l = whole ? s.n : wide_block_size(s.kmax, s.n);
         x.V = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
         x.Y = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
         x.sY = Buf(&ctx.pool, (size_t)x.l * 8);
@@ -759,7 +765,7 @@ void eigh_wide(Ctx& ctx, const std::vector<EighIn>& in, const std::vector<int>& 
         TNR_HIP(hipMemcpyAsync(o.lam.d(), t, (size_t)x.l * 8, hipMemcpyDeviceToDevice, ctx.stream));
         o.U = Buf(&ctx.pool, (size_t)s.n * x.l * 8);
         o.ncols = x.l;
-        o.direct = false;
+        o.direct = whole;     // all n eigenvalues: the truncation error is exact
         vu.push_back(GemmJob{x.V.d(), x.Wt.d(), o.U.d(), s.n, x.l, x.l, s.n, x.l, s.n, 0, 0});
     }
     gemm_grouped(ctx, vu);
@@ -773,6 +779,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
     out.resize(in.size());
     const SolverCfg& cfg = solver_cfg();
     Buf failv = fail_flags(ctx, in.size());
+    std::vector<char> retried(in.size(), 0);   // refactorised whole after a stall
     std::vector<int> direct, iter, wide;
     for (size_t p = 0; p < in.size(); ++p) {
         const SvdIn& s = in[p];
@@ -932,6 +939,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
         std::vector<double> prods(P);
         ctx.fetch(its.data(), cs.it(0), P * sizeof(int));
         ctx.fetch(prods.data(), cs.prod(0), P * sizeof(double));
+        std::vector<int> stalled;
         for (int i = 0; i < P; ++i) {
             SvdOut& o = out[iter[i]];
             IW& x = w[i];
@@ -940,7 +948,20 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             o.s = std::move(x.sigU);
             o.iters = its[i];
             o.conv = prods[i];
-            if (!(prods[i] < cfg.conv_target)) o.status = TNR_ERR_NOCONV;   // stopped at max_iters
+            if (!(prods[i] < cfg.conv_target)) {   // stopped at max_iters
+                const SvdIn& si = in[iter[i]];
+                if (std::min(si.m, si.n) <= kWholeMax) stalled.push_back(iter[i]);
+                else o.status = TNR_ERR_NOCONV;
+            }
+        }
+        // a slowly decaying spectrum (sigma_l / sigma_k near 1) can stall the
+        // iteration: such problems, when small enough, are factorised whole
+        if (!stalled.empty()) {
+            for (int p : stalled) {
+                out[p] = SvdOut();
+                retried[p] = 1;
+            }
+            svd_wide(ctx, in, stalled, out, true);
         }
     }
     // singular values and the non-convergence flags to the host: packed on
@@ -959,7 +980,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
     dst[in.size()] = &failh;
     readback_vectors(ctx, src, cnt, dst);
     for (size_t p = 0; p < in.size(); ++p)
-        out[p].status = problem_status(out[p].s_host, failh[p], out[p].status);
+        out[p].status = problem_status(out[p].s_host, retried[p] ? 0.0 : failh[p], out[p].status);
 }
 
 // ---------------------------------------------------------------- eigh
@@ -968,6 +989,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
     out.resize(in.size());
     const SolverCfg& cfg = solver_cfg();
     Buf failv = fail_flags(ctx, in.size());
+    std::vector<char> retried(in.size(), 0);   // refactorised whole after a stall
     std::vector<int> direct, iter, wide;
     for (size_t p = 0; p < in.size(); ++p) {
         const EighIn& s = in[p];
@@ -1102,10 +1124,23 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
         std::vector<double> prods(P);
         ctx.fetch(its.data(), cs.it(0), P * sizeof(int));
         ctx.fetch(prods.data(), cs.prod(0), P * sizeof(double));
+        std::vector<int> stalled;
         for (int i = 0; i < P; ++i) {
             out[iter[i]].iters = its[i];
             out[iter[i]].conv = prods[i];
-            if (!(prods[i] < cfg.conv_target)) out[iter[i]].status = TNR_ERR_NOCONV;   // at max_iters
+            if (!(prods[i] < cfg.conv_target)) {   // at max_iters
+                if (in[iter[i]].n <= kWholeMax) stalled.push_back(iter[i]);
+                else out[iter[i]].status = TNR_ERR_NOCONV;
+            }
+        }
+        // a stalled iteration (|lambda_l / lambda_k| near 1) of a small enough
+        // matrix: the whole eigendecomposition instead
+        if (!stalled.empty()) {
+            for (int p : stalled) {
+                out[p] = EighOut();
+                retried[p] = 1;
+            }
+            eigh_wide(ctx, in, stalled, out, true);
         }
     }
     std::vector<const double*> lsrc(in.size() + 1);
@@ -1122,7 +1157,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
     ldst[in.size()] = &failh;
     readback_vectors(ctx, lsrc, lcnt, ldst);
     for (size_t p = 0; p < in.size(); ++p)
-        out[p].status = problem_status(out[p].lam_host, failh[p], out[p].status);
+        out[p].status = problem_status(out[p].lam_host, retried[p] ? 0.0 : failh[p], out[p].status);
 }
 
 }  // namespace tnr
