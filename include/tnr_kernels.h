@@ -112,7 +112,8 @@ int tnr_autotune_enable(int on);
  * threshold, 1e-10), tol_first (the first iteration's, 1e-4),
  * conv_target (1e-14), block_mult (2), block_extra (2),
  * cholqr_y (1), cholqr_z (1), cholqr_eigh (0) (block iteration steps by
- * pivoted Cholesky QR instead of Rayleigh-Ritz).  A solve that reaches max_iters or a sweep cap
+ * pivoted Cholesky QR instead of Rayleigh-Ritz), rv_passes (2), rf_passes (1)
+ * (Jacobi passes of the final extraction).  A solve that reaches max_iters or a sweep cap
  * unconverged fails its step with TNR_ERR_NOCONV.  Not thread-safe: call it
  * while no other entry point runs. */
 int tnr_solver_config(const char* spec);

@@ -17,7 +17,8 @@ namespace tnr {
 
 // "key=value,..." overrides of the defaults (keys: tol_iter, tol_first, sweeps_iter,
 // sweeps, block_mult, block_extra, conv_target, max_iters, cholqr_y,
-// cholqr_z, cholqr_eigh); unknown keys are an argument error.  TNR_SOLVER_CFG in the
+// cholqr_z, cholqr_eigh, rv_passes, rf_passes); unknown keys are an
+// argument error.  TNR_SOLVER_CFG in the
 // environment is applied first, then tnr_solver_config's string.
 static void apply_cfg(SolverCfg& c, const char* e) {
     if (!e) return;
@@ -44,6 +45,8 @@ static void apply_cfg(SolverCfg& c, const char* e) {
             else if (k == "cholqr_y") c.cholqr_y = v != 0.0;
             else if (k == "cholqr_z") c.cholqr_z = v != 0.0;
             else if (k == "cholqr_eigh") c.cholqr_eigh = v != 0.0;
+            else if (k == "rv_passes") c.rv_passes = (int)v;
+            else if (k == "rf_passes") c.rf_passes = (int)v;
             else TNR_REQUIRE(false, TNR_ERR_ARG, "unknown solver option '" + k + "'");
         }
         i = j + 1;
@@ -69,7 +72,8 @@ void set_solver_cfg(const char* spec) {
     SolverCfg c = env_cfg();
     apply_cfg(c, spec);
     TNR_REQUIRE(c.max_iters >= 1 && c.jacobi_sweeps >= 0 && c.jacobi_sweeps_iter >= 0 &&
-                    c.block_mult >= 1.0 && c.block_extra >= 0,
+                    c.block_mult >= 1.0 && c.block_extra >= 0 && c.rv_passes >= 1 &&
+                    c.rv_passes <= 2 && c.rf_passes >= 1 && c.rf_passes <= 2,
                 TNR_ERR_ARG, "solver option out of range");
     solver_cfg() = c;
 }
@@ -78,7 +82,7 @@ namespace {
 
 // One Gram/Jacobi Rayleigh-Ritz job: rotates the columns of Y (rows x l) to
 // the Ritz basis of span(Y), normalises them into U and writes the column
-// norms to sigma.  Wtot (two-pass jobs only) receives the total rotation.
+// norms to sigma.  Wtot (optional) receives the total rotation.
 struct RR {
     double* Y;
     int rows, l, ldy;
@@ -123,7 +127,7 @@ RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool chol = fa
     P.tight = tight;
     P.lmin = jobs.empty() ? 0 : jobs[0].l;
     for (auto& j : jobs) {
-        TNR_REQUIRE(passes == 2 || !j.Wtot, TNR_ERR_INTERNAL, "total rotation needs two passes");
+        TNR_REQUIRE(!chol || !j.Wtot, TNR_ERR_INTERNAL, "no total rotation with Cholesky QR");
         P.lmax = std::max(P.lmax, j.l);
         P.lmin = std::min(P.lmin, j.l);
     }
@@ -132,11 +136,13 @@ RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool chol = fa
         for (auto& j : jobs) {
             double* Y = pass == 0 ? j.Y : j.Y2;
             const int ld = pass == 0 ? j.ldy : j.rows;
+            // one pass: its rotation is the total one
+            double* W1 = passes == 1 && j.Wtot ? j.Wtot : j.W;
             GemmJob g{Y, Y, j.G, j.l, j.l, j.rows, ld, ld, j.l, 1, 0};   // G = Y^T Y
             g.skip = j.skip;
             gj.push_back(g);
             GemmJob r = pass == 0
-                            ? GemmJob{j.Y, j.W, j.Y2, j.rows, j.l, j.l, j.ldy, j.l, j.rows, 0, 0}
+                            ? GemmJob{j.Y, W1, j.Y2, j.rows, j.l, j.l, j.ldy, j.l, j.rows, 0, 0}
                             : GemmJob{j.Y2, j.W2, j.Y, j.rows, j.l, j.l, j.rows, j.l, j.ldy, 0, 0};
             r.skip = j.skip;
             rj.push_back(r);
@@ -145,7 +151,7 @@ RRPlan rr_plan(Ctx& ctx, const std::vector<RR>& jobs, int passes, bool chol = fa
                 w.skip = j.skip;
                 rj.push_back(w);
             }
-            P.jac[pass].push_back(JacobiDesc{j.G, pass == 0 ? j.W : j.W2, j.lam, j.sweeps, j.l, j.l,
+            P.jac[pass].push_back(JacobiDesc{j.G, pass == 0 ? W1 : j.W2, j.lam, j.sweeps, j.l, j.l,
                                              j.l, j.skip, j.fail});
         }
         P.gram[pass] = gemm_plan(ctx, gj);
@@ -930,10 +936,10 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
         // V^T V = I.  Its last rotated Z has columns orthogonal to that
         // threshold, so one pass at the final threshold does what the second
         // pass of a two-pass step does.
-        if (cfg.cholqr_z) rr_grouped(ctx, rV, 2);
+        if (cfg.cholqr_z) rr_grouped(ctx, rV, cfg.rv_passes, cfg.rv_passes == 1);
         else rr_grouped(ctx, rV, 1, true);
         gemm_grouped(ctx, jF);
-        rr_grouped(ctx, rF, 2);
+        rr_grouped(ctx, rF, cfg.rf_passes, cfg.rf_passes == 1);
         gemm_grouped(ctx, jV);
         std::vector<int> its(P);
         std::vector<double> prods(P);

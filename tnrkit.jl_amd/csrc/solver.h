@@ -93,6 +93,15 @@ struct SolverCfg {
     bool cholqr_y = true;
     bool cholqr_z = true;
     bool cholqr_eigh = false;
+    // Rayleigh-Ritz passes that rebuild V from the last Z (with Cholesky QR
+    // on the Z side) and that extract the Ritz triplets from Y = M V.  V is
+    // a Ritz basis of the converged subspace, so the columns of M V are
+    // already orthogonal to roundoff relative to their own norms: one pass
+    // has the relative accuracy a second pass gives (measured: the parity
+    // and LAPACK tests unchanged, single BTRG chi = 32 runs 4 % faster).  The
+    // last Z is not orthogonal enough for one pass (1 fails parity).
+    int rv_passes = 2;
+    int rf_passes = 1;
     int block_extra = 2;
     int min_iters = 2;
     int max_iters = 80;
