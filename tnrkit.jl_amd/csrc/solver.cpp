@@ -897,7 +897,10 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             else
                 rV.push_back(RR{x.Y2.d(), s.n, l, s.n, x.Z.d(), x.G.d(), x.W.d(), x.W2.d(),
                                 x.lam.d(), x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, nullptr, fl});
-            // Ritz extraction from Y = M V (two passes), V <- V Wtot
+            // Ritz extraction from Y = M V over the whole basis (the top
+            // triplets recombine all l columns: extracting them from the
+            // leading columns only moves them by the subspace's residual
+            // angle, ~1e-7), V <- V Wtot
             jF.push_back(GemmJob{s.M, x.V.d(), x.Y.d(), s.m, l, s.n, s.ld, s.n, s.m, 0, 0});
             rF.push_back(RR{x.Y.d(), s.m, l, s.m, x.Y2.d(), x.G.d(), x.W.d(), x.W2.d(), x.lam.d(),
                             x.sw.as<int>(), x.Wt.d(), x.U.d(), s.m, x.sigU.d(), nullptr, fl});
