@@ -614,9 +614,19 @@ __global__ __launch_bounds__(256) void k_normalize(const TraceProb* __restrict__
     double s = fabs(block_sum_fixed(v, red));
     if (blockIdx.x == 0 && threadIdx.x == 0) *P.norm_slot = s;
     const double inv = 1.0 / (P.divide_by ? *P.divide_by : s);
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < P.size;
-         e += (int64_t)gridDim.x * blockDim.x)
-        P.T[e] *= inv;
+    // four elements a grid-stride apart per pass, loaded before they are
+    // stored, so each thread keeps four loads in flight
+    gmut_double* T = as_global(P.T);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; e + 3 * stride < P.size; e += 4 * stride) {
+        const double x0 = T[e], x1 = T[e + stride], x2 = T[e + 2 * stride], x3 = T[e + 3 * stride];
+        T[e] = x0 * inv;
+        T[e + stride] = x1 * inv;
+        T[e + 2 * stride] = x2 * inv;
+        T[e + 3 * stride] = x3 * inv;
+    }
+    for (; e < P.size; e += stride) T[e] *= inv;
 }
 
 // ---------------------------------------------------- linear combinations
