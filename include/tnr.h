@@ -119,7 +119,7 @@ int tnr_get_tensor(tnr_scheme* s, double* data, size_t ndata);
  * Each works on the scheme's current T (BTRG: T with S1, S2 absorbed as
  * the reference does); the transfer matrix of `unitcell` (1 or 2) tensors
  * in a row needs equal left/right and top/bottom spaces, and every
- * coupled-charge block of it at most 128 wide. */
+ * coupled-charge block of it at most 512 wide (TNR_ERR_ARG above). */
 /* All eigenvalues of the transfer matrix (eig_full in cft.jl:16, 50 and
  * 296, 326): re/im hold `cap` values; *count receives the number written
  * (or needed).  Order: decreasing modulus, ties by decreasing real part,

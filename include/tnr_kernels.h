@@ -49,7 +49,7 @@ int tnr_kernel_svd(int m, int n, const double* M, int k, double* U,
 int tnr_kernel_pivoted_gauge(int m, int k, const double* U, const double* s, double* Q);
 
 /* All eigenvalues of `count` real n x n matrices A (column-major, ld = n,
- * stored one after another; n <= 128): wr/wi (count x n) receive real and
+ * stored one after another; n <= 512): wr/wi (count x n) receive real and
  * imaginary parts, conjugate pairs adjacent, in the solver's deflation order.
  * Balancing, Householder reduction to Hessenberg form and Francis
  * double-shift QR, one wavefront per matrix; the engine of cft_data and

@@ -32,7 +32,9 @@ def kernel_eig(tk, As):
     return (wr + 1j * wi).reshape(cnt, n)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 96, 128])
+# above 128 the matrix lives in global memory (the same solver); the
+# reference's eig_full (cft.jl:16) has no size limit
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 96, 128, 129, 160, 300])
 def test_eig_kernel_vs_lapack(tk, n):
     rng = np.random.default_rng(n)
     As = [rng.standard_normal((n, n))]
@@ -131,6 +133,18 @@ def test_unitcell_two_spectrum_vs_oracle(tk):
     ev_g, ev_o = tk.transfer_spectrum(g, 2), ob.transfer_spectrum(o, 2)
     assert same_multiset(_big(ev_g), _big(ev_o), 1e-8)
     assert abs(tk.ground_state_degeneracy(g, 2) / ob.ground_state_degeneracy(o, 2) - 1) < 1e-8
+    g.close()
+
+
+def test_unitcell_two_spectrum_above_128_vs_oracle(tk):
+    """Trivial TRG at chi = 12: the two-site transfer matrix is one 144 x 144
+    block, solved with the matrix in global memory (cft.jl:16, 296)."""
+    g, o, _ = _pair(tk, "TRG", "Trivial", om.ising_betac_mp, 12, 4)
+    ev_g, ev_o = tk.transfer_spectrum(g, 2), ob.transfer_spectrum(o, 2)
+    assert len(ev_g) == 144
+    assert same_multiset(_big(ev_g), _big(ev_o), 1e-8)
+    cg, co = tk.cft_data(g, unitcell=2), ob.cft_data(o, unitcell=2)
+    assert np.max(np.abs(cg[:4] - co[:4])) < 1e-7
     g.close()
 
 

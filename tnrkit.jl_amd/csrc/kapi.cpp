@@ -136,18 +136,22 @@ int tnr_kernel_pivoted_gauge(int m, int k, const double* U, const double* s, dou
 
 int tnr_kernel_eig(int count, int n, const double* A, double* wr, double* wi) {
     return kguard([&] {
-        TNR_REQUIRE(count > 0 && n > 0 && n <= EIG_NMAX && A && wr && wi, TNR_ERR_ARG,
+        TNR_REQUIRE(count > 0 && n > 0 && n <= EIG_NMAX_G && A && wr && wi, TNR_ERR_ARG,
                     "bad eigenvalue arguments");
         Ctx ctx;
         Buf dA = up(ctx, A, (size_t)count * n * n);
         Buf dr(&ctx.pool, (size_t)count * n * 8), di(&ctx.pool, (size_t)count * n * 8);
         Buf dinfo(&ctx.pool, (size_t)count * sizeof(int));
+        Buf work;
+        if (n > EIG_NMAX) work = Buf(&ctx.pool, (size_t)count * eig_work_doubles(n) * 8);
         std::vector<EigDesc> d;
         for (int i = 0; i < count; ++i)
             d.push_back(EigDesc{dA.d() + (size_t)i * n * n, dr.d() + (size_t)i * n,
-                                di.d() + (size_t)i * n, dinfo.as<int>() + i, n, n});
+                                di.d() + (size_t)i * n, dinfo.as<int>() + i, n, n,
+                                n > EIG_NMAX ? work.d() + (size_t)i * eig_work_doubles(n)
+                                             : nullptr});
         EigDesc* dd = ctx.stage->put(ctx.stream, d);
-        launch_eig(ctx.stream, dd, count, n);
+        launch_eig(ctx.stream, dd, count, n, n);
         std::vector<int> info(count);
         ctx.fetch(wr, dr.d(), (size_t)count * n * 8);
         ctx.fetch(wi, di.d(), (size_t)count * n * 8);

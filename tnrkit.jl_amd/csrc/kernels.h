@@ -142,16 +142,21 @@ struct DotDesc {
     int64_t n;
 };
 
-// All eigenvalues of a real n x n matrix (n <= EIG_NMAX), k_eig_hqr: wr/wi
-// receive the real/imaginary parts (conjugate pairs adjacent), *info 0 or 1
-// when the QR iteration did not converge.
+// All eigenvalues of a real n x n matrix, k_eig_hqr (n <= EIG_NMAX, the
+// matrix in LDS) or k_eig_hqr_g (n <= EIG_NMAX_G, in `work`,
+// eig_work_doubles(n) doubles of device memory): wr/wi receive the
+// real/imaginary parts (conjugate pairs adjacent), *info 0 or 1 when the QR
+// iteration did not converge.
 constexpr int EIG_NMAX = 128;
+constexpr int EIG_NMAX_G = 512;
+inline int64_t eig_work_doubles(int n) { return (int64_t)n * (n + 1) + n; }
 struct EigDesc {
     const double* A;
     double* wr;
     double* wi;
     int* info;
     int32_t n, lda;
+    double* work;      // n > EIG_NMAX only
 };
 
 // Device-side stopping rule of the block iteration (one thread per problem):
@@ -223,7 +228,7 @@ void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd);
 void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, double tie);
 void launch_dot(hipStream_t st, const DotDesc* d_descs, int nd);
 // nmax: the largest n of the problems (sizes the LDS)
-void launch_eig(hipStream_t st, const EigDesc* d_descs, int nd, int nmax);
+void launch_eig(hipStream_t st, const EigDesc* d_descs, int nd, int nmin, int nmax);
 void launch_randfill(hipStream_t st, const FillDesc* d_descs, int nd, int64_t total);
 void launch_gather(hipStream_t st, const GatherDesc* d_descs, int nd, int64_t total);
 

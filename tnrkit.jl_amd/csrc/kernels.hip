@@ -1470,11 +1470,11 @@ __global__ __launch_bounds__(64) void k_dot_final(const DotDesc* __restrict__ de
 constexpr int EIGT = 64;
 constexpr int EIG_MAXITS = 60;   // QR iterations per eigenvalue before giving up
 
-__global__ __launch_bounds__(EIGT) void k_eig_hqr(const EigDesc* __restrict__ descs, int nd) {
-    extern __shared__ double lds[];
-    const EigDesc D = descs[blockIdx.x];
+// The solver body; `hf` holds the n x n matrix (ld n+1) and n more values:
+// LDS for n <= EIG_NMAX (k_eig_hqr), the problem's workspace in global
+// memory above (k_eig_hqr_g: the same code, slower memory).
+__device__ __forceinline__ void eig_hqr_body(const EigDesc& D, double* hf) {
     const int nf = D.n, t = threadIdx.x, ld = nf + 1;
-    double* hf = lds;
     double* v = hf + (size_t)nf * ld;
 #define HF(i, j) hf[(i) + (j) * ld]
     for (int e = t; e < nf * nf; e += EIGT) {
@@ -1749,6 +1749,20 @@ __global__ __launch_bounds__(EIGT) void k_eig_hqr(const EigDesc* __restrict__ de
     }
 #undef HH
     if (t == 0) D.info[0] = info;
+}
+
+
+__global__ __launch_bounds__(EIGT) void k_eig_hqr(const EigDesc* __restrict__ descs, int nd) {
+    extern __shared__ double lds[];
+    const EigDesc D = descs[blockIdx.x];
+    if (D.n > EIG_NMAX) return;
+    eig_hqr_body(D, lds);
+}
+
+__global__ __launch_bounds__(EIGT) void k_eig_hqr_g(const EigDesc* __restrict__ descs, int nd) {
+    const EigDesc D = descs[blockIdx.x];
+    if (D.n <= EIG_NMAX) return;
+    eig_hqr_body(D, D.work);
 }
 
 // stopping rule of the block iteration, one thread per problem
@@ -2232,16 +2246,28 @@ void launch_dot(hipStream_t st, const DotDesc* d_descs, int nd) {
     TNR_HIP(hipGetLastError());
 }
 
-void launch_eig(hipStream_t st, const EigDesc* d_descs, int nd, int nmax) {
+// Problems up to EIG_NMAX run with the matrix in LDS, larger ones (up to
+// EIG_NMAX_G, each with its workspace of eig_work_doubles(n)) in global
+// memory; each kernel skips the other's, so a problem's kernel depends on
+// its own size only.
+void launch_eig(hipStream_t st, const EigDesc* d_descs, int nd, int nmin, int nmax) {
     if (nd <= 0) return;
-    TNR_REQUIRE(nmax >= 1 && nmax <= EIG_NMAX, TNR_ERR_INTERNAL, "eigenproblem above EIG_NMAX");
-    auto bytes = [](int n) { return ((size_t)n * (n + 1) + (size_t)n) * sizeof(double); };
+    TNR_REQUIRE(nmin >= 1 && nmax <= EIG_NMAX_G, TNR_ERR_INTERNAL, "eigenproblem above EIG_NMAX_G");
+    auto bytes = [](int n) { return (size_t)eig_work_doubles(n) * sizeof(double); };
     static const hipError_t attr = hipFuncSetAttribute(
         (const void*)k_eig_hqr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes(EIG_NMAX));
     TNR_HIP(attr);
-    ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    TNR_LAUNCH(ps, k_eig_hqr, dim3(nd), dim3(EIGT), bytes(nmax), st, d_descs, nd);
-    TNR_HIP(hipGetLastError());
+    if (nmin <= EIG_NMAX) {
+        ProfScope ps(st, PK_MISC, 0.0, 0.0);
+        TNR_LAUNCH(ps, k_eig_hqr, dim3(nd), dim3(EIGT), bytes(std::min(nmax, EIG_NMAX)), st, d_descs,
+                   nd);
+        TNR_HIP(hipGetLastError());
+    }
+    if (nmax > EIG_NMAX) {
+        ProfScope ps(st, PK_MISC, 0.0, 0.0);
+        TNR_LAUNCH(ps, k_eig_hqr_g, dim3(nd), dim3(EIGT), 0, st, d_descs, nd);
+        TNR_HIP(hipGetLastError());
+    }
 }
 
 void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax) {

@@ -121,31 +121,38 @@ std::vector<cplx> transfer_spectrum(Ctx& ctx, const Scheme& s, int unitcell) {
     const Layout& L = M.L;
     std::vector<EigDesc> d;
     std::vector<int> at;
-    int64_t total = 0;
-    int nmax = 0;
+    int64_t total = 0, wtotal = 0;
+    int nmin = EIG_NMAX_G, nmax = 0;
+    std::vector<int64_t> wat;
     for (int c = 0; c < L.N; ++c) {
         if (L.R[c] == 0) continue;
         TNR_REQUIRE(L.R[c] == L.C[c], TNR_ERR_INTERNAL, "transfer block not square");
-        TNR_REQUIRE(L.R[c] <= EIG_NMAX, TNR_ERR_ARG,
+        TNR_REQUIRE(L.R[c] <= EIG_NMAX_G, TNR_ERR_ARG,
                     "transfer-matrix block of dimension " + std::to_string(L.R[c]) +
-                        " is above the eigensolver's limit of " + std::to_string(EIG_NMAX));
+                        " is above the eigensolver's limit of " + std::to_string(EIG_NMAX_G));
         at.push_back((int)total);
+        wat.push_back(L.R[c] > EIG_NMAX ? wtotal : -1);
+        if (L.R[c] > EIG_NMAX) wtotal += eig_work_doubles((int)L.R[c]);
         d.push_back(EigDesc{M.d() + L.off[c], nullptr, nullptr, nullptr, (int32_t)L.R[c],
                             (int32_t)L.R[c]});
         total += L.R[c];
+        nmin = std::min(nmin, (int)L.R[c]);
         nmax = std::max(nmax, (int)L.R[c]);
     }
     std::vector<cplx> out;
     if (d.empty()) return out;
     Buf wr(&ctx.pool, total * sizeof(double)), wi(&ctx.pool, total * sizeof(double));
     Buf info(&ctx.pool, d.size() * sizeof(int));
+    Buf work;
+    if (wtotal) work = Buf(&ctx.pool, (size_t)wtotal * sizeof(double));
     for (size_t i = 0; i < d.size(); ++i) {
         d[i].wr = wr.d() + at[i];
         d[i].wi = wi.d() + at[i];
         d[i].info = info.as<int>() + i;
+        d[i].work = wat[i] >= 0 ? work.d() + wat[i] : nullptr;
     }
     EigDesc* dd = ctx.stage->put(ctx.stream, d);
-    launch_eig(ctx.stream, dd, (int)d.size(), nmax);
+    launch_eig(ctx.stream, dd, (int)d.size(), nmin, nmax);
     std::vector<double> hr(total), hi(total);
     std::vector<int> hinfo(d.size());
     ctx.fetch(hr.data(), wr.d(), total * sizeof(double));
