@@ -22,7 +22,9 @@ extern "C" {
  * split kernel, 2 / 3 / 4 / 5 = FP64 MFMA with 64x64 / 128x64 / 128x128 /
  * 64x64 (32-deep k-tiles) tiles;
  * mode | 8 never splits K over workgroups, mode | 16 always does (for K of
- * two or more 256-deep chunks); the default decides by launch size.
+ * two or more 256-deep chunks); the default decides by launch size;
+ * mode | 32 loads the 16-deep tiles through registers instead of LDS-DMA
+ * (the same bits).
  * repeat >= 1 launches the product that many times on device-resident
  * copies; *ms_out (optional) receives the device time of the last launch
  * from HIP events. */

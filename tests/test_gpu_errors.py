@@ -69,7 +69,7 @@ def test_run_rejects_chi_below_one_and_wrong_state_calls(tk):
 
 
 def test_eigensolver_rejects_blocks_above_its_limit(tk):
-    n = 129
+    n = 513
     A = np.eye(n)
     wr = np.zeros(n)
     wi = np.zeros(n)

@@ -78,6 +78,51 @@ def test_gemm_split_k_does_not_change_bits(tk, M, N, K, ta, tb):
     assert np.max(np.abs(outs[0] - ref)) <= 1e-14 * (np.sqrt(K) + 1) * 8
 
 
+def gemm_padded(tk, A, B, Cm, ta, tb, pad, mode):
+    """tnr_kernel_gemm on operands stored with leading dimensions `pad`
+    larger than their rows; the padding holds NaN, which must not reach
+    the product."""
+    M, N = Cm.shape
+    K = A.shape[0] if ta else A.shape[1]
+    As = np.full((A.shape[0] + pad, A.shape[1]), np.nan, order="F")
+    As[:A.shape[0]] = A
+    Bs = np.full((B.shape[0] + pad, B.shape[1]), np.nan, order="F")
+    Bs[:B.shape[0]] = B
+    Co = np.asfortranarray(Cm.copy())
+    st = tk.lib().tnr_kernel_gemm(M, N, K, p(As), As.shape[0], ta, p(Bs), Bs.shape[0], tb, p(Co),
+                                  M, 1.0, 0.5, mode, 1, None)
+    assert st == 0, tk.lib().tnr_last_error()
+    return Co
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb,pad", [
+    (1024, 64, 1024, 0, 0, 0), (1024, 64, 1024, 1, 0, 0), (1024, 64, 1024, 0, 1, 0),
+    (1024, 64, 1024, 1, 1, 0), (300, 130, 2049, 0, 1, 0), (129, 257, 33, 1, 0, 3),
+    (200, 100, 77, 0, 0, 5), (200, 100, 77, 1, 1, 5), (100, 37, 53, 1, 0, 1),
+    (257, 129, 4, 0, 0, 2), (128, 64, 16, 0, 0, 0), (5, 3, 1, 1, 1, 1)])
+def test_gemm_lds_dma_matches_register_loads_bitwise(tk, M, N, K, ta, tb, pad):
+    """The 16-deep tile shapes load operands through LDS-DMA (three LDS
+    stages) by default and through registers with mode | 32: the MFMA
+    sequence is the same, so the bits are.  Covered: all four storage
+    orders, K, M and N not multiples of the tile, split and unsplit K, and
+    leading dimensions above the operand's rows with NaN in the padding
+    (DMA reads past a row or past K must contribute exact zeros)."""
+    rng = np.random.default_rng(M + 3 * N + 7 * K + ta + 2 * tb)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    Cm = rng.standard_normal((M, N))
+    ref = (A.T if ta else A) @ (B.T if tb else B) + 0.5 * Cm
+    for shape in (2, 3, 4):
+        for flag in (8, 16):
+            if flag == 16 and K < 512:
+                continue
+            dma = gemm_padded(tk, A, B, Cm, ta, tb, pad, shape | flag)
+            reg = gemm_padded(tk, A, B, Cm, ta, tb, pad, shape | flag | 32)
+            assert np.array_equal(dma, reg), (shape, flag)
+            assert np.all(np.isfinite(dma))
+            assert np.max(np.abs(dma - ref)) <= 1e-14 * (np.sqrt(K) + 1) * 8
+
+
 @pytest.mark.parametrize("M,N,K", [(128, 128, 256), (200, 100, 1000)])
 def test_gemm_split32(tk, M, N, K):
     rng = np.random.default_rng(K)

@@ -528,6 +528,7 @@ GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int
         P.tiles += mn * ks;
         if (ks > 1) work_elems += (size_t)j.M * j.N * ks;
         P.flops += 2.0 * j.M * j.N * j.K;
+        P.dma = P.dma && gemm_dma_ok(g);
         P.descs.push_back(g);
     }
     if (work_elems) {
@@ -557,6 +558,16 @@ void gemm_bind(GemmPlan& p, const DescArena& arena) {
     p.d_starts = arena.at<int64_t>(p.off_starts);
 }
 
+// TNR_GEMM_DMA=0 keeps the 16-deep tile shapes on register-staged loads
+// (the LDS-DMA kernels give the same bits)
+static bool gemm_dma_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("TNR_GEMM_DMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 void gemm_issue(Ctx& ctx, GemmPlan& p) {
     if (p.descs.empty()) return;
     TuneTrial tr;
@@ -564,15 +575,16 @@ void gemm_issue(Ctx& ctx, GemmPlan& p) {
     p.trial = false;   // one timing per plan
     if (trial) tune_record_begin(ctx, p, tr);
     launch_gemm(ctx.stream, p.d_descs, (int)p.descs.size(), p.tiles, p.split32, p.flops,
-                p.shape);
+                p.shape, p.dma && gemm_dma_enabled());
     if (trial) tune_record_end(ctx, p, tr);
     if (!p.red.empty())
         launch_gemm_reduce(ctx.stream, p.d_red, (int)p.red.size(), p.d_starts, p.red_total);
 }
 
 void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape,
-                  int force_split) {
+                  int force_split, bool allow_dma) {
     GemmPlan p = gemm_plan(ctx, jobs, split32, force_shape, force_split);
+    p.dma = p.dma && allow_dma;
     if (p.descs.empty()) return;
     DescArena arena;
     gemm_stage(p, arena);

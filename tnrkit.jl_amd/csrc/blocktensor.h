@@ -146,8 +146,9 @@ struct GemmJob {
     double alpha = 1.0, beta = 0.0;
     const int* skip = nullptr;       // device flag: problem finished
 };
+// allow_dma = false keeps the 16-deep tile shapes on register-staged loads
 void gemm_grouped(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32 = false,
-                  int force_shape = -1, int force_split = -1);
+                  int force_shape = -1, int force_split = -1, bool allow_dma = true);
 
 // Descriptor arrays gathered on the host and uploaded to the device in one
 // copy (through the staging ring); every launch queued before the ring's
@@ -176,6 +177,7 @@ struct GemmPlan {
     double flops = 0;
     bool split32 = false;
     int shape = 0;                     // GemmShape of the launch
+    bool dma = true;                   // every descriptor fits the LDS-DMA kernels
     bool trial = false;                // next issue is timed for the autotuner
     std::array<int, 9> tune_key{};
     Buf work;

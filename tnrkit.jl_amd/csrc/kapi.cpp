@@ -37,8 +37,9 @@ int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta, const
                     int repeat, double* ms_out) {
     return kguard([&] {
         const int kind = mode & 7, split = (mode & 8) ? 0 : (mode & 16) ? 1 : -1;
+        const bool dma = (mode & 32) == 0;
         TNR_REQUIRE(M > 0 && N > 0 && K >= 0 && A && B && C && repeat >= 1 && kind <= 5 &&
-                        (mode & ~31) == 0 && (mode & 24) != 24,
+                        (mode & ~63) == 0 && (mode & 24) != 24,
                     TNR_ERR_ARG, "bad GEMM arguments");
         TNR_REQUIRE(lda >= (ta ? std::max(K, 1) : M) && ldb >= (tb ? N : std::max(K, 1)) && ldc >= M,
                     TNR_ERR_ARG, "leading dimension too small");
@@ -58,7 +59,7 @@ int tnr_kernel_gemm(int M, int N, int K, const double* A, int lda, int ta, const
             if (r > 0 && beta != 0.0)
                 TNR_HIP(hipMemcpyAsync(dC.d(), dC0.d(), nc * 8, hipMemcpyDeviceToDevice, ctx.stream));
             if (r == repeat - 1 || r == 0) TNR_HIP(hipEventRecord(e0, ctx.stream));
-            gemm_grouped(ctx, j, kind == 1, kind >= 2 ? kind - 2 : -1, split);
+            gemm_grouped(ctx, j, kind == 1, kind >= 2 ? kind - 2 : -1, split, dma);
             if (r == repeat - 1 || r == 0) {
                 TNR_HIP(hipEventRecord(e1, ctx.stream));
                 TNR_HIP(hipEventSynchronize(e1));

@@ -197,16 +197,24 @@ void launch_transpose(hipStream_t st, const TransDesc* d_descs, int nd, int64_t 
                       int64_t elems);
 // tile shapes of the FP64 GEMM kernel (the split-FP32 kernel is 64 x 64)
 // FP64 tile shapes: workgroup tile BM x BN, waves, k-tile depth
-//   GEMM_64x64       64x64,  4 waves of 32x32, k 16
-//   GEMM_128x64     128x64,  8 waves of 32x32, k 16
-//   GEMM_128x128   128x128,  8 waves of 32x64, k 16
-//   GEMM_64x64_K32   64x64,  4 waves of 32x32, k 32
+//   GEMM_64x64       64x64,  4 waves of 32x32, k 16 (LDS-DMA, 3 stages)
+//   GEMM_128x64     128x64,  8 waves of 32x32, k 16 (LDS-DMA, 3 stages)
+//   GEMM_128x128   128x128,  8 waves of 32x64, k 16 (LDS-DMA, 3 stages)
+//   GEMM_64x64_K32   64x64,  4 waves of 32x32, k 32 (register-staged loads)
 enum GemmShape { GEMM_64x64 = 0, GEMM_128x64 = 1, GEMM_128x128 = 2, GEMM_64x64_K32 = 3 };
 constexpr int GEMM_NSHAPES = 4;
 inline int gemm_shape_bm(int s) { return (s == GEMM_64x64 || s == GEMM_64x64_K32) ? 64 : 128; }
 inline int gemm_shape_bn(int s) { return s == GEMM_128x128 ? 128 : 64; }
+// dma: the 16-deep shapes load through LDS-DMA (k_gemm_dl, same bits); needs
+// every descriptor's per-tile offsets below 2^31 bytes (gemm_dma_ok)
 void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32,
-                 double flops, int shape);
+                 double flops, int shape, bool dma);
+inline bool gemm_dma_ok(const GemmDesc& g) {
+    const int64_t lim = (int64_t)1 << 31;
+    const int64_t la = g.ta ? 8 * (16 + 128 * (int64_t)g.lda) : 8 * (128 + 16 * (int64_t)g.lda);
+    const int64_t lb = g.tb ? 8 * (128 + 16 * (int64_t)g.ldb) : 8 * (16 + 128 * (int64_t)g.ldb);
+    return la < lim && lb < lim;
+}
 void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const int64_t* d_starts,
                         int64_t total);
 void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const TraceProb* d_probs,

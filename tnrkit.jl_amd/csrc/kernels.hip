@@ -441,6 +441,230 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
     write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, tot);
 }
 
+// ------------------------------------------------ FP64 MFMA, LDS-DMA pipeline
+// The same products as k_gemm_f64 (same MFMA sequence over k, same K-chunk
+// totals, so the same bits), with the operands going from global memory
+// straight into LDS (buffer_load_dwordx4 ... lds: lane l of a wave writes its
+// 16 bytes at the instruction's LDS base + 16 l) through three LDS stages:
+// the loads of k-tile t + 2 are in flight while k-tile t is multiplied, and
+// no registers hold tiles in transit.
+//
+// Each operand's stage region takes one of two layouts, by storage order:
+//  * row layout (operand contiguous along its M / N index: A as stored, B
+//    stored transposed): k-row kk holds the D values of that k; even rows
+//    are packed from offset 0, odd rows from 8 D + 16, so rows kk and kk + 1
+//    sit in opposite halves of the 64 LDS banks;
+//  * column layout (contiguous along k): 16-byte slots (i, kp) hold
+//    k = 2 kp, 2 kp + 1 of index i, slot i * 8 + (kp ^ ((i >> 1) & 7)): one
+//    DMA instruction fetches 8 whole 128-byte lines, and the MFMA operand
+//    reads of 16 consecutive i at one k fall in distinct banks.
+// Loads go through buffer descriptors rebased per k-tile whose extent ends
+// at the operand's last element: reads past it return zero and never fault.
+// Rows past M (columns past N) may hold other elements of the operand; they
+// only reach output rows (columns) that are not written.  k past K is zeroed
+// in LDS before the last k-tile is used, so padded products are 0 * 0 as in
+// k_gemm_f64.
+constexpr int DL_KB = 16;
+
+__host__ __device__ constexpr int dl_region(int d) { return 16 * d + 16; }
+
+// LDS offset (doubles) of element (i, kk) of a D-wide operand region
+__device__ __forceinline__ int dl_off(bool rowl, int D, int i, int kk) {
+    if (rowl) return ((kk & 1) ? 8 * D + 16 : 0) + (kk >> 1) * D + i;
+    return (i * 8 + ((kk >> 1) ^ ((i >> 1) & 7))) * 2 + (kk & 1);
+}
+
+// What DMA instruction u (of D / 8) of a D-wide operand region fetches for
+// this lane: its LDS base (doubles, uniform), the lane's element (i, k pair)
+__device__ __forceinline__ void dl_slot(bool rowl, int D, int u, int lane, int& base, int& i,
+                                        int& kk0, int& kk1) {
+    if (rowl) {
+        const int half = D / 16;       // instructions per row parity
+        const int odd = u >= half, uu = odd ? u - half : u;
+        const int e = 128 * uu + 2 * lane;
+        base = (odd ? 8 * D + 16 : 0) + 128 * uu;
+        i = e % D;
+        kk0 = kk1 = 2 * (e / D) + odd;
+    } else {
+        base = 128 * u;
+        i = 8 * u + (lane >> 3);
+        const int kp = (lane & 7) ^ ((i >> 1) & 7);
+        kk0 = 2 * kp;
+        kk1 = 2 * kp + 1;
+    }
+}
+
+template <int BM, int BN, int NW>
+struct DlPlan {
+    static constexpr int SA = dl_region(BM), SB = dl_region(BN), ST = SA + SB;
+    static constexpr int QA = BM / 8, Q = (BM + BN) / 8, PER = Q / NW;   // DMA instructions
+    static_assert(Q % NW == 0, "DMA instructions must divide over the waves");
+};
+
+// LDS-DMA for one k-tile: this wave's PER instructions into `stage`
+template <int BM, int BN, int NW>
+__device__ __forceinline__ void dl_issue(double* stage, __amdgpu_buffer_rsrc_t ra,
+                                         __amdgpu_buffer_rsrc_t rb, const int (&lbase)[DlPlan<BM, BN, NW>::PER],
+                                         const int (&vofs)[DlPlan<BM, BN, NW>::PER], int w) {
+    using P = DlPlan<BM, BN, NW>;
+#pragma unroll
+    for (int r = 0; r < P::PER; ++r) {
+        const int q = w + NW * r;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(q < P::QA ? ra : rb,
+                                                 (__attribute__((address_space(3))) void*)(stage + lbase[r]),
+                                                 16, vofs[r], 0, 0, 0);
+    }
+}
+
+// buffer descriptor over [base, base + n) doubles (n clamped to the 32-bit
+// byte range)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dl_rsrc(const double* base, int64_t n) {
+    const int64_t bytes = n > 0 ? n * 8 : 0;
+    const int nb = (int)(bytes > 0x7fffffffLL ? 0x7fffffffLL : bytes);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, nb, 0x00020000);
+}
+
+// OCC waves per SIMD: 4 lets two 128x64 workgroups share a CU (their three
+// stages take 73 KB of LDS each), 3 fits three 64x64 ones (49 KB each)
+template <int BM, int BN, int WM, int WN, int OCC>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
+void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* flop_ctr) {
+    constexpr int NW = WM * WN;
+    using P = DlPlan<BM, BN, NW>;
+    constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
+    __shared__ __align__(16) double S0[P::ST];
+    __shared__ __align__(16) double S1[P::ST];
+    __shared__ __align__(16) double S2[P::ST];
+    // the wave index is uniform: keep it (and the LDS bases derived from it)
+    // in scalar registers, as the DMA's LDS address must be
+    const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wm0 = (w % WM) * (BM / WM), wn0 = (w / WM) * (BN / WN);
+    const GemmDesc g = descs[find_desc_gemm(descs, nd, blockIdx.x)];
+    if (g.skip && *g.skip) return;
+    const TileCoord tc = tile_coord<BM, BN>(g, (int64_t)blockIdx.x - g.tile_start);
+    count_tile_flops<BM, BN>(flop_ctr, g, tc);
+    if (flop_ctr) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const bool arow = !g.ta, brow = g.tb != 0;
+
+    // per-lane DMA offsets (bytes, relative to the k-tile's descriptors) and
+    // LDS bases of this wave's instructions
+    int lbase[P::PER], vofs[P::PER];
+#pragma unroll
+    for (int r = 0; r < P::PER; ++r) {
+        const int q = w + NW * r;
+        int base, i, kk0, kk1;
+        if (q < P::QA) {
+            dl_slot(arow, BM, q, lane, base, i, kk0, kk1);
+            lbase[r] = __builtin_amdgcn_readfirstlane(base);
+            vofs[r] = 8 * (arow ? i + kk0 * g.lda : kk0 + i * g.lda);
+        } else {
+            dl_slot(brow, BN, q - P::QA, lane, base, i, kk0, kk1);
+            lbase[r] = __builtin_amdgcn_readfirstlane(P::SA + base);
+            vofs[r] = 8 * (brow ? i + kk0 * g.ldb : kk0 + i * g.ldb);
+        }
+    }
+    // the operands' last elements bound the descriptors
+    const int64_t extA = arow ? (int64_t)g.lda * (g.K - 1) + g.M : (int64_t)g.lda * (g.M - 1) + g.K;
+    const int64_t extB = brow ? (int64_t)g.ldb * (g.K - 1) + g.N : (int64_t)g.ldb * (g.N - 1) + g.K;
+    auto rsrc_a = [&](int k0) {
+        const int64_t o = arow ? (int64_t)tc.m0 + (int64_t)k0 * g.lda : (int64_t)k0 + (int64_t)tc.m0 * g.lda;
+        return dl_rsrc(g.A + o, extA - o);
+    };
+    auto rsrc_b = [&](int k0) {
+        const int64_t o = brow ? (int64_t)tc.n0 + (int64_t)k0 * g.ldb : (int64_t)k0 + (int64_t)tc.n0 * g.ldb;
+        return dl_rsrc(g.B + o, extB - o);
+    };
+    // MFMA operand offsets for each 4-deep k-step (row i / column j at mi,
+    // ni = 0; +16 rows is +16 doubles in the row layout, +256 in the column one)
+    int aoff[DL_KB / 4], boff[DL_KB / 4];
+#pragma unroll
+    for (int ks = 0; ks < DL_KB / 4; ++ks) {
+        const int kk = ks * 4 + (lane >> 4);
+        aoff[ks] = dl_off(arow, BM, wm0 + (lane & 15), kk);
+        boff[ks] = P::SA + dl_off(brow, BN, wn0 + (lane & 15), kk);
+    }
+    const int astep = arow ? 16 : 256, bstep = brow ? 16 : 256;
+
+    dbl4 acc[MI][NI], tot[MI][NI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) {
+            acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+            tot[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
+    const int nt = (tc.kend - tc.kbeg + DL_KB - 1) / DL_KB;
+    if (nt > 0) dl_issue<BM, BN, NW>(S0, rsrc_a(tc.kbeg), rsrc_b(tc.kbeg), lbase, vofs, w);
+    if (nt > 1) dl_issue<BM, BN, NW>(S1, rsrc_a(tc.kbeg + DL_KB), rsrc_b(tc.kbeg + DL_KB), lbase, vofs, w);
+
+    // one k-tile: wait for this wave's loads of tile it, zero k >= K on the
+    // last tile, barrier (every wave's loads of tile it have landed and every
+    // wave is done with the stage tile it + 2 goes into), issue tile it + 2,
+    // multiply
+    auto step = [&](double* cur, double* nxt, int it) {
+        const int k0 = tc.kbeg + it * DL_KB;
+        if (it + 1 < nt) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(P::PER) : "memory");
+        else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (k0 + DL_KB > g.K) {
+            const int kv = g.K - k0;
+#pragma unroll
+            for (int r = 0; r < P::PER; ++r) {
+                const int q = w + NW * r;
+                int base, i, kk0, kk1;
+                if (q < P::QA) dl_slot(arow, BM, q, lane, base, i, kk0, kk1);
+                else dl_slot(brow, BN, q - P::QA, lane, base, i, kk0, kk1);
+                double* d = cur + lbase[r] + 2 * lane;
+                if (kk0 >= kv) d[0] = 0.0;
+                if (kk1 >= kv) d[1] = 0.0;
+            }
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        if (it + 2 < nt)
+            dl_issue<BM, BN, NW>(nxt, rsrc_a(k0 + 2 * DL_KB), rsrc_b(k0 + 2 * DL_KB), lbase, vofs, w);
+        __builtin_amdgcn_sched_barrier(0);
+        double a[2][MI], b[2][NI];
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi) a[0][mi] = cur[aoff[0] + mi * astep];
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) b[0][ni] = cur[boff[0] + ni * bstep];
+#pragma unroll
+        for (int ks = 0; ks < DL_KB / 4; ++ks) {
+            const int c = ks & 1;
+            if (ks + 1 < DL_KB / 4) {
+#pragma unroll
+                for (int mi = 0; mi < MI; ++mi) a[c ^ 1][mi] = cur[aoff[ks + 1] + mi * astep];
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni) b[c ^ 1][ni] = cur[boff[ks + 1] + ni * bstep];
+            }
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][mi], b[c][ni],
+                                                                       acc[mi][ni], 0, 0, 0);
+        }
+        if ((k0 + DL_KB) % g.kchunk == 0 || it + 1 == nt) {
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) tot[mi][ni][r] += acc[mi][ni][r];
+                    acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
+                }
+        }
+        // the stage's reads are complete before the next barrier lets another
+        // wave's DMA overwrite it
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    for (int it = 0; it < nt; it += 3) {
+        step(S0, S2, it);
+        if (it + 1 < nt) step(S1, S0, it + 1);
+        if (it + 2 < nt) step(S2, S1, it + 2);
+    }
+    write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, tot);
+}
+
 // ------------------------------------------------ FP32 MFMA, hi/lo split
 // x = hi + lo with hi = fp32(x), lo = fp32(x - hi).  Per k-tile the three
 // significant products hi*hi, hi*lo, lo*hi are accumulated on FP32 MFMA
@@ -2085,12 +2309,21 @@ void launch_transpose(hipStream_t st, const TransDesc* d_descs, int nd, int64_t 
 }
 
 void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32,
-                 double flops, int shape) {
+                 double flops, int shape, bool dma) {
     if (tiles <= 0 || nd <= 0) return;
     ProfScope ps(st, split32 ? PK_GEMM_SPLIT32 : PK_GEMM_F64, flops, 0.0);
     unsigned long long* fc = ps.active ? prof_flop_counter(split32) : nullptr;
     if (split32)
         TNR_LAUNCH(ps, k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd, fc);
+    else if (shape == GEMM_128x128 && dma)
+        TNR_LAUNCH(ps, (k_gemm_dl<128, 128, 4, 2, 2>), dim3((unsigned)tiles), dim3(512), 0, st,
+                           d_descs, nd, fc);
+    else if (shape == GEMM_128x64 && dma)
+        TNR_LAUNCH(ps, (k_gemm_dl<128, 64, 4, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, st,
+                           d_descs, nd, fc);
+    else if (shape == GEMM_64x64 && dma)
+        TNR_LAUNCH(ps, (k_gemm_dl<64, 64, 2, 2, 3>), dim3((unsigned)tiles), dim3(256), 0, st,
+                           d_descs, nd, fc);
     else if (shape == GEMM_128x128)
         TNR_LAUNCH(ps, (k_gemm_f64<128, 128, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
                            st, d_descs, nd, fc);
