@@ -86,7 +86,9 @@ def _gemm_roofline(prof, name, steps=1):
     ach = flops / (d["busy_ms"] * 1e-3) if d["busy_ms"] else 0.0
     per_launch = flops / (d["ms"] * 1e-3) if d["ms"] else 0.0
     n = max(1, d["launches"])
-    return {"kernel": "k_gemm_split32" if name == "gemm_split32" else "k_gemm_f64",
+    return {"kernel": "k_gemm_split32" if name == "gemm_split32" else "fp64_gemm",
+            "kernels": "k_gemm_split32" if name == "gemm_split32" else
+                       "k_gemm_dl (LDS-DMA tiles) + k_gemm_f64 (32-deep register tile)",
             "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": ach / peak, "launches": d["launches"],
             "flops_per_launch": d["flops"] / n,

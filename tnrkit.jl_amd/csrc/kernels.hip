@@ -524,17 +524,22 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t dl_rsrc(const double* base, in
     return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, nb, 0x00020000);
 }
 
-// OCC waves per SIMD: 4 lets two 128x64 workgroups share a CU (their three
-// stages take 73 KB of LDS each), 3 fits three 64x64 ones (49 KB each)
-template <int BM, int BN, int WM, int WN, int OCC>
+// NS LDS stages (3 or 4, each its own array so that the compiler's wait
+// for a stage's loads does not include the other stages'); OCC waves per
+// SIMD: 4 lets two 128x64 workgroups share a CU (three stages: 73 KB of LDS
+// each), 3 fits three 64x64 ones (49 KB each)
+template <int BM, int BN, int WM, int WN, int OCC, int NS>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* flop_ctr) {
+    static_assert(NS == 3 || NS == 4, "three or four LDS stages");
     constexpr int NW = WM * WN;
     using P = DlPlan<BM, BN, NW>;
     constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
     __shared__ __align__(16) double S0[P::ST];
     __shared__ __align__(16) double S1[P::ST];
     __shared__ __align__(16) double S2[P::ST];
+    __shared__ __align__(16) double S3[NS > 3 ? P::ST : 2];
+    double* const SS[4] = {S0, S1, S2, S3};
     // the wave index is uniform: keep it (and the LDS bases derived from it)
     // in scalar registers, as the DMA's LDS address must be
     const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -594,16 +599,21 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
             tot[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
     const int nt = (tc.kend - tc.kbeg + DL_KB - 1) / DL_KB;
-    if (nt > 0) dl_issue<BM, BN, NW>(S0, rsrc_a(tc.kbeg), rsrc_b(tc.kbeg), lbase, vofs, w);
-    if (nt > 1) dl_issue<BM, BN, NW>(S1, rsrc_a(tc.kbeg + DL_KB), rsrc_b(tc.kbeg + DL_KB), lbase, vofs, w);
+#pragma unroll
+    for (int s = 0; s < NS - 1; ++s)
+        if (s < nt)
+            dl_issue<BM, BN, NW>(SS[s], rsrc_a(tc.kbeg + s * DL_KB), rsrc_b(tc.kbeg + s * DL_KB),
+                                 lbase, vofs, w);
 
     // one k-tile: wait for this wave's loads of tile it, zero k >= K on the
     // last tile, barrier (every wave's loads of tile it have landed and every
-    // wave is done with the stage tile it + 2 goes into), issue tile it + 2,
-    // multiply
+    // wave is done with the stage tile it + NS - 1 goes into), issue tile
+    // it + NS - 1, multiply
     auto step = [&](double* cur, double* nxt, int it) {
         const int k0 = tc.kbeg + it * DL_KB;
-        if (it + 1 < nt) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(P::PER) : "memory");
+        const int ahead = min(NS - 2, nt - 1 - it);   // tiles issued after this one
+        if (NS > 3 && ahead >= 2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * P::PER) : "memory");
+        else if (ahead >= 1) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(P::PER) : "memory");
         else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         if (k0 + DL_KB > g.K) {
             const int kv = g.K - k0;
@@ -619,8 +629,9 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
             }
         }
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-        if (it + 2 < nt)
-            dl_issue<BM, BN, NW>(nxt, rsrc_a(k0 + 2 * DL_KB), rsrc_b(k0 + 2 * DL_KB), lbase, vofs, w);
+        if (it + NS - 1 < nt)
+            dl_issue<BM, BN, NW>(nxt, rsrc_a(k0 + (NS - 1) * DL_KB), rsrc_b(k0 + (NS - 1) * DL_KB),
+                                 lbase, vofs, w);
         __builtin_amdgcn_sched_barrier(0);
         double a[2][MI], b[2][NI];
 #pragma unroll
@@ -657,10 +668,10 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
         // wave's DMA overwrite it
         __builtin_amdgcn_sched_barrier(0);
     };
-    for (int it = 0; it < nt; it += 3) {
-        step(S0, S2, it);
-        if (it + 1 < nt) step(S1, S0, it + 1);
-        if (it + 2 < nt) step(S2, S1, it + 2);
+    for (int it = 0; it < nt; it += NS) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            if (it + s < nt) step(SS[s], SS[(s + NS - 1) % NS], it + s);
     }
     write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, tot);
 }
@@ -2316,13 +2327,13 @@ void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles,
     if (split32)
         TNR_LAUNCH(ps, k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd, fc);
     else if (shape == GEMM_128x128 && dma)
-        TNR_LAUNCH(ps, (k_gemm_dl<128, 128, 4, 2, 2>), dim3((unsigned)tiles), dim3(512), 0, st,
+        TNR_LAUNCH(ps, (k_gemm_dl<128, 128, 4, 2, 2, 3>), dim3((unsigned)tiles), dim3(512), 0, st,
                            d_descs, nd, fc);
     else if (shape == GEMM_128x64 && dma)
-        TNR_LAUNCH(ps, (k_gemm_dl<128, 64, 4, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, st,
+        TNR_LAUNCH(ps, (k_gemm_dl<128, 64, 4, 2, 4, 3>), dim3((unsigned)tiles), dim3(512), 0, st,
                            d_descs, nd, fc);
     else if (shape == GEMM_64x64 && dma)
-        TNR_LAUNCH(ps, (k_gemm_dl<64, 64, 2, 2, 3>), dim3((unsigned)tiles), dim3(256), 0, st,
+        TNR_LAUNCH(ps, (k_gemm_dl<64, 64, 2, 2, 3, 3>), dim3((unsigned)tiles), dim3(256), 0, st,
                            d_descs, nd, fc);
     else if (shape == GEMM_128x128)
         TNR_LAUNCH(ps, (k_gemm_f64<128, 128, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,

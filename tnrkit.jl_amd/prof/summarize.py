@@ -18,6 +18,8 @@ import json
 import re
 
 COPY_KIB = 256 * 1024
+# the FP64 GEMM launches: LDS-DMA tiles and the 32-deep register-staged tile
+FP64_GEMM = ("k_gemm_dl", "k_gemm_f64")
 
 
 def short(name):
@@ -57,12 +59,13 @@ def _union_ms(spans):
 
 
 def phases(trace, kernel, spec):
-    """Per consecutive group of `kernel`'s dispatches in start order (e.g.
-    warmup, timed steps, a last pass), from the trace: the average duration,
-    the summed durations and the union of the dispatch intervals (dispatches
-    on different streams overlap; the union is the time the kernel was on
-    the device)."""
-    rows = [r for r in csv.DictReader(open(trace)) if short(r["Kernel_Name"]).split("<")[0] == kernel]
+    """Per consecutive group of the dispatches of `kernel` (a name or a
+    comma-separated list of names) in start order (e.g. warmup, timed steps,
+    a last pass), from the trace: the average duration, the summed durations
+    and the union of the dispatch intervals (dispatches on different streams
+    overlap; the union is the time the kernels were on the device)."""
+    names = set(kernel.split(","))
+    rows = [r for r in csv.DictReader(open(trace)) if short(r["Kernel_Name"]).split("<")[0] in names]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     spans = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows]
     dur = [(e - s) / 1e3 for s, e in spans]
@@ -101,7 +104,10 @@ def traffic(fetch, write, cal_fetch, cal_write, command):
     per = collections.defaultdict(lambda: [0, 0.0, 0, 0.0])
     for col, data in ((0, _per_kernel(fetch)), (2, _per_kernel(write))):
         for name, xs in data.items():
-            for key in {short(name), short(name).split("<")[0]}:
+            keys = {short(name), short(name).split("<")[0]}
+            if short(name).split("<")[0] in FP64_GEMM:
+                keys.add("fp64_gemm")
+            for key in keys:
                 per[key][col] += len(xs)
                 per[key][col + 1] += sum(xs)
     kernels = {}
