@@ -46,8 +46,11 @@ __device__ __forceinline__ int find_desc_copy(const CopyDesc* d, int n, int64_t 
 // One chunk of COPY_CHUNK consecutive elements (destination order) of one
 // job per workgroup iteration: the job is looked up once per chunk, and the
 // index decomposition runs in 32-bit arithmetic (a job holds < 2^31
-// elements; checked on the host).
+// elements; checked on the host).  Each thread decodes COPY_GROUP elements,
+// issues their loads together, then scales and stores them, so that many
+// loads are in flight per thread.
 constexpr int COPY_CHUNK = 4096;
+constexpr int COPY_GROUP = 8;
 
 __global__ __launch_bounds__(256) void k_copy(const CopyDesc* __restrict__ descs, int nd,
                                               int64_t chunks) {
@@ -69,27 +72,45 @@ __global__ __launch_bounds__(256) void k_copy(const CopyDesc* __restrict__ descs
         }
         const uint32_t base = (uint32_t)(c - D.start) * COPY_CHUNK;
         const int wl0 = D.wleg[0], wl1 = D.wleg[1];
-        for (int u = 0; u < COPY_CHUNK / 256; ++u) {
-            uint32_t r = base + (uint32_t)(t + 256 * u);
-            if (r >= cnt) break;
-            int64_t so = 0, dof = 0;
-            uint32_t i_w0 = 0, i_w1 = 0;
+        gconst_double* src = as_global(D.src);
+        gmut_double* dst = as_global(D.dst);
+        for (int u0 = 0; u0 < COPY_CHUNK / 256; u0 += COPY_GROUP) {
+            if (base + (uint32_t)(256 * u0) >= cnt) break;
+            int64_t so[COPY_GROUP], dof[COPY_GROUP];
+            uint32_t iw0[COPY_GROUP], iw1[COPY_GROUP];
+            bool ok[COPY_GROUP];
 #pragma unroll
-            for (int q = 0; q < MAXLEG; ++q) {
-                if (q < ndim) {
-                    const uint32_t nq = r / dim[q];
-                    const uint32_t iq = r - nq * dim[q];
-                    r = nq;
-                    so += (int64_t)iq * D.sstr[q];
-                    dof += (int64_t)iq * D.dstr[q];
-                    if (q == wl0) i_w0 = iq;
-                    if (q == wl1) i_w1 = iq;
+            for (int g = 0; g < COPY_GROUP; ++g) {
+                uint32_t r = base + (uint32_t)(t + 256 * (u0 + g));
+                ok[g] = r < cnt;
+                if (!ok[g]) r = 0;          // element 0: a valid address, not stored
+                so[g] = 0;
+                dof[g] = 0;
+                iw0[g] = 0;
+                iw1[g] = 0;
+#pragma unroll
+                for (int q = 0; q < MAXLEG; ++q) {
+                    if (q < ndim) {
+                        const uint32_t nq = r / dim[q];
+                        const uint32_t iq = r - nq * dim[q];
+                        r = nq;
+                        so[g] += (int64_t)iq * D.sstr[q];
+                        dof[g] += (int64_t)iq * D.dstr[q];
+                        if (q == wl0) iw0[g] = iq;
+                        if (q == wl1) iw1[g] = iq;
+                    }
                 }
             }
-            double v = D.src[so];
-            if (wl0 >= 0) v *= D.w[0][i_w0];
-            if (wl1 >= 0) v *= D.w[1][i_w1];
-            D.dst[dof] = v;
+            double v[COPY_GROUP];
+#pragma unroll
+            for (int g = 0; g < COPY_GROUP; ++g) v[g] = src[so[g]];
+#pragma unroll
+            for (int g = 0; g < COPY_GROUP; ++g) {
+                double x = v[g];
+                if (wl0 >= 0) x *= D.w[0][iw0[g]];
+                if (wl1 >= 0) x *= D.w[1][iw1[g]];
+                if (ok[g]) dst[dof[g]] = x;
+            }
         }
     }
 }
