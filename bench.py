@@ -77,7 +77,7 @@ def profile_batch(tk, reqs, precision):
 def _gemm_roofline(prof, name, steps=1):
     """achieved = device-counted GEMM flops of the problems that ran / the
     union of the GEMM launch intervals (the time some GEMM was on the device;
-    launches of the two lanes overlap, so summed durations would double-count
+    launches of the lanes overlap, so summed durations would double-count
     shared time); beside it the same flops over the summed launch durations,
     the per-launch figure a kernel trace gives."""
     d = prof[name]
@@ -825,8 +825,8 @@ def main():
             else:
                 os.environ["TNR_LANES"] = prev
         g1 = _gemm_roofline(p1, gname)
-        g1["note"] = ("one untimed batch on one lane (TNR_LANES=1); the timed steps run two "
-                      "lanes whose kernels overlap, which lengthens each launch")
+        g1["note"] = ("one untimed batch on one lane (TNR_LANES=1); the timed steps run "
+                      "several lanes whose kernels overlap, which lengthens each launch")
         out["roofline_gemm_one_lane"] = g1
     if cpu is not None:
         out["cpu_baseline"] = cpu
