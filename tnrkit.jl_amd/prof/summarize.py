@@ -1,7 +1,8 @@
 """Turn rocprofv3 CSV output into the short summaries kept under profiles/.
 
   python tnrkit.jl_amd/prof/summarize.py stats KERNEL_STATS_CSV --command CMD \
-      [--trace KERNEL_TRACE_CSV --kernel NAME --phases name:count,...] > profiles/X.md
+      [--trace KERNEL_TRACE_CSV --kernel NAME --phases name:count,... [--also K1,K2]] \
+      > profiles/X.md
   python tnrkit.jl_amd/prof/summarize.py traffic FETCH_CSV WRITE_CSV CAL_FETCH_CSV CAL_WRITE_CSV \
       --command CMD > profiles/traffic_X.json
 
@@ -85,6 +86,37 @@ def phases(trace, kernel, spec):
     return "\n".join(out)
 
 
+def phases_also(trace, kernel, spec, others):
+    """Average launch duration of each kernel in `others` (comma-separated)
+    per phase, the phases bounded by the start of the first dispatch of
+    `kernel` in each phase of `spec`."""
+    names = set(kernel.split(","))
+    allrows = sorted(csv.DictReader(open(trace)), key=lambda r: int(r["Start_Timestamp"]))
+    prim = [int(r["Start_Timestamp"]) for r in allrows
+            if short(r["Kernel_Name"]).split("<")[0] in names]
+    bounds, i = [], 0
+    labels = []
+    for item in spec.split(","):
+        name, cnt = item.split(":")
+        bounds.append(prim[i])
+        labels.append(name)
+        i += int(cnt)
+    out = ["", "Other kernels per phase (phases start at the first `%s` dispatch of each):" % kernel,
+           "", "| kernel | " + " | ".join("%s: launches, avg us" % x for x in labels) + " |",
+           "|---|" + "---|" * len(labels)]
+    for k in others.split(","):
+        cells = []
+        rows = [r for r in allrows if short(r["Kernel_Name"]).split("<")[0] == k]
+        for p in range(len(bounds)):
+            lo = bounds[p]
+            hi = bounds[p + 1] if p + 1 < len(bounds) else None
+            d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows
+                 if int(r["Start_Timestamp"]) >= lo and (hi is None or int(r["Start_Timestamp"]) < hi)]
+            cells.append("%d, %.1f" % (len(d), sum(d) / len(d)) if d else "0, -")
+        out.append("| %s | %s |" % (k, " | ".join(cells)))
+    return "\n".join(out)
+
+
 def _per_kernel(path):
     d = collections.defaultdict(list)
     for r in csv.DictReader(open(path)):
@@ -131,6 +163,7 @@ def main():
     s.add_argument("--trace")
     s.add_argument("--kernel")
     s.add_argument("--phases")
+    s.add_argument("--also", help="comma-separated kernels timed per phase as well")
     t = sub.add_parser("traffic")
     for x in ("fetch", "write", "cal_fetch", "cal_write"):
         t.add_argument(x)
@@ -140,6 +173,8 @@ def main():
         print(stats(a.csv, a.command))
         if a.trace:
             print(phases(a.trace, a.kernel, a.phases))
+            if a.also:
+                print(phases_also(a.trace, a.kernel, a.phases, a.also))
     else:
         print(traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.command))
 
