@@ -375,6 +375,45 @@ __device__ __forceinline__ void write_tile(const GemmDesc& g, const TileCoord& t
         }
 }
 
+// Output through LDS: a register of the FP64 MFMA result holds 4 rows x 16
+// columns, which a direct store writes as 16 pieces of 32 bytes.  Instead
+// each wave parks 32 x 16 blocks of its result in its own LDS region `ep`
+// (EP_REGION doubles, column stride 34: conflict-free both ways) and stores
+// whole 256-byte column runs.  Same values and formula as write_tile.
+constexpr int EP_LD = 34, EP_REGION = 16 * EP_LD;
+
+template <int MI, int NI>
+__device__ __forceinline__ void write_tile_lds(const GemmDesc& g, const TileCoord& tc, int lane,
+                                               int wm0, int wn0, const dbl4 (&acc)[MI][NI],
+                                               double* ep) {
+    static_assert(MI == 2, "32-row wave blocks");
+    const bool part = g.ksplit > 1;
+    gmut_double* out = as_global(part ? g.work + (int64_t)tc.kz * g.M * g.N : g.C);
+    const int64_t ldo = part ? g.M : g.ldc;
+    const int row = lane & 31, cpar = lane >> 5;
+    const int grow = tc.m0 + wm0 + row;
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                ep[(lane & 15) * EP_LD + mi * 16 + (lane >> 4) + 4 * r] = acc[mi][ni][r];
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int col = 2 * c + cpar, gcol = tc.n0 + wn0 + ni * 16 + col;
+            const double v = ep[col * EP_LD + row];
+            if (grow < g.M && gcol < g.N) {
+                gmut_double* p = out + (int64_t)grow + (int64_t)gcol * ldo;
+                if (part) *p = v;
+                else *p = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * *p;
+            }
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    }
+}
+
 // WM x WN waves, each owning a (BM/WM) x (BN/WN) block of the tile; k-tiles
 // of KB.  Every shape runs the MFMAs over k in the same 4-wide steps.
 template <int BM, int BN, int WM, int WN, int KB>
@@ -459,7 +498,11 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
         __syncthreads();
         buf ^= 1;
     }
-    write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, tot);
+    // the loop's last barrier ends every read of As and Bs
+    if constexpr (MI == 2 && 2 * KB * (BM + 1) >= (NT / 64) * EP_REGION)
+        write_tile_lds<MI, NI>(g, tc, lane, wm0, wn0, tot, &As[0][0][0] + w * EP_REGION);
+    else
+        write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, tot);
 }
 
 // ------------------------------------------------ FP64 MFMA, LDS-DMA pipeline
@@ -694,7 +737,14 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
         for (int s = 0; s < NS; ++s)
             if (it + s < nt) step(SS[s], SS[(s + NS - 1) % NS], it + s);
     }
-    write_tile<true, MI, NI>(g, tc, lane, wm0, wn0, tot);
+    // the stages are free: the last k-tile waited for every load, and the
+    // barrier below for every wave's reads; wave w parks its blocks in slot
+    // w / NS of stage w % NS
+    static_assert(((NW + NS - 1) / NS) * EP_REGION <= P::ST, "epilogue fits the stages");
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    const int es = w % NS;
+    write_tile_lds<MI, NI>(g, tc, lane, wm0, wn0, tot,
+                           (es == 0 ? S0 : es == 1 ? S1 : es == 2 ? S2 : S3) + (w / NS) * EP_REGION);
 }
 
 // ------------------------------------------------ FP32 MFMA, hi/lo split
