@@ -109,14 +109,19 @@ int tnr_autotune_enable(int on);
 /* Solver limits and thresholds of the truncated SVD / eigensolver: the
  * defaults, then the TNR_SOLVER_CFG environment variable, then `spec`
  * ("key=value,..."; NULL or "" = defaults and environment only).  Keys:
- * max_iters (block iterations, default 80), sweeps / sweeps_iter (Jacobi
+ * max_iters (block iterations, default 80), stall_from (8: from that
+ * iteration on, a block iteration of a matrix whose smaller side is at most
+ * 384 stops as soon as even twice its current convergence rate could not
+ * reach conv_target by max_iters; 0 = iterate to max_iters; such a stall,
+ * like one at max_iters, is factorised whole), sweeps / sweeps_iter (Jacobi
  * sweep caps of final and in-loop solves, 40), tol_iter (in-loop Jacobi
  * threshold, 1e-10), tol_first (the first iteration's, 1e-4),
  * conv_target (1e-14), block_mult (2), block_extra (2),
  * cholqr_y (1), cholqr_z (1), cholqr_eigh (0) (block iteration steps by
  * pivoted Cholesky QR instead of Rayleigh-Ritz), rv_passes (2), rf_passes (1)
- * (Jacobi passes of the final extraction).  A solve that reaches max_iters or a sweep cap
- * unconverged fails its step with TNR_ERR_NOCONV.  Not thread-safe: call it
+ * (Jacobi passes of the final extraction).  A solve of a larger matrix that
+ * reaches max_iters, or one that reaches a sweep cap, unconverged fails its
+ * step with TNR_ERR_NOCONV.  Not thread-safe: call it
  * while no other entry point runs. */
 int tnr_solver_config(const char* spec);
 /* hipDeviceSynchronize() on the current device (timing brackets). */

@@ -171,6 +171,7 @@ struct ConvDesc {
     int* it;
     double* prod;
     int* done;
+    int32_t whole_ok;             // a stall is refactorised whole: give up early
 };
 
 // Deterministic uniform(-1,1) fill of a rows x cols column-major matrix.
@@ -229,8 +230,11 @@ void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, 
                    int max_sweeps);
 void launch_colnorm(hipStream_t st, const ColDesc* d_descs, int nd, int ncol_max, double rtol);
 void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax);
+// stall_from > 0: from that iteration on, a problem with whole_ok set stops
+// (unconverged) once prod * r^(2 (max_iters - it)) >= target, i.e. even at twice
+// its current rate it could not reach the target in the iterations left
 void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rtol, double target,
-                 int min_iters, int max_iters);
+                 int min_iters, int max_iters, int stall_from);
 void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd);
 // kmax: the largest k of the problems (sizes the LDS); tie: relative pivot tie
 void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, double tie);

@@ -2073,7 +2073,7 @@ __global__ __launch_bounds__(EIGT) void k_eig_hqr_g(const EigDesc* __restrict__ 
 
 // stopping rule of the block iteration, one thread per problem
 __global__ void k_conv(const ConvDesc* __restrict__ descs, int nd, double null_rtol,
-                       double target, int min_iters, int max_iters) {
+                       double target, int min_iters, int max_iters, int stall_from) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nd) return;
     const ConvDesc D = descs[p];
@@ -2094,6 +2094,11 @@ __global__ void k_conv(const ConvDesc* __restrict__ descs, int nd, double null_r
     *D.it = it;
     *D.prod = prod;
     if ((it >= min_iters && prod < target) || it >= max_iters) *D.done = 1;
+    // a stall that will be factorised whole anyway: stop iterating once even
+    // twice the current rate could not reach the target in the iterations left
+    else if (D.whole_ok && stall_from > 0 && it >= stall_from && it >= min_iters &&
+             prod * pow(r, 2.0 * (max_iters - it)) >= target)
+        *D.done = 1;
 }
 
 // in-place (M + M^T)/2 of square blocks (project_hermitian)
@@ -2594,11 +2599,11 @@ void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax)
 }
 
 void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rtol, double target,
-                 int min_iters, int max_iters) {
+                 int min_iters, int max_iters, int stall_from) {
     if (nd <= 0) return;
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
     TNR_LAUNCH(ps, k_conv, dim3((nd + 63) / 64), dim3(64), 0, st, d_descs, nd, null_rtol, target,
-                       min_iters, max_iters);
+                       min_iters, max_iters, stall_from);
     TNR_HIP(hipGetLastError());
 }
 

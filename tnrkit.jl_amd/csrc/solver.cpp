@@ -16,8 +16,8 @@
 namespace tnr {
 
 // "key=value,..." overrides of the defaults (keys: tol_iter, tol_first, sweeps_iter,
-// sweeps, block_mult, block_extra, conv_target, max_iters, cholqr_y,
-// cholqr_z, cholqr_eigh, rv_passes, rf_passes); unknown keys are an
+// sweeps, block_mult, block_extra, conv_target, max_iters, stall_from,
+// cholqr_y, cholqr_z, cholqr_eigh, rv_passes, rf_passes); unknown keys are an
 // argument error.  TNR_SOLVER_CFG in the
 // environment is applied first, then tnr_solver_config's string.
 static void apply_cfg(SolverCfg& c, const char* e) {
@@ -42,6 +42,7 @@ static void apply_cfg(SolverCfg& c, const char* e) {
             else if (k == "block_extra") c.block_extra = (int)v;
             else if (k == "conv_target") c.conv_target = v;
             else if (k == "max_iters") c.max_iters = (int)v;
+            else if (k == "stall_from") c.stall_from = (int)v;
             else if (k == "cholqr_y") c.cholqr_y = v != 0.0;
             else if (k == "cholqr_z") c.cholqr_z = v != 0.0;
             else if (k == "cholqr_eigh") c.cholqr_eigh = v != 0.0;
@@ -71,7 +72,7 @@ SolverCfg& solver_cfg() {
 void set_solver_cfg(const char* spec) {
     SolverCfg c = env_cfg();
     apply_cfg(c, spec);
-    TNR_REQUIRE(c.max_iters >= 1 && c.jacobi_sweeps >= 0 && c.jacobi_sweeps_iter >= 0 &&
+    TNR_REQUIRE(c.max_iters >= 1 && c.stall_from >= 0 && c.jacobi_sweeps >= 0 && c.jacobi_sweeps_iter >= 0 &&
                     c.block_mult >= 1.0 && c.block_extra >= 0 && c.rv_passes >= 1 &&
                     c.rv_passes <= 2 && c.rf_passes >= 1 && c.rf_passes <= 2,
                 TNR_ERR_ARG, "solver option out of range");
@@ -887,7 +888,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip, fl});
             // Ritz values of the Z side, or with Cholesky QR the pivots R_jj
             conv.push_back(ConvDesc{cfg.cholqr_z ? x.lam.d() : sv, l, s.kmax, 1, cs.it(i),
-                                    cs.prod(i), cs.done(i)});
+                                    cs.prod(i), cs.done(i), std::min(s.m, s.n) <= kWholeMax});
             // the loop's last Rayleigh-Ritz step left Z W in Y2; one pass at the
             // final threshold (into Z, free until V Wtot) rebuilds V.  With
             // Cholesky QR on the Z side Y2 is not a Ritz basis: two passes on Z.
@@ -932,7 +933,7 @@ void svd_grouped(Ctx& ctx, const std::vector<SvdIn>& in, std::vector<SvdOut>& ou
             gemm_issue(ctx, pMtU);
             rr_issue(ctx, pZ, it == 0);
             launch_conv(ctx.stream, d_conv, P, cfg.null_rtol, cfg.conv_target, cfg.min_iters,
-                        cfg.max_iters);
+                        cfg.max_iters, cfg.stall_from);
         });
         // the loop orthonormalises only to its looser Jacobi threshold: V is
         // rebuilt at full accuracy before the final extraction relies on
@@ -1063,7 +1064,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
                             x.sw.as<int>(), nullptr, x.V.d(), s.n, sv, skip, failv.d() + iter[i]});
             // Ritz values, or with Cholesky QR the pivots R_jj
             conv.push_back(ConvDesc{cfg.cholqr_eigh ? x.lam.d() : sv, l, s.kmax, 0, cs.it(i),
-                                    cs.prod(i), cs.done(i)});
+                                    cs.prod(i), cs.done(i), s.n <= kWholeMax});
             // one tight pass on the loop's last rotated Y (in Y2), output into
             // Y; after Cholesky QR steps Y2 is no Ritz basis: two passes on Y
             if (cfg.cholqr_eigh)
@@ -1096,7 +1097,7 @@ void eigh_grouped(Ctx& ctx, const std::vector<EighIn>& in, std::vector<EighOut>&
             gemm_issue(ctx, pMV);
             rr_issue(ctx, pY, it == 0);
             launch_conv(ctx.stream, d_conv, P, cfg.null_rtol, cfg.conv_target, cfg.min_iters,
-                        cfg.max_iters);
+                        cfg.max_iters, cfg.stall_from);
         });
         // V at full orthonormality from the loop's last rotated Y = M V W
         // (the loop uses a looser Jacobi threshold; one pass at the final

@@ -105,6 +105,11 @@ struct SolverCfg {
     int block_extra = 2;
     int min_iters = 2;
     int max_iters = 80;
+    // from this iteration on, a block iteration of a matrix small enough to
+    // be refactorised whole stops as soon as even twice its current
+    // convergence rate could not reach conv_target by max_iters (it would
+    // stall and be factorised whole anyway); 0 iterates to max_iters
+    int stall_from = 8;
 };
 SolverCfg& solver_cfg();
 // Defaults, then TNR_SOLVER_CFG, then `spec` ("key=value,...", see
