@@ -605,6 +605,32 @@ def workload_copy():
     return {"workload": "copy calibration: 3 device copies of 256 MiB", "bytes": nbytes}
 
 
+def workload_gemm(tk, shapes, modes, repeat=5):
+    """FP64 GEMM microbenchmark through tnr_kernel_gemm: C = A B on
+    device-resident copies, the device time of the last of `repeat` launches
+    (HIP events).  modes: 2 / 3 / 4 = 64x64 / 128x64 / 128x128 tiles, | 8 no
+    split-K, | 32 register-staged loads instead of LDS-DMA, 0 = the engine's
+    choice."""
+    D = C.POINTER(C.c_double)
+    rows = []
+    for (M, N, K) in shapes:
+        rng = np.random.default_rng(M + N + K)
+        A = np.asfortranarray(rng.standard_normal((M, K)))
+        B = np.asfortranarray(rng.standard_normal((K, N)))
+        Cm = np.zeros((M, N), order="F")
+        for mode in modes:
+            ms = C.c_double()
+            st = tk.lib().tnr_kernel_gemm(M, N, K, A.ctypes.data_as(D), M, 0, B.ctypes.data_as(D),
+                                          K, 0, Cm.ctypes.data_as(D), M, 1.0, 0.0, mode, repeat,
+                                          C.byref(ms))
+            if st != 0:
+                raise RuntimeError(tk.lib().tnr_last_error())
+            rows.append({"M": M, "N": N, "K": K, "mode": mode, "ms": ms.value,
+                         "tflops": 2.0 * M * N * K / (ms.value * 1e-3) / 1e12})
+    return {"workload": "FP64 GEMM through tnr_kernel_gemm, device-resident operands",
+            "cases": rows}
+
+
 def workload_c4(tk, n, maxiter, check):
     """512 BTRG runs, beta ~ U[0.35, 0.55], chi ~ U{16, 24, 32, 48}."""
     rng = np.random.default_rng(SEED + 4)
@@ -697,11 +723,15 @@ def main():
     ap.add_argument("--check", type=int, default=4, help="runs checked against the plain path")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gemm-shapes", default="4096x4096x1024,8192x8192x2048",
+                    help="--workload gemm: MxNxK,...")
+    ap.add_argument("--gemm-modes", default="2,3,4,34,35,36",
+                    help="--workload gemm: tnr_kernel_gemm modes")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrivals per second")
     ap.add_argument("--workload", choices=["scan", "c2", "c3", "c4", "imp", "obs", "atrg", "3d",
-                                           "plain", "copy"],
+                                           "plain", "copy", "gemm"],
                     default="scan",
                     help="scan = the headline beta scan; c2-c4, imp, obs, atrg, 3d and plain print a "
                          "report line of their own")
@@ -728,6 +758,9 @@ def main():
             r = workload_3d(tk, a.maxiter, 0.0 if a.no_cpu else a.cpu_seconds)
         elif a.workload == "plain":
             r = workload_plain(tk, a.chi, a.maxiter, 6)
+        elif a.workload == "gemm":
+            shapes = [tuple(int(x) for x in s.split("x")) for s in a.gemm_shapes.split(",")]
+            r = workload_gemm(tk, shapes, [int(m) for m in a.gemm_modes.split(",")])
         elif a.workload == "atrg":
             r = workload_atrg(tk, a.maxiter, 64 if a.batch == 128 else a.batch,
                               0.0 if a.no_cpu else a.cpu_seconds)
