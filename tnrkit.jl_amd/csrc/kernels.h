@@ -235,7 +235,11 @@ void launch_symmetrize(hipStream_t st, const ColDesc* d_descs, int nd, int nmax)
 // its current rate it could not reach the target in the iterations left
 void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rtol, double target,
                  int min_iters, int max_iters, int stall_from);
-void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd);
+// sigma[0] of each descriptor = squared Frobenius norm of its nrow x ncol
+// matrix Y (ld ldy), summed in a fixed order over FROB_PARTS column ranges;
+// partial: nd * FROB_PARTS doubles of scratch
+constexpr int FROB_PARTS = 64;
+void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd, double* partial);
 // kmax: the largest k of the problems (sizes the LDS); tie: relative pivot tie
 void launch_canon(hipStream_t st, const CanonDesc* d_descs, int nd, int kmax, double tie);
 void launch_dot(hipStream_t st, const DotDesc* d_descs, int nd);

@@ -2164,16 +2164,46 @@ __global__ __launch_bounds__(256) void k_gather(const GatherDesc* __restrict__ d
 }
 
 // ------------------------------------------------ Frobenius norm of blocks
-__global__ __launch_bounds__(256) void k_frob(const ColDesc* __restrict__ descs, int nd) {
+// Squared Frobenius norm of each problem's nrow x ncol matrix, in a fixed
+// order that depends on the matrix alone: grid (FROB_PARTS, problems); part
+// q sums columns [q ncol / FROB_PARTS, (q + 1) ncol / FROB_PARTS), each
+// thread its rows t + 256 u into four accumulators by u mod 4 (independent
+// loads in flight), then a fixed tree; k_frob_final adds the parts in order.
+__global__ __launch_bounds__(256) void k_frob_part(const ColDesc* __restrict__ descs,
+                                                   double* __restrict__ partial) {
     __shared__ double red[256];
-    const ColDesc D = descs[blockIdx.x];
-    double acc = 0.0;
-    for (int j = 0; j < D.ncol; ++j) {
-        const double* y = D.Y + (int64_t)j * D.ldy;
-        for (int i = threadIdx.x; i < D.nrow; i += 256) acc += y[i] * y[i];
+    const ColDesc D = descs[blockIdx.y];
+    const int q = blockIdx.x;
+    const int j0 = (int)((int64_t)q * D.ncol / FROB_PARTS);
+    const int j1 = (int)((int64_t)(q + 1) * D.ncol / FROB_PARTS);
+    gconst_double* Y = as_global(D.Y);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = j0; j < j1; ++j) {
+        gconst_double* y = Y + (int64_t)j * D.ldy;
+        int i = threadIdx.x;
+        for (; i + 768 < D.nrow; i += 1024) {
+            const double a = y[i], b = y[i + 256], c = y[i + 512], d = y[i + 768];
+            acc[0] += a * a;
+            acc[1] += b * b;
+            acc[2] += c * c;
+            acc[3] += d * d;
+        }
+        // fewer than 1024 rows left: at most three more
+        if (i < D.nrow) acc[0] += y[i] * y[i];
+        if (i + 256 < D.nrow) acc[1] += y[i + 256] * y[i + 256];
+        if (i + 512 < D.nrow) acc[2] += y[i + 512] * y[i + 512];
     }
-    double s = block_sum_fixed(acc, red);
-    if (threadIdx.x == 0) D.sigma[0] = s;
+    const double s = block_sum_fixed((acc[0] + acc[1]) + (acc[2] + acc[3]), red);
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * FROB_PARTS + q] = s;
+}
+
+__global__ __launch_bounds__(64) void k_frob_final(const ColDesc* __restrict__ descs, int nd,
+                                                   const double* __restrict__ partial) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nd) return;
+    double s = 0.0;
+    for (int q = 0; q < FROB_PARTS; ++q) s += partial[(int64_t)p * FROB_PARTS + q];
+    descs[p].sigma[0] = s;
 }
 
 // ================================================================ profiling
@@ -2607,10 +2637,12 @@ void launch_conv(hipStream_t st, const ConvDesc* d_descs, int nd, double null_rt
     TNR_HIP(hipGetLastError());
 }
 
-void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd) {
+void launch_frob(hipStream_t st, const ColDesc* d_descs, int nd, double* partial) {
     if (nd <= 0) return;
     ProfScope ps(st, PK_MISC, 0.0, 0.0);
-    TNR_LAUNCH(ps, k_frob, dim3(nd), dim3(256), 0, st, d_descs, nd);
+    ps.bracket();
+    TNR_LAUNCH(ps, k_frob_part, dim3(FROB_PARTS, nd), dim3(256), 0, st, d_descs, partial);
+    TNR_LAUNCH(ps, k_frob_final, dim3((nd + 63) / 64), dim3(64), 0, st, d_descs, nd, partial);
     TNR_HIP(hipGetLastError());
 }
 

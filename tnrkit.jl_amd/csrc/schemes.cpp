@@ -341,7 +341,8 @@ std::vector<EFactor> eigh_trunc_many(Ctx& ctx, const std::vector<BT*>& Ms,
             gather_many(ctx, gd);
             gemm_grouped(ctx, gj);
             ColDesc* dfd = ctx.stage->put(ctx.stream, fd);
-            launch_frob(ctx.stream, dfd, (int)fd.size());
+            Buf part(&ctx.pool, fd.size() * FROB_PARTS * sizeof(double));
+            launch_frob(ctx.stream, dfd, (int)fd.size(), part.d());
             std::vector<double> h(which.size());
             ctx.fetch(h.data(), fro.d(), h.size() * 8);
             for (size_t i = 0; i < which.size(); ++i) eps2[owner[which[i]].first] += h[i];
