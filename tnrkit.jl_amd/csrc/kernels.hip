@@ -11,6 +11,7 @@
 
 #include <hip/hip_ext.h>
 
+#include <condition_variable>
 #include <map>
 #include <mutex>
 
@@ -2212,7 +2213,15 @@ struct ProfState {
     std::mutex mu;
     int level = 0;
     struct Rec { hipEvent_t a, b; int kind; double flops, bytes; };
+    // a completed launch's times (ms): duration, start and end since `ref`
+    struct Times { float ms, t0, t1; };
     std::vector<Rec> recs;
+    bool draining = false;   // one thread files a batch of records unlocked
+    std::condition_variable idle;   // signalled when that batch is filed
+    // with mu held (through lk): wait until no batch is being filed unlocked
+    void wait_idle(std::unique_lock<std::mutex>& lk) {
+        idle.wait(lk, [this] { return !draining; });
+    }
     std::vector<hipEvent_t> spare;
     ProfTotals tot[PK_COUNT];
     // start / end of every filed launch (ms since `ref`), per kind: their
@@ -2237,33 +2246,29 @@ struct ProfState {
         TNR_HIP(hipEventCreate(&e));
         return e;
     }
-    void file(const Rec& r) {   // with mu held; r.b has completed
-        float ms = 0, t0 = 0, t1 = 0;
-        TNR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-        TNR_HIP(hipEventElapsedTime(&t0, ref, r.a));
-        TNR_HIP(hipEventElapsedTime(&t1, ref, r.b));
+    Times times(const Rec& r) const {   // r.b has completed; no lock needed
+        Times x{0, 0, 0};
+        TNR_HIP(hipEventElapsedTime(&x.ms, r.a, r.b));
+        TNR_HIP(hipEventElapsedTime(&x.t0, ref, r.a));
+        TNR_HIP(hipEventElapsedTime(&x.t1, ref, r.b));
+        return x;
+    }
+    void file_times(const Rec& r, const Times& x) {   // with mu held
         tot[r.kind].launches++;
-        tot[r.kind].ms += ms;
+        tot[r.kind].ms += x.ms;
         tot[r.kind].flops += r.flops;
         tot[r.kind].bytes += r.bytes;
-        spans[r.kind].emplace_back(t0, t1);
+        spans[r.kind].emplace_back(x.t0, x.t1);
         spare.push_back(r.a);
         spare.push_back(r.b);
     }
+    void file(const Rec& r) { file_times(r, times(r)); }   // with mu held; r.b has completed
     void drain() {      // with mu held; waits for every pending launch
         for (auto& r : recs) {
             TNR_HIP(hipEventSynchronize(r.b));
             file(r);
         }
         recs.clear();
-    }
-    void drain_ready() {   // with mu held; files the completed launches, never waits
-        size_t keep = 0;
-        for (size_t i = 0; i < recs.size(); ++i) {
-            if (hipEventQuery(recs[i].b) == hipSuccess) file(recs[i]);
-            else recs[keep++] = recs[i];
-        }
-        recs.resize(keep);
     }
 };
 ProfState& prof() {
@@ -2274,7 +2279,8 @@ ProfState& prof() {
 
 void prof_enable(int level) {
     ProfState& p = prof();
-    std::lock_guard<std::mutex> lk(p.mu);
+    std::unique_lock<std::mutex> lk(p.mu);
+    p.wait_idle(lk);
     p.drain();
     p.level = level;
     for (auto& t : p.tot) t = ProfTotals();
@@ -2343,24 +2349,43 @@ ProfScope::~ProfScope() {
     if (!active) return;
     end();
     ProfState& p = prof();
-    std::lock_guard<std::mutex> lk(p.mu);
+    std::unique_lock<std::mutex> lk(p.mu);
     if (mode == 0) {   // nothing was launched
         p.spare.push_back(ea);
         p.spare.push_back(eb);
         return;
     }
     p.recs.push_back(ProfState::Rec{ea, eb, kind, flops, bytes});
-    if (p.recs.size() > 4096) {
-        try {
-            p.drain_ready();
-        } catch (...) {
+    if (p.recs.size() <= 4096 || p.draining) return;
+    // File the completed launches.  The event queries take a few ms for
+    // thousands of records, so they run without the lock: the other lanes'
+    // threads keep launching meanwhile.
+    std::vector<ProfState::Rec> batch;
+    batch.swap(p.recs);
+    p.draining = true;
+    lk.unlock();
+    std::vector<std::pair<ProfState::Rec, ProfState::Times>> done;
+    std::vector<ProfState::Rec> pending;
+    try {
+        for (const auto& r : batch) {
+            if (hipEventQuery(r.b) == hipSuccess) done.emplace_back(r, p.times(r));
+            else pending.push_back(r);
         }
+    } catch (...) {
+        pending = batch;   // try again later
+        done.clear();
     }
+    lk.lock();
+    for (const auto& d : done) p.file_times(d.first, d.second);
+    p.recs.insert(p.recs.end(), pending.begin(), pending.end());
+    p.draining = false;
+    p.idle.notify_all();
 }
 
 ProfTotals prof_read(int kind) {
     ProfState& p = prof();
-    std::lock_guard<std::mutex> lk(p.mu);
+    std::unique_lock<std::mutex> lk(p.mu);
+    p.wait_idle(lk);
     p.drain();
     if (kind < 0 || kind >= PK_COUNT) return ProfTotals();
     ProfTotals t = p.tot[kind];
