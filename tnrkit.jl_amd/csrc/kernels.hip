@@ -510,9 +510,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
 // The same products as k_gemm_f64 (same MFMA sequence over k, same K-chunk
 // totals, so the same bits), with the operands going from global memory
 // straight into LDS (buffer_load_dwordx4 ... lds: lane l of a wave writes its
-// 16 bytes at the instruction's LDS base + 16 l) through three LDS stages:
-// the loads of k-tile t + 2 are in flight while k-tile t is multiplied, and
-// no registers hold tiles in transit.
+// 16 bytes at the instruction's LDS base + 16 l) through NS LDS stages (three
+// in every launch): the loads of k-tile t + NS - 1 are in flight while
+// k-tile t is multiplied, and no registers hold tiles in transit.
 //
 // Each operand's stage region takes one of two layouts, by storage order:
 //  * row layout (operand contiguous along its M / N index: A as stored, B
