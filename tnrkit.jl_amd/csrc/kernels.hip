@@ -529,49 +529,59 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(const GemmDesc* __res
 // only reach output rows (columns) that are not written.  k past K is zeroed
 // in LDS before the last k-tile is used, so padded products are 0 * 0 as in
 // k_gemm_f64.
-constexpr int DL_KB = 16;
+// (KB, the k-tile depth, is 16 or 32: KB / 2 k-pairs per index; the column
+// layout's XOR mask is (i >> 1) & 7 for 8 pairs and i & 15 for 16, either
+// way distinct over 16 consecutive i.  Every launch uses 16: a 32-deep
+// 128 x 128 tile on two stages needs more than 256 VGPRs and spills.)
+template <int KB>
+__host__ __device__ constexpr int dl_region(int d) { return KB * d + 16; }
 
-__host__ __device__ constexpr int dl_region(int d) { return 16 * d + 16; }
+template <int KB>
+__device__ __forceinline__ int dl_mask(int i) { return KB == 16 ? ((i >> 1) & 7) : (i & 15); }
 
 // LDS offset (doubles) of element (i, kk) of a D-wide operand region
+template <int KB>
 __device__ __forceinline__ int dl_off(bool rowl, int D, int i, int kk) {
-    if (rowl) return ((kk & 1) ? 8 * D + 16 : 0) + (kk >> 1) * D + i;
-    return (i * 8 + ((kk >> 1) ^ ((i >> 1) & 7))) * 2 + (kk & 1);
+    if (rowl) return ((kk & 1) ? (KB / 2) * D + 16 : 0) + (kk >> 1) * D + i;
+    return (i * (KB / 2) + ((kk >> 1) ^ dl_mask<KB>(i))) * 2 + (kk & 1);
 }
 
-// What DMA instruction u (of D / 8) of a D-wide operand region fetches for
-// this lane: its LDS base (doubles, uniform), the lane's element (i, k pair)
+// What DMA instruction u (of D KB / 128) of a D-wide operand region fetches
+// for this lane: its LDS base (doubles, uniform), the lane's element (i, k pair)
+template <int KB>
 __device__ __forceinline__ void dl_slot(bool rowl, int D, int u, int lane, int& base, int& i,
                                         int& kk0, int& kk1) {
     if (rowl) {
-        const int half = D / 16;       // instructions per row parity
+        const int half = (KB / 2) * D / 128;   // instructions per row parity
         const int odd = u >= half, uu = odd ? u - half : u;
         const int e = 128 * uu + 2 * lane;
-        base = (odd ? 8 * D + 16 : 0) + 128 * uu;
+        base = (odd ? (KB / 2) * D + 16 : 0) + 128 * uu;
         i = e % D;
         kk0 = kk1 = 2 * (e / D) + odd;
     } else {
         base = 128 * u;
-        i = 8 * u + (lane >> 3);
-        const int kp = (lane & 7) ^ ((i >> 1) & 7);
+        i = (128 / KB) * u + lane / (KB / 2);
+        const int kp = (lane % (KB / 2)) ^ dl_mask<KB>(i);
         kk0 = 2 * kp;
         kk1 = 2 * kp + 1;
     }
 }
 
-template <int BM, int BN, int NW>
+template <int BM, int BN, int NW, int KB>
 struct DlPlan {
-    static constexpr int SA = dl_region(BM), SB = dl_region(BN), ST = SA + SB;
-    static constexpr int QA = BM / 8, Q = (BM + BN) / 8, PER = Q / NW;   // DMA instructions
+    static constexpr int SA = dl_region<KB>(BM), SB = dl_region<KB>(BN), ST = SA + SB;
+    // DMA instructions (1 KB each) per stage: for A, for both, per wave
+    static constexpr int QA = BM * KB / 128, Q = (BM + BN) * KB / 128, PER = Q / NW;
     static_assert(Q % NW == 0, "DMA instructions must divide over the waves");
 };
 
 // LDS-DMA for one k-tile: this wave's PER instructions into `stage`
-template <int BM, int BN, int NW>
+template <int BM, int BN, int NW, int KB>
 __device__ __forceinline__ void dl_issue(double* stage, __amdgpu_buffer_rsrc_t ra,
-                                         __amdgpu_buffer_rsrc_t rb, const int (&lbase)[DlPlan<BM, BN, NW>::PER],
-                                         const int (&vofs)[DlPlan<BM, BN, NW>::PER], int w) {
-    using P = DlPlan<BM, BN, NW>;
+                                         __amdgpu_buffer_rsrc_t rb,
+                                         const int (&lbase)[DlPlan<BM, BN, NW, KB>::PER],
+                                         const int (&vofs)[DlPlan<BM, BN, NW, KB>::PER], int w) {
+    using P = DlPlan<BM, BN, NW, KB>;
 #pragma unroll
     for (int r = 0; r < P::PER; ++r) {
         const int q = w + NW * r;
@@ -593,16 +603,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t dl_rsrc(const double* base, in
 // for a stage's loads does not include the other stages'); OCC waves per
 // SIMD: 4 lets two 128x64 workgroups share a CU (three stages: 73 KB of LDS
 // each), 3 fits three 64x64 ones (49 KB each)
-template <int BM, int BN, int WM, int WN, int OCC, int NS>
+template <int BM, int BN, int WM, int WN, int OCC, int NS, int KB>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* flop_ctr) {
-    static_assert(NS == 3 || NS == 4, "three or four LDS stages");
+    static_assert(NS >= 2 && NS <= 4, "two to four LDS stages");
+    static_assert(KB == 16 || KB == 32, "16- or 32-deep k-tiles");
     constexpr int NW = WM * WN;
-    using P = DlPlan<BM, BN, NW>;
+    using P = DlPlan<BM, BN, NW, KB>;
     constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
     __shared__ __align__(16) double S0[P::ST];
     __shared__ __align__(16) double S1[P::ST];
-    __shared__ __align__(16) double S2[P::ST];
+    __shared__ __align__(16) double S2[NS > 2 ? P::ST : 2];
     __shared__ __align__(16) double S3[NS > 3 ? P::ST : 2];
     double* const SS[4] = {S0, S1, S2, S3};
     // the wave index is uniform: keep it (and the LDS bases derived from it)
@@ -624,11 +635,11 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
         const int q = w + NW * r;
         int base, i, kk0, kk1;
         if (q < P::QA) {
-            dl_slot(arow, BM, q, lane, base, i, kk0, kk1);
+            dl_slot<KB>(arow, BM, q, lane, base, i, kk0, kk1);
             lbase[r] = __builtin_amdgcn_readfirstlane(base);
             vofs[r] = 8 * (arow ? i + kk0 * g.lda : kk0 + i * g.lda);
         } else {
-            dl_slot(brow, BN, q - P::QA, lane, base, i, kk0, kk1);
+            dl_slot<KB>(brow, BN, q - P::QA, lane, base, i, kk0, kk1);
             lbase[r] = __builtin_amdgcn_readfirstlane(P::SA + base);
             vofs[r] = 8 * (brow ? i + kk0 * g.ldb : kk0 + i * g.ldb);
         }
@@ -645,15 +656,16 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
         return dl_rsrc(g.B + o, extB - o);
     };
     // MFMA operand offsets for each 4-deep k-step (row i / column j at mi,
-    // ni = 0; +16 rows is +16 doubles in the row layout, +256 in the column one)
-    int aoff[DL_KB / 4], boff[DL_KB / 4];
+    // ni = 0; +16 rows is +16 doubles in the row layout, +16 KB doubles in
+    // the column one)
+    int aoff[KB / 4], boff[KB / 4];
 #pragma unroll
-    for (int ks = 0; ks < DL_KB / 4; ++ks) {
+    for (int ks = 0; ks < KB / 4; ++ks) {
         const int kk = ks * 4 + (lane >> 4);
-        aoff[ks] = dl_off(arow, BM, wm0 + (lane & 15), kk);
-        boff[ks] = P::SA + dl_off(brow, BN, wn0 + (lane & 15), kk);
+        aoff[ks] = dl_off<KB>(arow, BM, wm0 + (lane & 15), kk);
+        boff[ks] = P::SA + dl_off<KB>(brow, BN, wn0 + (lane & 15), kk);
     }
-    const int astep = arow ? 16 : 256, bstep = brow ? 16 : 256;
+    const int astep = arow ? 16 : 16 * KB, bstep = brow ? 16 : 16 * KB;
 
     dbl4 acc[MI][NI], tot[MI][NI];
 #pragma unroll
@@ -663,11 +675,11 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
             acc[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
             tot[mi][ni] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
-    const int nt = (tc.kend - tc.kbeg + DL_KB - 1) / DL_KB;
+    const int nt = (tc.kend - tc.kbeg + KB - 1) / KB;
 #pragma unroll
     for (int s = 0; s < NS - 1; ++s)
         if (s < nt)
-            dl_issue<BM, BN, NW>(SS[s], rsrc_a(tc.kbeg + s * DL_KB), rsrc_b(tc.kbeg + s * DL_KB),
+            dl_issue<BM, BN, NW, KB>(SS[s], rsrc_a(tc.kbeg + s * KB), rsrc_b(tc.kbeg + s * KB),
                                  lbase, vofs, w);
 
     // one k-tile: wait for this wave's loads of tile it, zero k >= K on the
@@ -675,19 +687,19 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
     // wave is done with the stage tile it + NS - 1 goes into), issue tile
     // it + NS - 1, multiply
     auto step = [&](double* cur, double* nxt, int it) {
-        const int k0 = tc.kbeg + it * DL_KB;
+        const int k0 = tc.kbeg + it * KB;
         const int ahead = min(NS - 2, nt - 1 - it);   // tiles issued after this one
         if (NS > 3 && ahead >= 2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * P::PER) : "memory");
         else if (ahead >= 1) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(P::PER) : "memory");
         else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        if (k0 + DL_KB > g.K) {
+        if (k0 + KB > g.K) {
             const int kv = g.K - k0;
 #pragma unroll
             for (int r = 0; r < P::PER; ++r) {
                 const int q = w + NW * r;
                 int base, i, kk0, kk1;
-                if (q < P::QA) dl_slot(arow, BM, q, lane, base, i, kk0, kk1);
-                else dl_slot(brow, BN, q - P::QA, lane, base, i, kk0, kk1);
+                if (q < P::QA) dl_slot<KB>(arow, BM, q, lane, base, i, kk0, kk1);
+                else dl_slot<KB>(brow, BN, q - P::QA, lane, base, i, kk0, kk1);
                 double* d = cur + lbase[r] + 2 * lane;
                 if (kk0 >= kv) d[0] = 0.0;
                 if (kk1 >= kv) d[1] = 0.0;
@@ -695,7 +707,7 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
         }
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
         if (it + NS - 1 < nt)
-            dl_issue<BM, BN, NW>(nxt, rsrc_a(k0 + (NS - 1) * DL_KB), rsrc_b(k0 + (NS - 1) * DL_KB),
+            dl_issue<BM, BN, NW, KB>(nxt, rsrc_a(k0 + (NS - 1) * KB), rsrc_b(k0 + (NS - 1) * KB),
                                  lbase, vofs, w);
         __builtin_amdgcn_sched_barrier(0);
         double a[2][MI], b[2][NI];
@@ -704,9 +716,9 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
 #pragma unroll
         for (int ni = 0; ni < NI; ++ni) b[0][ni] = cur[boff[0] + ni * bstep];
 #pragma unroll
-        for (int ks = 0; ks < DL_KB / 4; ++ks) {
+        for (int ks = 0; ks < KB / 4; ++ks) {
             const int c = ks & 1;
-            if (ks + 1 < DL_KB / 4) {
+            if (ks + 1 < KB / 4) {
 #pragma unroll
                 for (int mi = 0; mi < MI; ++mi) a[c ^ 1][mi] = cur[aoff[ks + 1] + mi * astep];
 #pragma unroll
@@ -719,7 +731,7 @@ void k_gemm_dl(const GemmDesc* __restrict__ descs, int nd, unsigned long long* f
                     acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][mi], b[c][ni],
                                                                        acc[mi][ni], 0, 0, 0);
         }
-        if ((k0 + DL_KB) % g.kchunk == 0 || it + 1 == nt) {
+        if ((k0 + KB) % g.kchunk == 0 || it + 1 == nt) {
 #pragma unroll
             for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -2458,13 +2470,13 @@ void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles,
     if (split32)
         TNR_LAUNCH(ps, k_gemm_split32, dim3((unsigned)tiles), dim3(256), 0, st, d_descs, nd, fc);
     else if (shape == GEMM_128x128 && dma)
-        TNR_LAUNCH(ps, (k_gemm_dl<128, 128, 4, 2, 2, 3>), dim3((unsigned)tiles), dim3(512), 0, st,
+        TNR_LAUNCH(ps, (k_gemm_dl<128, 128, 4, 2, 2, 3, 16>), dim3((unsigned)tiles), dim3(512), 0, st,
                            d_descs, nd, fc);
     else if (shape == GEMM_128x64 && dma)
-        TNR_LAUNCH(ps, (k_gemm_dl<128, 64, 4, 2, 4, 3>), dim3((unsigned)tiles), dim3(512), 0, st,
+        TNR_LAUNCH(ps, (k_gemm_dl<128, 64, 4, 2, 4, 3, 16>), dim3((unsigned)tiles), dim3(512), 0, st,
                            d_descs, nd, fc);
     else if (shape == GEMM_64x64 && dma)
-        TNR_LAUNCH(ps, (k_gemm_dl<64, 64, 2, 2, 3, 3>), dim3((unsigned)tiles), dim3(256), 0, st,
+        TNR_LAUNCH(ps, (k_gemm_dl<64, 64, 2, 2, 3, 3, 16>), dim3((unsigned)tiles), dim3(256), 0, st,
                            d_descs, nd, fc);
     else if (shape == GEMM_128x128)
         TNR_LAUNCH(ps, (k_gemm_f64<128, 128, 4, 2, 16>), dim3((unsigned)tiles), dim3(512), 0,
