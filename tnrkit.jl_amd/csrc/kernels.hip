@@ -862,22 +862,46 @@ __global__ __launch_bounds__(256) void k_gemm_split32(const GemmDesc* __restrict
 __global__ __launch_bounds__(256) void k_gemm_reduce(const GemmDesc* __restrict__ descs, int nd,
                                                      const int64_t* __restrict__ starts,
                                                      int64_t total) {
+    // A thread's elements increase, so its job index only moves forward: the
+    // search continues from the last job (most steps stay in it or move to
+    // the next), and that job's fields stay in registers until it changes.
+    int lo = 0, cur = -1;
+    const double* work = nullptr;
+    double* C = nullptr;
+    int64_t mn = 0, base = 0;
+    int M = 1, ldc = 1, ks = 1;
+    double alpha = 1.0, beta = 0.0;
+    bool skip = false;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
-        int lo = 0, hi = nd - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (starts[mid] <= e) lo = mid; else hi = mid - 1;
+        if (lo + 1 < nd && starts[lo + 1] <= e) {
+            int hi = nd - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (starts[mid] <= e) lo = mid; else hi = mid - 1;
+            }
         }
-        const GemmDesc& g = descs[lo];
-        if (g.skip && *g.skip) continue;
-        int64_t r = e - starts[lo];
-        int64_t mn = (int64_t)g.M * g.N;
+        if (lo != cur) {
+            cur = lo;
+            const GemmDesc& g = descs[lo];
+            skip = g.skip && *g.skip;
+            work = g.work;
+            C = g.C;
+            M = g.M;
+            ldc = g.ldc;
+            ks = g.ksplit;
+            alpha = g.alpha;
+            beta = g.beta;
+            mn = (int64_t)g.M * g.N;
+            base = starts[lo];
+        }
+        if (skip) continue;
+        const int64_t r = e - base;
         double s = 0.0;
-        for (int z = 0; z < g.ksplit; ++z) s += g.work[z * mn + r];
-        int row = (int)(r % g.M), col = (int)(r / g.M);
-        double* p = g.C + row + (int64_t)col * g.ldc;
-        *p = (g.beta == 0.0) ? g.alpha * s : g.alpha * s + g.beta * *p;
+        for (int z = 0; z < ks; ++z) s += work[z * mn + r];
+        const int row = (int)(r % M), col = (int)(r / M);
+        double* p = C + row + (int64_t)col * ldc;
+        *p = (beta == 0.0) ? alpha * s : alpha * s + beta * *p;
     }
 }
 
