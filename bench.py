@@ -658,9 +658,10 @@ def workload_c4(tk, n, maxiter, check):
         fp = tk.free_energy(d, float(betas[i]))
         for k in errs:
             errs[k].append(abs(fs[k][i] - fp) / abs(fp))
-    out["plain_fp64_one_at_a_time_per_s"] = len(idx) / t_plain
-    for k in errs:
-        out[k]["max_rel_df_vs_plain_fp64"] = max(errs[k])
+    if idx:   # --check 0 runs no plain reference
+        out["plain_fp64_one_at_a_time_per_s"] = len(idx) / t_plain
+        for k in errs:
+            out[k]["max_rel_df_vs_plain_fp64"] = max(errs[k])
     return out
 
 
