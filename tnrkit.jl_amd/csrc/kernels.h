@@ -221,7 +221,6 @@ void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const i
 void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const TraceProb* d_probs,
                             int nprob, int64_t max_size);
 size_t jacobi_lds_bytes(int n);
-size_t cholqr_lds_bytes(int n);
 // pivoted Cholesky QR of Gram matrices (JacobiDesc: G in, W = P R^{-1} out,
 // lam = R_jj); columns below null_rel2 * max diag are dropped
 void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmin, int nmax,

@@ -1245,9 +1245,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
 }
 
 // ------------------------------------------------ pivoted Cholesky QR
-// One 256-thread workgroup per n x n Gram matrix G = Y^T Y (n <= LMAX;
-// measured at 64 x 64: 178 us with 256 threads, 220 us with 128, 333 us with
-// one wavefront, 236 us with 1024):
+// One workgroup per n x n Gram matrix G = Y^T Y (n <= LMAX):
 // diagonal-pivoted Cholesky G[p, p] = R^T R, stopped where the largest
 // remaining diagonal falls to null_rel2 * max diag(G) (numerical rank r),
 // then X = R^{-1} (r x r).  Output W = P X (W[perm[i], j] = X[i, j], zero
@@ -1255,171 +1253,22 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
 // span(Y), ordered by decreasing pivot; lam[j] = R[j][j] (the norm of
 // column j of Y after projecting out the earlier pivots).  Ties in the
 // pivot search go to the lowest index, so the result is a function of G.
-constexpr int CQT = 256;
+// k_cholqr64 takes n <= 64 in one wavefront, k_cholqr96 the larger sizes in
+// four; both keep the working matrix in registers.
 constexpr int CQ64 = 64;   // Gram matrices up to this size go to k_cholqr64
 
-__device__ __forceinline__ void argmax_wave(double& v, int& i) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double v2 = __shfl_xor(v, o, 64);
-        const int i2 = __shfl_xor(i, o, 64);
-        if (v2 > v || (v2 == v && i2 < i)) {
-            v = v2;
-            i = i2;
-        }
-    }
-}
-
-__global__ __launch_bounds__(CQT) void k_cholqr(const JacobiDesc* __restrict__ descs, int nd,
-                                                double null_rel2) {
-    extern __shared__ double lds[];
-    __shared__ double bv[CQT / 64];
-    __shared__ int bi[CQT / 64];
-    __shared__ double dmax_s;
-    const JacobiDesc D = descs[blockIdx.x];
-    if (D.n <= CQ64 || (D.skip && *D.skip)) return;
-    const int n = D.n, ld = n + 1, t = threadIdx.x, lane = t & 63, w = t >> 6;
-    double* A = lds;                   // symmetric working copy; R in its upper triangle
-    double* X = A + n * ld;            // R^{-1}
-    int* perm = (int*)(X + n * ld);
-    for (int e = t; e < n * n; e += CQT) {
-        const int j = e / n, i = e - j * n;
-        A[i + j * ld] = 0.5 * (D.G[i + (int64_t)j * D.ldg] + D.G[j + (int64_t)i * D.ldg]);
-        X[i + j * ld] = 0.0;
-    }
-    for (int i = t; i < n; i += CQT) perm[i] = i;
-    __syncthreads();
-    {
-        double v = 0.0;
-        int ii = t;
-        for (int c = t; c < n; c += CQT) v = fmax(v, A[c + c * ld]);
-        argmax_wave(v, ii);
-        if (lane == 0) bv[w] = v;
-        __syncthreads();
-        if (t == 0) {
-            double m = bv[0];
-            for (int q = 1; q < CQT / 64; ++q) m = fmax(m, bv[q]);
-            dmax_s = m;
-        }
-        __syncthreads();
-    }
-    const double nfloor = null_rel2 * dmax_s;
-    int rank = n;
-    for (int j = 0; j < n; ++j) {
-        double v = -1.0;
-        int ii = n;
-        for (int c = j + t; c < n; c += CQT) {
-            const double dc = A[c + c * ld];
-            if (dc > v) {
-                v = dc;
-                ii = c;
-            }
-        }
-        argmax_wave(v, ii);
-        if (lane == 0) {
-            bv[w] = v;
-            bi[w] = ii;
-        }
-        __syncthreads();
-        // every thread reduces the per-wave candidates itself (same order,
-        // same pivot everywhere); bv / bi are next written after this step's
-        // last barrier
-        int p;
-        {
-            double b = bv[0];
-            int bj = bi[0];
-            for (int q = 1; q < CQT / 64; ++q)
-                if (bv[q] > b || (bv[q] == b && bi[q] < bj)) {
-                    b = bv[q];
-                    bj = bi[q];
-                }
-            p = (b > nfloor && dmax_s > 0.0) ? bj : -1;
-        }
-        if (p < 0) {
-            rank = j;
-            break;
-        }
-        if (p != j) {   // symmetric exchange of indices j and p
-            for (int k = t; k < n; k += CQT) {
-                if (k == j || k == p) continue;
-                const double a = A[j + k * ld], b = A[p + k * ld];
-                A[j + k * ld] = b;
-                A[p + k * ld] = a;
-                const double c = A[k + j * ld], d = A[k + p * ld];
-                A[k + j * ld] = d;
-                A[k + p * ld] = c;
-            }
-            if (t == 0) {
-                const double a = A[j + j * ld];
-                A[j + j * ld] = A[p + p * ld];
-                A[p + p * ld] = a;
-                const int q = perm[j];
-                perm[j] = perm[p];
-                perm[p] = q;
-            }
-            __syncthreads();
-        }
-        // A[j, j] is read by every thread here and overwritten only after
-        // the next barrier; the trailing update does not read it
-        const double r = sqrt(A[j + j * ld]);
-        for (int k = j + 1 + t; k < n; k += CQT) A[j + k * ld] /= r;
-        __syncthreads();
-        if (t == 0) A[j + j * ld] = r;
-        // trailing update: rows across the lanes, columns k = j+1+w (mod 4)
-        // per wave, four columns per pass so their loads overlap
-        constexpr int WS = CQT / 64;
-        for (int i = j + 1 + lane; i < n; i += 64) {
-            const double aji = A[j + i * ld];
-            int k = j + 1 + w;
-            for (; k + 3 * WS < n; k += 4 * WS) {
-                const double b0 = A[j + k * ld], b1 = A[j + (k + WS) * ld];
-                const double b2 = A[j + (k + 2 * WS) * ld], b3 = A[j + (k + 3 * WS) * ld];
-                const double x0 = A[i + k * ld], x1 = A[i + (k + WS) * ld];
-                const double x2 = A[i + (k + 2 * WS) * ld], x3 = A[i + (k + 3 * WS) * ld];
-                A[i + k * ld] = x0 - aji * b0;
-                A[i + (k + WS) * ld] = x1 - aji * b1;
-                A[i + (k + 2 * WS) * ld] = x2 - aji * b2;
-                A[i + (k + 3 * WS) * ld] = x3 - aji * b3;
-            }
-            for (; k < n; k += WS) A[i + k * ld] -= aji * A[j + k * ld];
-        }
-        __syncthreads();
-    }
-    // X = R^{-1}: solve R X = I by right-looking elimination from the last
-    // row up; each step is parallel over the columns (waves) and rows (lanes)
-    for (int c = t; c < rank; c += CQT) X[c + c * ld] = 1.0;
-    __syncthreads();
-    for (int kk = rank - 1; kk >= 0; --kk) {
-        const double rkk = A[kk + kk * ld];
-        for (int c = kk + t; c < rank; c += CQT) X[kk + c * ld] /= rkk;
-        __syncthreads();
-        for (int c = kk + w; c < rank; c += CQT / 64) {
-            const double xk = X[kk + c * ld];
-            for (int i = lane; i < kk; i += 64) X[i + c * ld] -= A[i + kk * ld] * xk;
-        }
-        __syncthreads();
-    }
-    for (int e = t; e < n * n; e += CQT) {
-        const int j = e / n, i = e - j * n;
-        const double x = (j < rank && i <= j) ? X[i + j * ld] : 0.0;
-        D.W[perm[i] + (int64_t)j * D.ldw] = x;
-    }
-    for (int j = t; j < n; j += CQT) D.lam[j] = (j < rank) ? A[j + j * ld] : 0.0;
-    if (t == 0 && D.sweeps) *D.sweeps = 0;
-}
-
-// Pivoted Cholesky QR of Gram matrices with at most CQ64 = 64 columns: the
-// algorithm of k_cholqr (same pivot rule and null floor, the same output)
-// in one wavefront that keeps the working matrix in registers, lane i
+// Pivoted Cholesky QR of Gram matrices with at most CQ64 = 64 columns in
+// one wavefront that keeps the working matrix in registers, lane i
 // holding row i, so a pivot step has one wave-local barrier and no LDS
 // round trip per element.  Pivoting does not move data: the pivot's row is
 // broadcast through LDS and rows already pivoted drop out of the search.
 // Step j with pivot p, r = sqrt(A_j[p][p]): R[j][i] = A_j[p][i] / r (kept
 // over the original column indices), A_{j+1}[i][k] = A_j[i][k] -
 // (A_j[i][p] / r^2) A_j[p][k].  X = R^{-1} (R in pivot order) is solved
-// column by column, lane c holding column c in registers, by the
-// right-looking elimination of k_cholqr with the reciprocal diagonal.
-// Ties in the pivot search go to the lowest row index.
+// column by column, lane c holding column c in registers, by right-looking
+// elimination: rows kk from the last up, each scaled by 1 / R[kk][kk] and
+// eliminated from the rows above it.  Ties in the pivot search go to the
+// lowest row index.
 constexpr int CQLD = CQ64 + 1;   // padded stride of the LDS matrices
 
 size_t cholqr64_lds_bytes() {
@@ -1591,6 +1440,226 @@ __global__ __launch_bounds__(CQ64) __attribute__((amdgpu_waves_per_eu(1, 1))) vo
                 (ip >= 0 && ip <= c && c < rank) ? Rr[ip + c * CQLD] : 0.0;
     if (live) D.lam[lane] = lane < rank ? rdiag[lane] : 0.0;
     if (lane == 0 && D.sweeps) *D.sweeps = 0;
+}
+
+// Pivoted Cholesky QR of Gram matrices with CQ64 < n <= LMAX columns: the
+// arithmetic of k_cholqr64 over four wavefronts.  Thread (w, lane) holds
+// row i = lane + 64 (w & 1) of the working matrix, columns CQH (w >> 1) to
+// CQH (w >> 1) + CQH - 1, in registers.  At each pivot step the best
+// remaining row of each row-wave (w & 1) publishes itself in LDS (buffers
+// alternate by step parity), so a step takes one workgroup barrier, after
+// which every thread picks the larger of the two candidates (ties to the
+// lower index) and reads its row.  X = R^{-1} is solved with thread (w,
+// lane) holding column c = lane + 64 (w & 1), rows CQH (w >> 1) to
+// CQH (w >> 1) + CQH - 1: the waves of the lower rows finish them first and
+// hand them to the upper rows through LDS, so every element takes the same
+// updates in the same order as in a one-thread right-looking elimination.
+constexpr int CQH = LMAX / 2;     // columns (factorisation) / rows (inverse) per thread
+constexpr int CQLDM = LMAX + 1;   // padded stride of the LDS matrices
+constexpr int CQ96T = 256;
+static_assert(LMAX * LMAX % (4 * CQ96T) == 0 && CQH % 2 == 0 && LMAX <= 2 * 64,
+              "k_cholqr96 tiling");
+
+size_t cholqr96_lds_bytes() {
+    return 2 * (size_t)LMAX * CQLDM * sizeof(double)   // G staging / pivot-ordered R; R rows / X
+           + 4 * (size_t)LMAX * sizeof(double)         // candidate rows [step parity][row-wave]
+           + 4 * sizeof(double)                        // candidate values
+           + 2 * (size_t)LMAX * sizeof(double)         // R[j][j], 1 / R[j][j]
+           + 4 * sizeof(int)                           // candidate rows' indices
+           + 2 * (size_t)LMAX * sizeof(int);           // perm, inverse perm
+}
+
+__global__ __launch_bounds__(CQ96T) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_cholqr96(
+    const JacobiDesc* __restrict__ descs, int nd, double null_rel2) {
+    extern __shared__ double lds[];
+    const JacobiDesc D = descs[blockIdx.x];
+    if (D.n <= CQ64 || D.n > LMAX || (D.skip && *D.skip)) return;
+    const int n = D.n, t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6), rw = w & 1, h = w >> 1;
+    const int i = lane + 64 * rw;    // row (factorisation), column (inverse)
+    const int ii = min(i, n - 1);    // in-bounds stand-in for rows past n
+    const bool live = i < n;
+    double* Gs = lds;                          // G staging; later R[r][kk] at Gs[kk * LMAX + r]
+    double* Rr = Gs + LMAX * CQLDM;            // R[j][k] by original column k; later X
+    double* cand = Rr + LMAX * CQLDM;          // [2][2][LMAX]
+    double* candv = cand + 4 * LMAX;           // [2][2]
+    double* rdiag = candv + 4;
+    double* rinv = rdiag + LMAX;
+    int* candp = (int*)(rinv + LMAX);          // [2][2]
+    int* perm = candp + 4;
+    int* iperm = perm + LMAX;
+    // G into LDS, zero outside n x n (clamped loads, masked bits, so the
+    // loads are not predicated and stay in flight together)
+    for (int e0 = 0; e0 < LMAX * LMAX; e0 += 4 * CQ96T) {
+        long long g[4];
+        int at[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * CQ96T + t, c = e / LMAX, r = e - c * LMAX;
+            g[u] = __double_as_longlong(D.G[min(r, n - 1) + (int64_t)min(c, n - 1) * D.ldg]);
+            g[u] &= (r < n && c < n) ? -1LL : 0LL;
+            at[u] = r + c * CQLDM;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) Gs[at[u]] = __longlong_as_double(g[u]);
+    }
+    if (t < LMAX) iperm[t] = -1;
+    __syncthreads();
+    double a[CQH];
+#pragma unroll
+    for (int k = 0; k < CQH; ++k) {
+        const int c = h * CQH + k;
+        a[k] = 0.5 * (Gs[ii + c * CQLDM] + Gs[c + ii * CQLDM]);
+    }
+    double d = live ? Gs[ii + ii * CQLDM] : 0.0;
+    {
+        // largest diagonal over both row-waves, through the parity-1 value
+        // slots (first rewritten at step 1, after the step-0 barrier)
+        double dm = d;
+        int di = i;
+        argmax_wave64(dm, di);
+        if (h == 0 && lane == 0) candv[2 + rw] = dm;
+    }
+    __syncthreads();
+    const double dmax = candv[3] > candv[2] ? candv[3] : candv[2];
+    const double nfloor = null_rel2 * dmax;
+    bool done = !live;
+    int rank = n;
+    double v = live ? d : -1.0;
+    int p = live ? i : n;
+    argmax_wave64(v, p);
+    for (int j = 0; j < n; ++j) {
+        const int b = j & 1;
+        double* cb = cand + b * 2 * LMAX;
+        if (p < n && i == p) {
+            double* dst = cb + rw * LMAX + h * CQH;
+#pragma unroll
+            for (int k = 0; k < CQH; k += 2) *(double2*)(dst + k) = make_double2(a[k], a[k + 1]);
+        }
+        if (h == 0 && lane == 0) {
+            candv[2 * b + rw] = v;
+            candp[2 * b + rw] = p;
+        }
+        __syncthreads();
+        // buffers of parity b are next written at step j + 2, after the
+        // step j + 1 barrier, which every thread reaches only once it has
+        // read them here
+        const double v0 = candv[2 * b], v1 = candv[2 * b + 1];
+        const int p0 = candp[2 * b], p1 = candp[2 * b + 1];
+        const bool s1 = (v1 > v0) | ((v1 == v0) & (p1 < p0));
+        const double vb = s1 ? v1 : v0;
+        const int pb = s1 ? p1 : p0;
+        if (!(vb > nfloor && dmax > 0.0)) {
+            rank = j;
+            break;
+        }
+        const double* pr = cb + (s1 ? LMAX : 0);
+        const double r = sqrt(vb), ri = 1.0 / r;
+        const double aip = pr[ii];   // A_j[p][i] = A_j[i][p]
+        if (h == 0 && live) Rr[j * CQLDM + i] = i == pb ? r : aip * ri;
+        if (t == 0) {
+            perm[j] = pb;
+            rdiag[j] = r;
+            rinv[j] = ri;
+        }
+        if (i == pb) done = true;
+        const double f = done ? 0.0 : (aip * ri) * ri;
+        d = d - f * aip;
+        v = done ? -1.0 : d;
+        p = done ? n : i;
+        double2 row[CQH / 2];
+#pragma unroll
+        for (int k = 0; k < CQH / 2; ++k) row[k] = *(const double2*)(pr + h * CQH + 2 * k);
+        argmax_wave64(v, p);
+#pragma unroll
+        for (int k = 0; k < CQH / 2; ++k) {
+            a[2 * k] = a[2 * k] - f * row[k].x;
+            a[2 * k + 1] = a[2 * k + 1] - f * row[k].y;
+        }
+    }
+    __syncthreads();
+    // R in pivot order, column kk contiguous: Rp[kk * LMAX + r] = R[r][kk]
+    double* Rp = Gs;
+    for (int e = t; e < rank * LMAX; e += CQ96T) {
+        const int kk = e / LMAX, r = e - kk * LMAX;
+        Rp[e] = r < rank ? Rr[r * CQLDM + perm[kk]] : 0.0;
+    }
+    if (t < rank) iperm[perm[t]] = t;
+    __syncthreads();
+    // X = R^{-1}, X[r][c] staged at X[r + c * CQLDM]
+    double* X = Rr;
+    const int c = i, cc = min(c, LMAX - 1);
+    double x[CQH];
+#pragma unroll
+    for (int k = 0; k < CQH; ++k) x[k] = (h * CQH + k == c && c < rank) ? 1.0 : 0.0;
+    if (h == 1) {
+        // rows CQH.. : rows kk from the last up, scaled and eliminated from
+        // the rows above them within this half
+#pragma unroll
+        for (int kk = LMAX - 1; kk >= CQH; --kk) {
+            if (kk < rank) {
+                const int q = kk - CQH;
+                x[q] *= rinv[kk];
+                const double* col = Rp + kk * LMAX + CQH;
+#pragma unroll
+                for (int m = 0; m + 1 < q; m += 2) {
+                    const double2 rr = *(const double2*)(col + m);
+                    x[m] = x[m] - rr.x * x[q];
+                    x[m + 1] = x[m + 1] - rr.y * x[q];
+                }
+                if (q & 1) x[q - 1] = x[q - 1] - col[q - 1] * x[q];
+            }
+        }
+        if (c < n)
+#pragma unroll
+            for (int k = 0; k < CQH; ++k) X[CQH + k + c * CQLDM] = x[k];
+    }
+    __syncthreads();
+    if (h == 0) {
+        // rows below CQH: first the eliminations of rows CQH.. (final now),
+        // in the same descending order, then their own
+#pragma unroll
+        for (int kk = LMAX - 1; kk >= CQH; --kk) {
+            if (kk < rank) {
+                const double xk = X[kk + cc * CQLDM];
+                const double* col = Rp + kk * LMAX;
+#pragma unroll
+                for (int m = 0; m < CQH; m += 2) {
+                    const double2 rr = *(const double2*)(col + m);
+                    x[m] = x[m] - rr.x * xk;
+                    x[m + 1] = x[m + 1] - rr.y * xk;
+                }
+            }
+        }
+#pragma unroll
+        for (int kk = CQH - 1; kk >= 0; --kk) {
+            if (kk < rank) {
+                x[kk] *= rinv[kk];
+                const double* col = Rp + kk * LMAX;
+#pragma unroll
+                for (int m = 0; m + 1 < kk; m += 2) {
+                    const double2 rr = *(const double2*)(col + m);
+                    x[m] = x[m] - rr.x * x[kk];
+                    x[m + 1] = x[m + 1] - rr.y * x[kk];
+                }
+                if (kk & 1) x[kk - 1] = x[kk - 1] - col[kk - 1] * x[kk];
+            }
+        }
+        if (c < n)
+#pragma unroll
+            for (int k = 0; k < CQH; ++k) X[k + c * CQLDM] = x[k];
+    }
+    __syncthreads();
+    // W[perm[r]][cj] = X[r][cj] for r <= cj < rank, zero elsewhere; one
+    // coalesced column per wave at a time
+    for (int cj = w; cj < n; cj += CQ96T / 64)
+        for (int r = lane; r < n; r += 64) {
+            const int ip = iperm[r];
+            D.W[r + (int64_t)cj * D.ldw] =
+                (ip >= 0 && ip <= cj && cj < rank) ? X[ip + cj * CQLDM] : 0.0;
+        }
+    if (t < n) D.lam[t] = t < rank ? rdiag[t] : 0.0;
+    if (t == 0 && D.sweeps) *D.sweeps = 0;
 }
 
 // ------------------------------------------------- column norms / normalise
@@ -2582,19 +2651,16 @@ void launch_jacobi(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmax, 
     }
 }
 
-size_t cholqr_lds_bytes(int n) {
-    return 2 * (size_t)n * (n + 1) * sizeof(double) + (size_t)n * sizeof(int);
-}
 
 // Problems of at most CQ64 columns run on k_cholqr64, larger ones on
-// k_cholqr (each kernel skips the other's), so the kernel a problem gets
+// k_cholqr96 (each kernel skips the other's), so the kernel a problem gets
 // depends on its own size only, never on the rest of its launch group.
 void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmin, int nmax,
                    double null_rel2) {
     if (nd <= 0) return;
     static const hipError_t attr = hipFuncSetAttribute(
-        (const void*)k_cholqr, hipFuncAttributeMaxDynamicSharedMemorySize,
-        (int)cholqr_lds_bytes(LMAX));
+        (const void*)k_cholqr96, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (int)cholqr96_lds_bytes());
     TNR_HIP(attr);
     static const hipError_t attr64 = hipFuncSetAttribute(
         (const void*)k_cholqr64, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2608,7 +2674,7 @@ void launch_cholqr(hipStream_t st, const JacobiDesc* d_descs, int nd, int nmin, 
     }
     if (nmax > CQ64) {
         ProfScope ps(st, PK_JACOBI, 0.0, 0.0);
-        TNR_LAUNCH(ps, k_cholqr, dim3(nd), dim3(CQT), cholqr_lds_bytes(nmax), st, d_descs, nd,
+        TNR_LAUNCH(ps, k_cholqr96, dim3(nd), dim3(CQ96T), cholqr96_lds_bytes(), st, d_descs, nd,
                    null_rel2);
         TNR_HIP(hipGetLastError());
     }
