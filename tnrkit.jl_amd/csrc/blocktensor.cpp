@@ -378,6 +378,7 @@ struct TuneTrial {
     int shape = 0;
 };
 struct TuneEntry {
+    bool cand[GEMM_NSHAPES] = {};      // shapes this key chooses among
     float ms[GEMM_NSHAPES] = {};
     int samples[GEMM_NSHAPES] = {};
     int requested[GEMM_NSHAPES] = {};
@@ -415,11 +416,11 @@ void tune_poll(TuneEntry& t) {
         t.pending.erase(t.pending.begin() + (long)i);
     }
     bool all = true;
-    for (int s = 0; s < GEMM_NSHAPES; ++s) all = all && t.samples[s] >= TUNE_SAMPLES;
+    for (int s = 0; s < GEMM_NSHAPES; ++s) all = all && (!t.cand[s] || t.samples[s] >= TUNE_SAMPLES);
     if (all) {
-        int best = 0;
-        for (int s = 1; s < GEMM_NSHAPES; ++s)
-            if (t.ms[s] < t.ms[best]) best = s;
+        int best = -1;
+        for (int s = 0; s < GEMM_NSHAPES; ++s)
+            if (t.cand[s] && (best < 0 || t.ms[s] < t.ms[best])) best = s;
         t.best = best;
     }
 }
@@ -438,8 +439,13 @@ void gemm_autotune_enable(int on) {
 }
 
 // Shape for a launch and whether this launch is a timing trial.
+// The 32-deep register-staged 64 x 64 tile competes only where the LDS-DMA
+// kernels cannot take the launch: measured alone it is 1.3-1.5x slower than
+// the LDS-DMA tiles on every product of the block iteration and the
+// contractions, and under the timing noise of concurrent lanes the tuner
+// picked it anyway.
 static int tuned_shape(const Ctx& ctx, const std::vector<GemmJob>& jobs, int heuristic,
-                       TuneKey* key_out, bool* trial) {
+                       bool dma_ok, TuneKey* key_out, bool* trial) {
     *trial = false;
     if (!tune_enabled()) return heuristic;
     int mM = 0, mN = 0, mK = 0, cnt = 0, ta = 0, tb = 0;
@@ -453,14 +459,15 @@ static int tuned_shape(const Ctx& ctx, const std::vector<GemmJob>& jobs, int heu
     }
     if (!cnt) return heuristic;
     const TuneKey key{ctx.device, ta, tb, ilog2(mM), ilog2(mN), ilog2(mK), ilog2(cnt),
-                      ilog2(fl), 0};
+                      ilog2(fl), dma_ok ? 1 : 0};
     std::lock_guard<std::mutex> lk(g_tune_mu);
     TuneEntry& t = g_tune[key];
     if (t.best >= 0) return t.best;
+    for (int s = 0; s < GEMM_NSHAPES; ++s) t.cand[s] = !(dma_ok && s == GEMM_64x64_K32);
     tune_poll(t);
     if (t.best >= 0) return t.best;
     for (int s = GEMM_NSHAPES - 1; s >= 0; --s)
-        if (t.requested[s] < TUNE_SAMPLES) {
+        if (t.cand[s] && t.requested[s] < TUNE_SAMPLES) {
             ++t.requested[s];
             *key_out = key;
             *trial = true;
@@ -489,13 +496,26 @@ static void tune_record_end(Ctx& ctx, const GemmPlan& p, TuneTrial& tr) {
     it->second.pending.push_back(tr);
 }
 
+// TNR_GEMM_DMA=0 keeps the 16-deep tile shapes on register-staged loads
+// (the LDS-DMA kernels give the same bits)
+static bool gemm_dma_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("TNR_GEMM_DMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 GemmPlan gemm_plan(Ctx& ctx, const std::vector<GemmJob>& jobs, bool split32, int force_shape,
                    int force_split) {
     GemmPlan P;
     P.split32 = split32;
+    bool dma_ok = gemm_dma_enabled();
+    for (const GemmJob& j : jobs)
+        if (j.M > 0 && j.N > 0) dma_ok = dma_ok && gemm_dma_ok(j.ta, j.lda, j.tb, j.ldb);
     if (force_shape >= 0 && !split32) P.shape = force_shape;
     else if (split32) P.shape = GEMM_64x64;
-    else P.shape = tuned_shape(ctx, jobs, pick_shape(jobs, false), &P.tune_key, &P.trial);
+    else P.shape = tuned_shape(ctx, jobs, pick_shape(jobs, false), dma_ok, &P.tune_key, &P.trial);
     const int BMs = gemm_shape_bm(P.shape), BNs = gemm_shape_bn(P.shape);
     // split K over workgroups only when the launch is too small to fill
     // the chip without it (results are the same either way)
@@ -556,16 +576,6 @@ void gemm_bind(GemmPlan& p, const DescArena& arena) {
     p.d_descs = arena.at<GemmDesc>(p.off_descs);
     p.d_red = arena.at<GemmDesc>(p.off_red);
     p.d_starts = arena.at<int64_t>(p.off_starts);
-}
-
-// TNR_GEMM_DMA=0 keeps the 16-deep tile shapes on register-staged loads
-// (the LDS-DMA kernels give the same bits)
-static bool gemm_dma_enabled() {
-    static const bool on = [] {
-        const char* e = std::getenv("TNR_GEMM_DMA");
-        return !(e && e[0] == '0');
-    }();
-    return on;
 }
 
 void gemm_issue(Ctx& ctx, GemmPlan& p) {

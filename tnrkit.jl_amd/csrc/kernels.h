@@ -210,12 +210,13 @@ inline int gemm_shape_bn(int s) { return s == GEMM_128x128 ? 128 : 64; }
 // every descriptor's per-tile offsets below 2^31 bytes (gemm_dma_ok)
 void launch_gemm(hipStream_t st, const GemmDesc* d_descs, int nd, int64_t tiles, bool split32,
                  double flops, int shape, bool dma);
-inline bool gemm_dma_ok(const GemmDesc& g) {
+inline bool gemm_dma_ok(int ta, int64_t lda, int tb, int64_t ldb) {
     const int64_t lim = (int64_t)1 << 31;
-    const int64_t la = g.ta ? 8 * (16 + 128 * (int64_t)g.lda) : 8 * (128 + 16 * (int64_t)g.lda);
-    const int64_t lb = g.tb ? 8 * (128 + 16 * (int64_t)g.ldb) : 8 * (16 + 128 * (int64_t)g.ldb);
+    const int64_t la = ta ? 8 * (16 + 128 * lda) : 8 * (128 + 16 * lda);
+    const int64_t lb = tb ? 8 * (128 + 16 * ldb) : 8 * (16 + 128 * ldb);
     return la < lim && lb < lim;
 }
+inline bool gemm_dma_ok(const GemmDesc& g) { return gemm_dma_ok(g.ta, g.lda, g.tb, g.ldb); }
 void launch_gemm_reduce(hipStream_t st, const GemmDesc* d_descs, int nd, const int64_t* d_starts,
                         int64_t total);
 void launch_trace_normalize(hipStream_t st, const TraceDesc* d_descs, const TraceProb* d_probs,
