@@ -190,7 +190,7 @@ def decaying(rng, m, n, rate):
 # one-sided Jacobi as the Rayleigh-Ritz step)
 @pytest.mark.parametrize("m,n,k,rate", [
     (4, 4, 4, 0.5), (16, 16, 16, 0.3), (50, 200, 20, 0.1), (200, 50, 20, 0.1),
-    (256, 256, 16, 0.05), (1024, 1024, 32, 0.02), (300, 500, 48, 0.05),
+    (256, 256, 16, 0.05), (1024, 1024, 32, 0.02), (400, 300, 40, 0.05), (300, 500, 48, 0.05),
     (300, 400, 120, 0.02), (1024, 1024, 160, 0.01)])
 def test_svd_truncated(tk, m, n, k, rate):
     rng = np.random.default_rng(m + n + k)
@@ -199,6 +199,27 @@ def test_svd_truncated(tk, m, n, k, rate):
     Ur, sr, Vtr = np.linalg.svd(M, full_matrices=False)
     assert np.max(np.abs(s - sr[:k])) < 1e-13 * sr[0]
     # truncated reconstruction equals LAPACK's (the cut has a gap here)
+    Mk = (U * s) @ V.T
+    Mr = (Ur[:, :k] * sr[:k]) @ Vtr[:k]
+    assert np.max(np.abs(Mk - Mr)) < 1e-12 * sr[0]
+    assert np.max(np.abs(U.T @ U - np.eye(k))) < 1e-12
+    assert np.max(np.abs(V.T @ V - np.eye(k))) < 1e-12
+
+
+# k = 40 iterates an 80-column block on a rank-70 matrix: the block's
+# Cholesky QR (65-96 columns) meets null directions and drops them; the kept
+# triplets still match LAPACK's
+def test_svd_block_wider_than_rank(tk):
+    rng = np.random.default_rng(70)
+    m, n, r, k = 400, 300, 70, 40
+    Ua, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    Va, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    sa = np.exp(-0.05 * np.arange(r))
+    M = (Ua * sa) @ Va.T
+    U, s, V, it = svd(tk, M, k)
+    assert it > 0   # the block iteration, not a direct solve
+    Ur, sr, Vtr = np.linalg.svd(M, full_matrices=False)
+    assert np.max(np.abs(s - sr[:k])) < 1e-13 * sr[0]
     Mk = (U * s) @ V.T
     Mr = (Ur[:, :k] * sr[:k]) @ Vtr[:k]
     assert np.max(np.abs(Mk - Mr)) < 1e-12 * sr[0]
