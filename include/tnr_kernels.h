@@ -100,10 +100,10 @@ int tnr_kernel_jacobi_noalloc(int count, int n, const double* G, double* W,
  * finished skips its tiles), *flops_planned is what the launches were asked
  * to do; copies count bytes read + written. */
 int tnr_profile_enable(int on);
-/* GEMM tile-shape autotuner: on = 1 (default; TNR_AUTOTUNE=0 in the
- * environment starts it off) times each tile shape on the first launches of
- * every launch signature and then keeps the fastest; on = 0 uses the static
- * rule.  Every call forgets what was learned.  Tile shapes never change
+/* GEMM tile-shape autotuner: on = 1 (TNR_AUTOTUNE=1 in the environment
+ * starts it on) times each tile shape on the first launches of every launch
+ * signature and then keeps the fastest; on = 0 (the default) uses the
+ * static rule.  Every call forgets what was learned.  Tile shapes never change
  * results: each output element is summed in the same order under all. */
 int tnr_autotune_enable(int on);
 /* Solver limits and thresholds of the truncated SVD / eigensolver: the
