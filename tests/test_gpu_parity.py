@@ -153,11 +153,11 @@ def test_lanes_do_not_change_results(tk, monkeypatch):
             for i in range(5)]
     reqs.append(tk.Request("HOTRG", tk.classical_ising(0.44, symmetry="Z2"), 8, 4))
     out = {}
-    for lanes in ("1", "2", "3"):
+    for lanes in ("1", "2", "3", "4"):
         monkeypatch.setenv("TNR_LANES", lanes)
         out[lanes] = tk.run_batch(reqs)
-    for a, b, c in zip(out["1"], out["2"], out["3"]):
-        assert np.array_equal(a, b) and np.array_equal(a, c)
+    for a, b, c, d in zip(out["1"], out["2"], out["3"], out["4"]):
+        assert np.array_equal(a, b) and np.array_equal(a, c) and np.array_equal(a, d)
 
 
 def test_device_run_equals_host_loop(tk):

@@ -537,11 +537,12 @@ static double work_estimate(const Req& r) {
 }
 
 static int default_lanes() {
-    // three lanes: the 128-run BTRG chi=32 batch measured 234 / 254 / 266 / 268
-    // free energies per second on 1 / 2 / 3 / 4 lanes; a fourth lane's stream
-    // plus the caller's would exceed the device's four hardware queues
+    // four lanes: on alternating runs of the 128-run BTRG chi=32 batch, 3 / 4
+    // lanes gave 282-284 / 292-295 free energies per second and serving p99
+    // 0.32 / 0.28 s.  With the caller's stream that is five streams on the
+    // device's four hardware queues, so two of them share a queue.
     const char* e = std::getenv("TNR_LANES");
-    int n = e ? std::atoi(e) : 3;
+    int n = e ? std::atoi(e) : 4;
     return std::max(1, std::min(n, 4));
 }
 
