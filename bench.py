@@ -168,8 +168,8 @@ def host_cpu_info():
 
 def host_cpu_share(info, cap=16):
     """Worker processes for the CPU leg: the CPUs this process may use, the
-    cgroup quota, and at most `cap` (a GPU box shares its host: one GPU's
-    share is 16 cores, whatever nproc reports)."""
+    cgroup quota, and at most `cap` (on a host shared by several GPUs, one
+    GPU's share is 16 cores, whatever nproc reports)."""
     n = info["affinity"] or 1
     if info["cgroup_quota_cores"]:
         n = min(n, max(1, int(info["cgroup_quota_cores"])))
