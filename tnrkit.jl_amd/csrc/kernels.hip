@@ -1082,6 +1082,9 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
     int* pos = (int*)(rot + LMAX / 2);           // output order (LMAX ints)
     const int t = threadIdx.x;
     const float inv_np = 1.0f / (float)np;
+    // G is symmetric and only its upper triangle (row <= column) is read
+    // or written after the load: gx(r, c) addresses entry (r, c) there
+    auto gx = [ld](int r, int c) { return r <= c ? r + c * ld : c + r * ld; };
     for (int e = t; e < np * np; e += JT) {
         const int j = (int)(((float)e + 0.5f) * inv_np), i = e - j * np;
         G[i + j * ld] = (i < n && j < n)
@@ -1150,10 +1153,10 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                 if (gi[u] < 0) continue;
                 jacobi_pair(gi[u], rnd, np, bp[u], bq[u]);
                 jacobi_pair(gj[u], rnd, np, cp[u], cq[u]);
-                b00[u] = G[bp[u] + cp[u] * ld];
-                b01[u] = G[bp[u] + cq[u] * ld];
-                b10[u] = G[bq[u] + cp[u] * ld];
-                b11[u] = G[bq[u] + cq[u] * ld];
+                b00[u] = G[gx(bp[u], cp[u])];
+                b01[u] = G[gx(bp[u], cq[u])];
+                b10[u] = G[gx(bq[u], cp[u])];
+                b11[u] = G[gx(bq[u], cq[u])];
             }
             int wa[WT], wb[WT];
             double wp[WT], wq[WT];
@@ -1180,7 +1183,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
             // suffices (a workgroup OR-reduction costs two)
             __syncthreads();
             // G <- J^T G J on the upper pair blocks; the lower ones are their
-            // transposes, so G stays exactly symmetric
+            // transposes, kept only through gx
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int i = gi[u], j = gj[u];
@@ -1192,8 +1195,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                     const double tt = ri.z, gpq = b01[u];
                     G[pi + pi * ld] = b00[u] - tt * gpq;
                     G[qi + qi * ld] = b11[u] + tt * gpq;
-                    G[pi + qi * ld] = 0.0;
-                    G[qi + pi * ld] = 0.0;
+                    G[pi + qi * ld] = 0.0;   // pi < qi
                     continue;
                 }
                 const double ci = ri.x, si = ri.y, cj = rj.x, sj = rj.y;
@@ -1201,10 +1203,10 @@ __global__ __launch_bounds__(JT) void k_jacobi(const JacobiDesc* __restrict__ de
                 const double x10 = si * b00[u] + ci * b10[u], x11 = si * b01[u] + ci * b11[u];
                 const double y00 = cj * x00 - sj * x01, y01 = sj * x00 + cj * x01;   // (.) J_j
                 const double y10 = cj * x10 - sj * x11, y11 = sj * x10 + cj * x11;
-                G[pi + pj * ld] = y00; G[pj + pi * ld] = y00;
-                G[pi + qj * ld] = y01; G[qj + pi * ld] = y01;
-                G[qi + pj * ld] = y10; G[pj + qi * ld] = y10;
-                G[qi + qj * ld] = y11; G[qj + qi * ld] = y11;
+                G[gx(pi, pj)] = y00;
+                G[gx(pi, qj)] = y01;
+                G[gx(qi, pj)] = y10;
+                G[gx(qi, qj)] = y11;
             }
             // W <- W J: this thread's (pair, row) tasks touch distinct columns
 #pragma unroll
