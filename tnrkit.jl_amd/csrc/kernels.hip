@@ -1453,9 +1453,10 @@ __global__ __launch_bounds__(CQ64) __attribute__((amdgpu_waves_per_eu(1, 1))) vo
 // which every thread picks the larger of the two candidates (ties to the
 // lower index) and reads its row.  X = R^{-1} is solved with thread (w,
 // lane) holding column c = lane + 64 (w & 1), rows CQH (w >> 1) to
-// CQH (w >> 1) + CQH - 1: the waves of the lower rows finish them first and
-// hand them to the upper rows through LDS, so every element takes the same
-// updates in the same order as in a one-thread right-looking elimination.
+// CQH (w >> 1) + CQH - 1: the elimination runs from the last row up, so the
+// waves holding rows CQH.. finish them first and hand them through LDS to
+// the waves holding rows 0..CQH-1; every element takes the same updates in
+// the same order as in a one-thread right-looking elimination.
 constexpr int CQH = LMAX / 2;     // columns (factorisation) / rows (inverse) per thread
 constexpr int CQLDM = LMAX + 1;   // padded stride of the LDS matrices
 constexpr int CQ96T = 256;
